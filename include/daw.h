@@ -1,0 +1,149 @@
+/*
+ * daw.h -- C-ABI of the MI355X DAW redatuming library (libdaw.so).
+ *
+ * Every entry point replaces one routine of the reference's Fortran hot path
+ * (paths relative to the reference tree) and keeps its argument meaning:
+ * arrays are the reference's column-major arrays passed as plain pointers
+ * (first index fastest), integer index tables are the (n,5) src_tab /
+ * rec_tab of subacqui.f90:33-58 in Fortran order.  All functions are
+ * synchronous, return 0 on success and a negative DAW_ERR_* code otherwise
+ * (message in daw_last_error()); the reference has no error returns and
+ * aborts instead, which the Fortran shim reproduces with `error stop`.
+ *
+ * Host-pointer entry points copy to and from HBM internally.  The batched
+ * entry point daw_redatum_batch takes device pointers for its large arrays.
+ */
+#ifndef DAW_H
+#define DAW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DAW_ABI_VERSION 1
+
+#define DAW_OK 0
+#define DAW_ERR_ARG (-1)     /* invalid argument */
+#define DAW_ERR_STATE (-2)   /* kwsinc2d not initialised */
+#define DAW_ERR_NOMEM (-3)   /* device allocation failed */
+#define DAW_ERR_HIP (-4)     /* HIP runtime error */
+
+const char *daw_last_error(void);
+int daw_version(void);
+
+/* ---- module kwsinc (kwsinc.f90) ------------------------------------- */
+
+/* kwsinc2d(ifreesurf), kwsinc.f90:212-378 (with allocatekws2d :64-71).
+ * Selects the free-surface flag the Hicks tables are built for; must precede
+ * the propagators, as main_redatum_DAW.f90:92-93 orders it.  Coefficient
+ * tiles are formed per point on demand instead of as the 194 MB table. */
+int daw_kwsinc2d(int ofs);
+/* deallocatekws2d, kwsinc.f90:73-77: drops the tables and cached model. */
+int daw_deallocatekws2d(void);
+/* kwsinc1d monopole table coef_hicks1D_monopole(-10:10,1:100),
+ * kwsinc.f90:94-179; out holds 2100 floats, Fortran order. */
+int daw_hicks1d(float *out);
+/* coef_hicks2D_monopole(:,:,ia1,ia2,ip) as kwsinc2d(ofs) builds it,
+ * kwsinc.f90:262-364; out holds 21x21 floats, first index fastest. */
+int daw_hicks2d_tile(int ia1, int ia2, int ip, int ofs, float *out);
+/* subsourcekws(2, (/z,x/), h, ofs, ix, ialpha, nalpha_frees),
+ * kwsinc.f90:582-606. */
+int daw_subsourcekws(float z, float x, float h, int ofs, int *ix, int *ialpha, int *nfs);
+
+/* ---- acquisition (subacqui.f90:12-62) -------------------------------- */
+
+/* src_tab/rec_tab rows for n points (z[i], x[i]) -> tab(n,5), Fortran order. */
+int daw_subacqui(const float *z, const float *x, int n, float h, int npml, int ofs, int *tab);
+
+/* ---- propagators (submodeling2.f90) ---------------------------------- */
+
+/* submodeling(is0,vp,n1,n2,npml,h,ofs,nt,dt,s,type_src,type_rec,is,ir,nr,sis,wavefield),
+ * submodeling2.f90:11-288.  vp(n1,n2); s(nt) already divided by h**2;
+ * is5 = src_tab(isrc,1:5); ir(nr,5); out sis(nt,nr); wavefield(n1,n2,nt) may
+ * be NULL (the reference always fills it; the batched path never needs it). */
+int daw_submodeling(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt, const float *s,
+                    int type_src, int type_rec, const int *is5, const int *ir, int nr, float *sis, float *wavefield);
+
+/* submodeling_adjoint_esource(is,vp,n1,n2,npml,h,ofs,nt,dt,res,type_rec,nr,ir,sis,wavefieldp),
+ * submodeling2.f90:523-878.  wavefieldp may be NULL. */
+int daw_submodeling_adjoint_esource(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                                    const float *res, int type_rec, int nr, const int *ir, float *sis,
+                                    float *wavefieldp);
+
+/* submodeling_adjoint_esource_data(is,vp,n1,n2,npml,h,ofs,nt,dt,res,type_rec,nr,ir,sis),
+ * submodeling2.f90:880-1217 -- the Hessian-type operator of the MINRES solve. */
+int daw_submodeling_adjoint_esource_data(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt,
+                                         float dt, const float *res, int type_rec, int nr, const int *ir, float *sis);
+
+/* subwavefieldresample(n1,n2,npml,nt,dt,ir,nr,sis,wavefield),
+ * subwavefieldresample.f90:1-41. */
+int daw_subwavefieldresample(int n1, int n2, int npml, int nt, float dt, const int *ir, int nr, float *sis,
+                             const float *wavefield);
+
+/* subresiduals(is,dataobs,datacal,res,nt,nr), subresiduals.f90:6-18. */
+int daw_subresiduals(const float *dobs, const float *dcal, float *res, int nt, int nr);
+
+/* sublinearminres(istot,grid,inv,ir), sublinearminres.f90:7-86, with the
+ * grid%/inv% fields it reads passed explicitly: c0(n1,n2), ir(nrec,5),
+ * deltadr(nt,nrec) in, deltade(nt,nrec) out.  maxiter is REAL as in
+ * grid_type (main_redatum_DAW.f90:14).  hist (may be NULL) receives
+ * qrnorm/beta1 per iteration (the column the MINRES log prints,
+ * minresModule.f90:323); info (may be NULL) receives 9 doubles: fac, Anorm,
+ * Acond, rnorm, Arnorm, ynorm, beta1, istop, itn. */
+int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt, int type_rec,
+                        int nrec, const int *ir, float mu, float maxiter, float error, const float *deltadr,
+                        float *deltade, double *hist, double *info);
+
+/* filmartin2d(deltadeo,nr,nt,dt,flow,fhig,llow,lhig,deltade),
+ * minresModule.f90:486-614. */
+int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
+                    float *dout);
+
+/* ---- batched redatuming (no reference counterpart: it replaces the
+ * para_range shot loop of sub_redatum_DAW.f90:30-155 plus mode 2
+ * :230-238 for a block of shots advanced together on one GPU) ---------- */
+typedef struct daw_redatum_args {
+    const float *c0;          /* host, background model c0(n1,n2) */
+    int n1, n2, npml;
+    float h, dt;
+    int ofs, nt, type_src;
+    const float *s_dev;       /* device, wavelet s(nt), already / h**2 */
+    int nshots;               /* shots redatumed by this call */
+    int nsrc_total;           /* leading dimension of src_tab */
+    const int *src_tab;       /* host, src_tab(nsrc_total,5); rows 1..nshots used */
+    int nrec;
+    const int *rec_tab;       /* host, rec_tab(nrec,5) acquisition receivers */
+    int ndat;
+    const int *dat_tab;       /* host, datum receivers (ndat,5), may be NULL if ndat = 0 */
+    const float *dobs_dev;    /* device, observed data [shot][nrec][nt] */
+    float mu;
+    int maxiter;
+    float rtol;               /* grid%error */
+    float *datum_dev;         /* device out, redatumed gathers [shot][ndat][nt] (unfiltered) */
+    float *datae_dev;         /* device out or NULL: datar + dataesc [shot][nrec][nt] */
+    float *deltade_dev;       /* device out or NULL: MINRES solutions [shot][nrec][nt] */
+    double *hist_host;        /* host out or NULL: qrnorm/beta1 [shot][maxiter] */
+    int max_batch;            /* shots advanced together; <= 0: admit by the HBM budget */
+    double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
+} daw_redatum_args;
+
+int daw_redatum_batch(const daw_redatum_args *args);
+
+/* ---- measurement ------------------------------------------------------ */
+typedef struct daw_stats {
+    double sweep_ms[3];           /* time in forward / back-propagation / re-injection sweeps (timing on) */
+    long long sweep_steps[3];     /* time steps in those sweeps */
+    double batch_ms;              /* device time of daw_redatum_batch calls */
+    long long point_updates;      /* sum over executed sweeps of n1e*n2e per shot per step */
+    long long step_launches;
+    int max_batch_used;
+} daw_stats;
+
+int daw_set_timing(int on);
+int daw_reset_stats(void);
+int daw_get_stats(daw_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

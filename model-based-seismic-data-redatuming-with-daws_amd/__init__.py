@@ -1,0 +1,326 @@
+"""MI355X DAW redatuming: Python host mirror of the reference's operator API.
+
+Each function below stands for the Fortran routine of the same name in the
+reference (paths relative to its tree) and calls the HIP library libdaw.so
+through its C-ABI (include/daw.h).  There is no CPU fallback: importing works
+anywhere, but every compute call raises if libdaw.so is missing or no GPU is
+visible.
+
+Array conventions follow the reference's Fortran arrays (first index fastest)
+seen from numpy in C order:
+    vp(n1,n2)            -> ndarray (n2, n1)  float32
+    sis(nt,nr)           -> ndarray (nr, nt)  float32
+    wavefield(n1,n2,nt)  -> ndarray (nt, n2, n1)
+    src_tab/rec_tab(n,5) -> ndarray (5, n)    int32
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdaw.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+_lib = None
+_hicks_ofs = None
+
+EXPORTS = [
+    "daw_last_error", "daw_version", "daw_kwsinc2d", "daw_deallocatekws2d", "daw_hicks1d", "daw_hicks2d_tile",
+    "daw_subsourcekws", "daw_subacqui", "daw_submodeling", "daw_submodeling_adjoint_esource",
+    "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
+    "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
+    "daw_get_stats",
+]
+
+
+class DawError(RuntimeError):
+    pass
+
+
+class RedatumArgs(C.Structure):
+    _fields_ = [
+        ("c0", C.c_void_p), ("n1", C.c_int), ("n2", C.c_int), ("npml", C.c_int),
+        ("h", C.c_float), ("dt", C.c_float),
+        ("ofs", C.c_int), ("nt", C.c_int), ("type_src", C.c_int),
+        ("s_dev", C.c_void_p), ("nshots", C.c_int), ("nsrc_total", C.c_int), ("src_tab", C.c_void_p),
+        ("nrec", C.c_int), ("rec_tab", C.c_void_p), ("ndat", C.c_int), ("dat_tab", C.c_void_p),
+        ("dobs_dev", C.c_void_p), ("mu", C.c_float), ("maxiter", C.c_int), ("rtol", C.c_float),
+        ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
+        ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("sweep_ms", C.c_double * 3), ("sweep_steps", C.c_longlong * 3), ("batch_ms", C.c_double),
+        ("point_updates", C.c_longlong), ("step_launches", C.c_longlong), ("max_batch_used", C.c_int),
+    ]
+
+
+def lib():
+    """Load libdaw.so (raises DawError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DawError("libdaw.so is not built: run __graft_entry__.build() or "
+                           "python model-based-seismic-data-redatuming-with-daws_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.daw_last_error.restype = C.c_char_p
+        L.daw_hicks1d.argtypes = [_f32p]
+        L.daw_hicks2d_tile.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _f32p]
+        L.daw_subsourcekws.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, _i32p, _i32p, _i32p]
+        L.daw_subacqui.argtypes = [_f32p, _f32p, C.c_int, C.c_float, C.c_int, C.c_int, _i32p]
+        L.daw_submodeling.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
+                                      _f32p, C.c_int, C.c_int, _i32p, _i32p, C.c_int, _f32p, C.c_void_p]
+        L.daw_submodeling_adjoint_esource.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                                      C.c_float, _f32p, C.c_int, C.c_int, _i32p, _f32p, C.c_void_p]
+        L.daw_submodeling_adjoint_esource_data.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                                           C.c_int, C.c_float, _f32p, C.c_int, C.c_int, _i32p, _f32p]
+        L.daw_subwavefieldresample.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, _i32p, C.c_int, _f32p,
+                                               _f32p]
+        L.daw_subresiduals.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int]
+        L.daw_sublinearminres.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
+                                          C.c_int, C.c_int, _i32p, C.c_float, C.c_float, C.c_float, _f32p, _f32p,
+                                          _f64p, _f64p]
+        L.daw_filmartin2d.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, _f32p]
+        L.daw_redatum_batch.argtypes = [C.POINTER(RedatumArgs)]
+        L.daw_get_stats.argtypes = [C.POINTER(Stats)]
+        L.daw_set_timing.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DawError(lib().daw_last_error().decode(errors="replace") or f"daw error {rc}")
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ------------------------------------------------------------------ kwsinc module (kwsinc.f90)
+
+def kwsinc2d(ofs):
+    """kwsinc2d(ifreesurf), kwsinc.f90:212 (after allocatekws2d :64)."""
+    global _hicks_ofs
+    _check(lib().daw_kwsinc2d(int(ofs)))
+    _hicks_ofs = int(ofs)
+
+
+def deallocatekws2d():
+    """deallocatekws2d, kwsinc.f90:73."""
+    global _hicks_ofs
+    _check(lib().daw_deallocatekws2d())
+    _hicks_ofs = None
+
+
+def _ensure_hicks(ofs):
+    if _hicks_ofs != ofs:
+        kwsinc2d(ofs)
+
+
+def hicks1d():
+    """coef_hicks1D_monopole(-10:10,1:100) (kwsinc.f90:94-179) as (100, 21)."""
+    out = np.zeros((100, 21), np.float32)
+    _check(lib().daw_hicks1d(out))
+    return out
+
+
+def hicks2d_tile(ia1, ia2, ip, ofs):
+    """coef_hicks2D_monopole(:,:,ia1,ia2,ip) (kwsinc.f90:262-364) as (21 [n2], 21 [n1])."""
+    out = np.zeros((21, 21), np.float32)
+    _check(lib().daw_hicks2d_tile(int(ia1), int(ia2), int(ip), int(ofs), out))
+    return out
+
+
+def subsourcekws(z, x, h, ofs):
+    """subsourcekws (kwsinc.f90:582-606): (ix(2), ialpha(2), nalpha_frees)."""
+    ix = np.zeros(2, np.int32)
+    ia = np.zeros(2, np.int32)
+    nf = np.zeros(1, np.int32)
+    _check(lib().daw_subsourcekws(float(z), float(x), float(h), int(ofs), ix, ia, nf))
+    return ix, ia, int(nf[0])
+
+
+def subacqui(z, x, h, npml, ofs):
+    """src_tab / rec_tab of subacqui.f90:12-62 for points (z[i], x[i]) -> (5, n) int32."""
+    z = _f32(np.atleast_1d(z))
+    x = _f32(np.atleast_1d(x))
+    tab = np.zeros((5, z.size), np.int32)
+    _check(lib().daw_subacqui(z, x, z.size, float(h), int(npml), int(ofs), tab))
+    return tab
+
+
+# ------------------------------------------------------------------ propagators (submodeling2.f90)
+
+def submodeling(vp, npml, h, ofs, nt, dt, s, src_row, rec_tab, type_src=0, type_rec=0, want_wavefield=False):
+    """submodeling (submodeling2.f90:11-288).  s is the wavelet already
+    divided by h**2 (main_redatum_DAW.f90:111).  Returns sis (nr, nt) and,
+    when asked, wavefield (nt, n2, n1)."""
+    _ensure_hicks(ofs)
+    vp = _f32(vp)
+    n2, n1 = vp.shape
+    rec_tab = _i32(rec_tab)
+    nr = rec_tab.shape[1]
+    sis = np.zeros((max(nr, 1), nt), np.float32)
+    wf = np.zeros((nt, n2, n1), np.float32) if want_wavefield else None
+    _check(lib().daw_submodeling(vp, n1, n2, int(npml), float(h), int(ofs), int(nt), float(dt), _f32(s),
+                                 int(type_src), int(type_rec), _i32(src_row), rec_tab, nr, sis,
+                                 None if wf is None else wf.ctypes.data))
+    sis = sis[:nr]
+    return (sis, wf) if want_wavefield else sis
+
+
+def submodeling_adjoint_esource(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, want_wavefield=False):
+    """submodeling_adjoint_esource (submodeling2.f90:523-878)."""
+    _ensure_hicks(ofs)
+    vp = _f32(vp)
+    n2, n1 = vp.shape
+    rec_tab = _i32(rec_tab)
+    nr = rec_tab.shape[1]
+    sis = np.zeros((nr, nt), np.float32)
+    wf = np.zeros((nt, n2, n1), np.float32) if want_wavefield else None
+    _check(lib().daw_submodeling_adjoint_esource(vp, n1, n2, int(npml), float(h), int(ofs), int(nt), float(dt),
+                                                 _f32(res), int(type_rec), nr, rec_tab, sis,
+                                                 None if wf is None else wf.ctypes.data))
+    return (sis, wf) if want_wavefield else sis
+
+
+def submodeling_adjoint_esource_data(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0):
+    """submodeling_adjoint_esource_data (submodeling2.f90:880-1217)."""
+    _ensure_hicks(ofs)
+    vp = _f32(vp)
+    n2, n1 = vp.shape
+    rec_tab = _i32(rec_tab)
+    nr = rec_tab.shape[1]
+    sis = np.zeros((nr, nt), np.float32)
+    _check(lib().daw_submodeling_adjoint_esource_data(vp, n1, n2, int(npml), float(h), int(ofs), int(nt),
+                                                      float(dt), _f32(res), int(type_rec), nr, rec_tab, sis))
+    return sis
+
+
+def subwavefieldresample(wavefield, npml, rec_tab, ofs, dt=0.0):
+    """subwavefieldresample (subwavefieldresample.f90:1-41): wavefield (nt, n2, n1) -> sis (nr, nt)."""
+    _ensure_hicks(ofs)
+    wf = _f32(wavefield)
+    nt, n2, n1 = wf.shape
+    rec_tab = _i32(rec_tab)
+    nr = rec_tab.shape[1]
+    sis = np.zeros((nr, nt), np.float32)
+    _check(lib().daw_subwavefieldresample(n1, n2, int(npml), nt, float(dt), rec_tab, nr, sis, wf))
+    return sis
+
+
+def subresiduals(dobs, dcal):
+    """subresiduals (subresiduals.f90:6-18): time-reversed dobs - dcal."""
+    dobs = _f32(dobs)
+    dcal = _f32(dcal)
+    nr, nt = dobs.shape
+    res = np.zeros_like(dobs)
+    _check(lib().daw_subresiduals(dobs, dcal, res, nt, nr))
+    return res
+
+
+def sublinearminres(c0, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, maxiter, error=0.0, type_rec=0):
+    """sublinearminres (sublinearminres.f90:7-86): returns (deltade (nr, nt),
+    hist (qrnorm/beta1 per iteration), info dict)."""
+    _ensure_hicks(ofs)
+    c0 = _f32(c0)
+    n2, n1 = c0.shape
+    rec_tab = _i32(rec_tab)
+    nr = rec_tab.shape[1]
+    x = np.zeros((nr, nt), np.float32)
+    hist = np.zeros(max(int(maxiter), 1), np.float64)
+    info = np.zeros(9, np.float64)
+    _check(lib().daw_sublinearminres(c0, n1, n2, int(npml), float(h), int(ofs), int(nt), float(dt), int(type_rec),
+                                     nr, rec_tab, float(mu), float(maxiter), float(error), _f32(deltadr), x, hist,
+                                     info))
+    d = dict(fac=info[0], Anorm=info[1], Acond=info[2], rnorm=info[3], Arnorm=info[4], ynorm=info[5],
+             beta1=info[6], istop=int(info[7]), itn=int(info[8]))
+    return x, hist[:max(d["itn"], 0)], d
+
+
+def filmartin2d(din, dt, flow, fhig, llow, lhig):
+    """filmartin2d (minresModule.f90:486-614) on traces (nr, nt)."""
+    din = _f32(din)
+    nr, nt = din.shape
+    out = np.zeros_like(din)
+    _check(lib().daw_filmartin2d(din, nr, nt, float(dt), float(flow), float(fhig), float(llow), float(lhig), out))
+    return out
+
+
+# ------------------------------------------------------------------ batched redatuming
+
+def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, dobs_dev, mu, maxiter, rtol=0.0,
+                  datum_dev=None, datae_dev=None, deltade_dev=None, want_hist=True, max_batch=0,
+                  hbm_budget_bytes=0.0, type_src=0):
+    """Mode-1 DAW plus the mode-2 datum gather of sub_redatum_DAW.f90:82-155,
+    :238 for all shots of src_tab, advanced together on the GPU.
+
+    s_dev, dobs_dev and the outputs are torch device tensors (float32):
+    dobs_dev [nshots, nrec, nt]; datum_dev [nshots, ndat, nt] receives the
+    redatumed (unfiltered) gathers.  Returns the MINRES histories
+    (nshots, maxiter) when want_hist."""
+    import torch
+    _ensure_hicks(ofs)
+    c0 = _f32(c0)
+    n2, n1 = c0.shape
+    src_tab = _i32(src_tab)
+    rec_tab = _i32(rec_tab)
+    dat_tab = _i32(dat_tab) if dat_tab is not None else None
+    nshots = src_tab.shape[1]
+    nrec = rec_tab.shape[1]
+    ndat = 0 if dat_tab is None else dat_tab.shape[1]
+    for t, shape in ((dobs_dev, (nshots, nrec, nt)), (datum_dev, (nshots, ndat, nt)),
+                     (datae_dev, (nshots, nrec, nt)), (deltade_dev, (nshots, nrec, nt))):
+        if t is not None:
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
+                raise ValueError(f"device tensor must be contiguous float32 cuda of shape {shape}")
+    if s_dev.dtype != torch.float32 or not s_dev.is_cuda or s_dev.numel() != nt:
+        raise ValueError("s_dev must be a float32 cuda tensor of nt samples")
+    hist = np.zeros((nshots, max(int(maxiter), 1)), np.float64) if want_hist else None
+    a = RedatumArgs()
+    a.c0 = c0.ctypes.data
+    a.n1, a.n2, a.npml = n1, n2, int(npml)
+    a.h, a.dt = float(h), float(dt)
+    a.ofs, a.nt, a.type_src = int(ofs), int(nt), int(type_src)
+    a.s_dev = s_dev.data_ptr()
+    a.nshots, a.nsrc_total, a.src_tab = nshots, nshots, src_tab.ctypes.data
+    a.nrec, a.rec_tab = nrec, rec_tab.ctypes.data
+    a.ndat, a.dat_tab = ndat, (dat_tab.ctypes.data if dat_tab is not None else None)
+    a.dobs_dev = dobs_dev.data_ptr()
+    a.mu, a.maxiter, a.rtol = float(mu), int(maxiter), float(rtol)
+    a.datum_dev = datum_dev.data_ptr() if datum_dev is not None else None
+    a.datae_dev = datae_dev.data_ptr() if datae_dev is not None else None
+    a.deltade_dev = deltade_dev.data_ptr() if deltade_dev is not None else None
+    a.hist_host = hist.ctypes.data if hist is not None else None
+    a.max_batch = int(max_batch)
+    a.hbm_budget_bytes = float(hbm_budget_bytes)
+    torch.cuda.synchronize()
+    _check(lib().daw_redatum_batch(C.byref(a)))
+    return hist
+
+
+def set_timing(on):
+    _check(lib().daw_set_timing(1 if on else 0))
+
+
+def reset_stats():
+    _check(lib().daw_reset_stats())
+
+
+def stats():
+    st = Stats()
+    _check(lib().daw_get_stats(C.byref(st)))
+    return dict(sweep_ms=list(st.sweep_ms), sweep_steps=list(st.sweep_steps), batch_ms=st.batch_ms,
+                point_updates=st.point_updates, step_launches=st.step_launches, max_batch_used=st.max_batch_used)

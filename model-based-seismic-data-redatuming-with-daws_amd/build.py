@@ -1,0 +1,58 @@
+"""Build libdaw.so (HIP kernels for gfx950 + C-ABI) in-tree with hipcc.
+
+    python model-based-seismic-data-redatuming-with-daws_amd/build.py
+
+The shared library lands next to this file; include/daw.h declares its
+exports.  -ffp-contract=off keeps every single-precision expression rounded
+as written (the reference's arithmetic); packed FP32 instructions are
+disabled for the device code.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libdaw.so")
+ARCH = os.environ.get("DAW_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["daw_host.cpp", "daw_kernels.hip", "daw_engine.hip"]
+
+
+def hipcc():
+    for c in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.exists(c) or c == "hipcc":
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    flags = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+             f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
+    objs = []
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(bdir, src + ".o")
+        cmd = [hipcc()] + flags
+        if src.endswith(".hip"):
+            cmd += ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+                    "-packed-fp32-ops", "-munsafe-fp-atomics"]
+        cmd += ["-c", path, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

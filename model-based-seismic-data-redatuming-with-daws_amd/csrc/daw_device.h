@@ -1,0 +1,85 @@
+// Device-side data layout shared by the kernels and the batch engine.
+//
+// Layout in HBM (all FP32 unless noted):
+//   * every extended-grid field is row-major with x fastest: f[z*ldx + x],
+//     z in [0,n1e), x in [0,n2e), row pitch ldx = n2e rounded up to 64 floats
+//     (256 B) so each wavefront row load is one aligned 256-byte segment;
+//   * the wave state px,pz,vx,vz is double-buffered (read [cur], write
+//     [cur^1]) so one fused launch per time step never reads a value a
+//     neighbouring workgroup has already overwritten;
+//   * back-propagation snapshots sp,svx,svz are interior-only [it][z][x],
+//     x fastest, n1*n2 per step;
+//   * data traces follow the reference: trace-major, time fastest [r][t].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace daw {
+
+enum SweepMode { kFwd = 0, kBack = 1, kReinject = 2 };
+
+// Owned tile per workgroup and the halo widths the 4th-order staggered
+// stencil needs (two cells each side for the derivative, and one more
+// pressure ring recomputed locally so the velocity half-step needs no
+// second launch).
+constexpr int TX = 64;
+constexpr int TZ = 16;
+constexpr int kBlock = 256;
+constexpr int PW = TX + 3, PH = TZ + 3;       // pressure rect: x in [x0-1, x0+TX+2), z in [z0-1, z0+TZ+2)
+constexpr int VXW = TX + 6, VXH = TZ + 3;     // vx: x in [x0-3, x0+TX+3)
+constexpr int VZW = TX + 3, VZH = TZ + 6;     // vz: z in [z0-3, z0+TZ+3)
+
+struct ShotDev {
+    // grid
+    int n1, n2, npml, n1e, n2e, ldx, nt, ofs;
+    float dt, dtdx;
+    // model (extended grid) and 1-D sponge profiles
+    const float *kappa;                 // [n1e][ldx]
+    const float *d1a, *d1a1, *d1b, *bw1; // [n1e]; bw1 = dtdx*damp1b1
+    const float *d2a, *d2a1, *d2b, *bw2; // [n2e]; bw2 = dtdx*damp2b1
+    // double-buffered wave state
+    float *px[2], *pz[2], *vx[2], *vz[2];
+    // Hicks injection gather into px (source or receiver residuals):
+    // inj_row[z] = q >= 0 for rows touched by a window; entries
+    // inj_ptr[q*(n2e+1)+x] .. [+1] list (receiver, c*dt) in receiver order.
+    const int *inj_row;
+    const int *inj_ptr;
+    const int *inj_r;
+    const float *inj_cdt;
+    const float *inj_amp;               // amplitude of receiver r at step it: inj_amp[r*amp_stride + it]
+    int amp_stride;
+    int vinj;                           // 1: the source is a vz source (type_src = 1), applied in the velocity phase
+    // record band: band_row[z] = slot >= 0 for rows some record window touches
+    const int *band_row;
+    float *pband[2];                    // [nslot][ldx], two time slots
+    // snapshots (kBack writes, kReinject reads)
+    float *sp, *svx, *svz;              // [nt][n1][n2]
+    int xlo1, xhi1, xlo2, xhi2;         // pressure extended-source range, extended coords inclusive
+    // optional full interior wavefield, Fortran (n1,n2,nt)
+    float *wf;
+};
+
+struct RecItem {
+    int shot;
+    int ix1, ix2;       // 0-based extended-grid centre node
+    int clip;           // 0: window clipped to the extended grid, 1: to the interior
+    int reverse;        // write time index nt-1-it instead of it
+    int pad;
+    const float *coef;  // 441 coefficients, (i1+10) + 21*(i2+10)
+    float *out;         // trace base (nt floats)
+};
+
+struct StepParams {
+    const ShotDev *shots;
+    const int2 *tiles;  // (shot, tile) in XCD-aware launch order
+    int ntiles;
+    int it;             // step advanced by the tile blocks
+    int cur;            // state buffers read
+    const RecItem *rec; // record of step rec_it (pband slot rec_slot) by the trailing blocks
+    int nrec;
+    int rec_it;
+    int rec_slot;
+    int nrec_blocks;
+};
+
+}  // namespace daw

@@ -1,0 +1,1032 @@
+// Batch engine and C-ABI of the DAW redatuming library (include/daw.h).
+//
+// The engine advances many shots together: every launch of k_step moves all
+// active shots of a batch one time step, so a C2-size batch of 32 shots is
+// one launch of ~7,000 workgroups instead of 32 launches of 224.  Geometry
+// tables (Hicks coefficients, injection gathers, record bands) are built on
+// the host once per acquisition and shared by every shot that uses them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "daw.h"
+#include "daw_device.h"
+#include "daw_host.h"
+#include "daw_kernels.h"
+
+namespace daw {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(DAW_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ state
+struct Global {
+    std::mutex mu;
+    int hicks_ofs = -1000;     // ofs given to kwsinc2d; -1000: not initialised
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    double t_ms[3] = {0, 0, 0};
+    long long t_cnt[3] = {0, 0, 0};
+    double sweep_ms = 0.0;
+    long long pts = 0;         // point-updates executed (sum n1e*n2e per shot per step)
+    long long launches = 0;
+    int max_batch_used = 0;
+};
+static Global g;
+
+static int ensure_stream()
+{
+    if (!g.stream) HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    return 0;
+}
+
+// RAII device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p; bytes = o.bytes;
+            o.p = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t n)
+    {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = n;
+        if (n == 0) return 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) { p = nullptr; return fail(DAW_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e)); }
+        return 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <class T>
+static int upload(DevBuf &b, const std::vector<T> &v)
+{
+    if (int rc = b.alloc(std::max<size_t>(v.size(), 1) * sizeof(T))) return rc;
+    if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ------------------------------------------------------------------ geometry
+struct Grid {
+    int n1, n2, npml, n1e, n2e, ldx, ofs;
+    float h, dt;
+    size_t plane_e() const { return (size_t)n1e * ldx; }
+    size_t plane_i() const { return (size_t)n1 * n2; }
+};
+
+static Grid make_grid(int n1, int n2, int npml, float h, float dt, int ofs)
+{
+    Grid G;
+    G.n1 = n1; G.n2 = n2; G.npml = npml; G.h = h; G.dt = dt; G.ofs = ofs;
+    G.n1e = n1 + 2 * npml;
+    G.n2e = n2 + 2 * npml;
+    G.ldx = (G.n2e + 63) / 64 * 64;
+    return G;
+}
+
+static int check_grid(int n1, int n2, int npml, int nt, float h, float dt)
+{
+    if (n1 < 1 || n2 < 1 || npml < 3 || nt < 1) return fail(DAW_ERR_ARG, "bad grid n1=%d n2=%d npml=%d nt=%d (npml >= 3)", n1, n2, npml, nt);
+    if (!(h > 0.0f) || !(dt > 0.0f)) return fail(DAW_ERR_ARG, "h and dt must be positive");
+    if ((long long)(n1 + 2 * npml) * (n2 + 2 * npml) > (1LL << 31)) return fail(DAW_ERR_ARG, "grid too large");
+    return 0;
+}
+
+// Model on the device: kappa (x fastest, pitch ldx) and the 1-D profiles.
+struct DevModel {
+    Grid G;
+    DevBuf kappa, prof;   // prof: d1a d1a1 d1b bw1 [n1e each], d2a d2a1 d2b bw2 [n2e each]
+    const float *d1a() const { return prof.as<float>(); }
+    const float *d1a1() const { return prof.as<float>() + G.n1e; }
+    const float *d1b() const { return prof.as<float>() + 2 * G.n1e; }
+    const float *bw1() const { return prof.as<float>() + 3 * G.n1e; }
+    const float *d2a() const { return prof.as<float>() + 4 * G.n1e; }
+    const float *d2a1() const { return prof.as<float>() + 4 * G.n1e + G.n2e; }
+    const float *d2b() const { return prof.as<float>() + 4 * G.n1e + 2 * G.n2e; }
+    const float *bw2() const { return prof.as<float>() + 4 * G.n1e + 3 * G.n2e; }
+};
+
+// vp is the reference's vp(n1,n2) (z fastest) on the host.
+static int make_model(DevModel &M, const Grid &G, const float *vp)
+{
+    M.G = G;
+    std::vector<float> kap((size_t)G.n1e * G.ldx, 0.0f);
+    for (int z = 0; z < G.n1e; ++z) {
+        const int j1 = std::min(std::max(z - G.npml, 0), G.n1 - 1);          // submodext edge replication
+        for (int x = 0; x < G.n2e; ++x) {
+            const int j2 = std::min(std::max(x - G.npml, 0), G.n2 - 1);
+            const double v = (double)vp[(size_t)j1 + (size_t)j2 * G.n1];
+            kap[(size_t)z * G.ldx + x] = (float)(1.0 * v * v);             // kappa = dble(rho)*dble(vp)**2, rho = 1
+        }
+    }
+    std::vector<float> d1a, d1a1, d1b, d1b1, d2a, d2a1, d2b, d2b1;
+    sponge_a(G.npml, G.n1, d1a, d1a1);
+    sponge_b(G.npml, G.n1, d1b, d1b1);
+    sponge_a(G.npml, G.n2, d2a, d2a1);
+    sponge_b(G.npml, G.n2, d2b, d2b1);
+    const float dtdx = G.dt / G.h;
+    std::vector<float> prof;
+    prof.reserve(4 * (G.n1e + G.n2e));
+    prof.insert(prof.end(), d1a.begin(), d1a.end());
+    prof.insert(prof.end(), d1a1.begin(), d1a1.end());
+    prof.insert(prof.end(), d1b.begin(), d1b.end());
+    for (float v : d1b1) prof.push_back(dtdx * 1.0f * v);      // buw1 = dtdx*bu1*damp1b1, bu1 = 1
+    prof.insert(prof.end(), d2a.begin(), d2a.end());
+    prof.insert(prof.end(), d2a1.begin(), d2a1.end());
+    prof.insert(prof.end(), d2b.begin(), d2b.end());
+    for (float v : d2b1) prof.push_back(dtdx * 1.0f * v);      // buw2
+    if (int rc = upload(M.kappa, kap)) return rc;
+    return upload(M.prof, prof);
+}
+
+// Hicks tables of one point set (sources, receivers or datum receivers).
+struct PointSet {
+    int n = 0;
+    std::vector<int> ix1, ix2;      // 0-based extended centre
+    std::vector<float> coef;        // n * 441
+};
+
+static int make_points(PointSet &P, const int *tab, int n, const Grid &G, bool interior_clip)
+{
+    P.n = n;
+    P.ix1.resize(n);
+    P.ix2.resize(n);
+    P.coef.resize((size_t)n * 441);
+    for (int r = 0; r < n; ++r) {
+        const int i1 = tab[r], i2 = tab[r + n], a1 = tab[r + 2 * n], a2 = tab[r + 3 * n], ip = tab[r + 4 * n];
+        if (a1 < 1 || a1 > kNalpha || a2 < 1 || a2 > kNalpha || ip < 0 || ip > kIrad)
+            return fail(DAW_ERR_ARG, "point %d: alpha/free-surface index out of range (%d,%d,%d)", r, a1, a2, ip);
+        const int lo1 = interior_clip ? G.npml + 1 : 1, hi1 = interior_clip ? G.npml + G.n1 : G.n1e;
+        const int lo2 = interior_clip ? G.npml + 1 : 1, hi2 = interior_clip ? G.npml + G.n2 : G.n2e;
+        if (i1 < lo1 - kIrad || i1 > hi1 + kIrad || i2 < lo2 - kIrad || i2 > hi2 + kIrad)
+            return fail(DAW_ERR_ARG, "point %d: grid index (%d,%d) outside the grid", r, i1, i2);
+        P.ix1[r] = i1 - 1;
+        P.ix2[r] = i2 - 1;
+        hicks2d_tile(a1, a2, ip, g.hicks_ofs, &P.coef[(size_t)r * 441]);
+    }
+    return 0;
+}
+
+// Injection gather of a point set, window clipped to the extended grid.
+struct Inject {
+    std::vector<int> row;      // n1e
+    std::vector<int> ptr;      // nq*(n2e+1)
+    std::vector<int> rid;
+    std::vector<float> cdt;
+    DevBuf d_row, d_ptr, d_rid, d_cdt;
+};
+
+static int make_inject(Inject &I, const PointSet &P, const Grid &G)
+{
+    I.row.assign(G.n1e, -1);
+    int nq = 0;
+    for (int r = 0; r < P.n; ++r)
+        for (int i1 = -kIrad; i1 <= kIrad; ++i1) {
+            const int z = P.ix1[r] + i1;
+            if (z >= 0 && z < G.n1e && I.row[z] < 0) I.row[z] = 0;
+        }
+    for (int z = 0; z < G.n1e; ++z)
+        if (I.row[z] >= 0) I.row[z] = nq++;
+    std::vector<int> cnt((size_t)nq * (G.n2e + 1), 0);
+    auto for_window = [&](int r, auto &&fn) {
+        const int c1 = P.ix1[r], c2 = P.ix2[r];
+        const int lo1 = std::max(-c1, -kIrad), hi1 = std::min(G.n1e - 1 - c1, kIrad);
+        const int lo2 = std::max(-c2, -kIrad), hi2 = std::min(G.n2e - 1 - c2, kIrad);
+        for (int i1 = lo1; i1 <= hi1; ++i1)
+            for (int i2 = lo2; i2 <= hi2; ++i2) fn(c1 + i1, c2 + i2, P.coef[(size_t)r * 441 + (i1 + 10) + 21 * (i2 + 10)]);
+    };
+    for (int r = 0; r < P.n; ++r)
+        for_window(r, [&](int z, int x, float) { cnt[(size_t)I.row[z] * (G.n2e + 1) + x]++; });
+    I.ptr.assign((size_t)nq * (G.n2e + 1), 0);
+    int acc = 0;
+    for (int q = 0; q < nq; ++q)
+        for (int x = 0; x <= G.n2e; ++x) {
+            const size_t k = (size_t)q * (G.n2e + 1) + x;
+            I.ptr[k] = acc;
+            if (x < G.n2e) acc += cnt[k];
+        }
+    // per-row tables are [ptr(x), ptr(x+1)) with ptr(n2e) closing the row
+    I.rid.assign(acc, 0);
+    I.cdt.assign(acc, 0.0f);
+    std::vector<int> fillp(I.ptr);
+    for (int r = 0; r < P.n; ++r)   // receivers in order: each point's list follows the reference's loop order
+        for_window(r, [&](int z, int x, float c) {
+            const size_t k = (size_t)I.row[z] * (G.n2e + 1) + x;
+            const int e = fillp[k]++;
+            I.rid[e] = r;
+            I.cdt[e] = c * G.dt;                              // coeff*dt, first product of :175 / :1042
+        });
+    if (int rc = upload(I.d_row, I.row)) return rc;
+    if (int rc = upload(I.d_ptr, I.ptr)) return rc;
+    if (int rc = upload(I.d_rid, I.rid)) return rc;
+    return upload(I.d_cdt, I.cdt);
+}
+
+// Record band: rows that any record window of the given sets touches.
+struct Band {
+    std::vector<int> row;
+    int nslot = 0;
+    DevBuf d_row;
+};
+
+static int make_band(Band &B, const std::vector<const PointSet *> &sets, const Grid &G)
+{
+    B.row.assign(G.n1e, -1);
+    for (const PointSet *P : sets)
+        for (int r = 0; r < P->n; ++r)
+            for (int i1 = -kIrad; i1 <= kIrad; ++i1) {
+                const int z = P->ix1[r] + i1;
+                if (z >= 0 && z < G.n1e) B.row[z] = 0;
+            }
+    B.nslot = 0;
+    for (int z = 0; z < G.n1e; ++z)
+        if (B.row[z] >= 0) B.row[z] = B.nslot++;
+    return upload(B.d_row, B.row);
+}
+
+// XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD one
+// contiguous run of tiles so halo re-reads of neighbouring tiles hit its L2.
+static std::vector<int2> make_tiles(const std::vector<int> &tiles_per_shot)
+{
+    std::vector<int2> flat;
+    for (size_t s = 0; s < tiles_per_shot.size(); ++s)
+        for (int t = 0; t < tiles_per_shot[s]; ++t) flat.push_back(make_int2((int)s, t));
+    const int N = (int)flat.size(), X = 8, q = N / X, rem = N % X;
+    std::vector<int2> out(N);
+    for (int b = 0; b < N; ++b) {
+        const int k = b % X, j = b / X;
+        const int start = k * q + std::min(k, rem);
+        out[b] = flat[start + j];
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ sweeps
+struct SweepSet {
+    int nshots = 0;
+    std::vector<ShotDev> host;
+    DevBuf d_shots, d_tiles;
+    int ntiles = 0;
+    int nt_max = 0;
+    long long pts_per_step = 0;
+};
+
+static int finalize_sweepset(SweepSet &W)
+{
+    std::vector<int> tps(W.nshots);
+    W.nt_max = 0;
+    W.pts_per_step = 0;
+    for (int s = 0; s < W.nshots; ++s) {
+        const ShotDev &S = W.host[s];
+        tps[s] = ((S.n1e + TZ - 1) / TZ) * ((S.n2e + TX - 1) / TX);
+        W.nt_max = std::max(W.nt_max, S.nt);
+        W.pts_per_step += (long long)S.n1e * S.n2e;
+    }
+    std::vector<int2> tiles = make_tiles(tps);
+    W.ntiles = (int)tiles.size();
+    if (int rc = upload(W.d_tiles, tiles)) return rc;
+    return upload(W.d_shots, W.host);
+}
+
+static int refresh_shots(SweepSet &W)
+{
+    HIPCHK(hipMemcpyAsync(W.d_shots.p, W.host.data(), W.host.size() * sizeof(ShotDev), hipMemcpyHostToDevice, g.stream));
+    return 0;
+}
+
+// Runs one sweep of nt_max steps over the batch and the trailing record.
+static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec)
+{
+    // zero the read half of the double-buffered state
+    for (int s = 0; s < W.nshots; ++s) {
+        const ShotDev &S = W.host[s];
+        const size_t pe = (size_t)S.n1e * S.ldx;
+        launch_fill(S.px[0], pe, 0.0f, g.stream);
+        launch_fill(S.pz[0], pe, 0.0f, g.stream);
+        launch_fill(S.vx[0], pe, 0.0f, g.stream);
+        launch_fill(S.vz[0], pe, 0.0f, g.stream);
+    }
+    StepParams P;
+    P.shots = W.d_shots.as<ShotDev>();
+    P.tiles = W.d_tiles.as<int2>();
+    P.rec = d_rec;
+    const int rec_blocks = (nrec + kBlock - 1) / kBlock;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g.timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g.stream));
+    }
+    int cur = 0;
+    for (int it = 0; it <= W.nt_max; ++it) {
+        P.ntiles = (it < W.nt_max) ? W.ntiles : 0;
+        P.it = it;
+        P.cur = cur;
+        const bool rec_prev = it > 0 && nrec > 0;
+        P.nrec = rec_prev ? nrec : 0;
+        P.nrec_blocks = rec_prev ? rec_blocks : 0;
+        P.rec_it = it - 1;
+        P.rec_slot = (it - 1) & 1;
+        launch_step(mode, P, g.stream);
+        cur ^= 1;
+    }
+    HIPCHK(hipGetLastError());
+    if (g.timing) {
+        HIPCHK(hipEventRecord(e1, g.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        g.t_ms[mode] += ms;
+        g.t_cnt[mode] += W.nt_max;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    g.pts += W.pts_per_step * W.nt_max;
+    g.launches += W.nt_max + 1;
+    return 0;
+}
+
+// Per-shot device storage of one batch.
+struct ShotStore {
+    DevBuf fields;   // 8 state arrays per shot
+    DevBuf band;     // 2 slots per shot
+    DevBuf snap;     // sp, svx, svz per shot
+};
+
+static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int nt, bool want_snap)
+{
+    const size_t pe = G.plane_e();
+    if (int rc = st.fields.alloc((size_t)nshots * 8 * pe * sizeof(float))) return rc;
+    if (int rc = st.band.alloc((size_t)nshots * 2 * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
+    if (want_snap)
+        if (int rc = st.snap.alloc((size_t)nshots * 3 * G.plane_i() * nt * sizeof(float))) return rc;
+    return 0;
+}
+
+static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &st, int s, int nt, int nslot,
+                      const Band *band)
+{
+    std::memset(&S, 0, sizeof S);
+    S.n1 = G.n1; S.n2 = G.n2; S.npml = G.npml; S.n1e = G.n1e; S.n2e = G.n2e; S.ldx = G.ldx; S.nt = nt; S.ofs = G.ofs;
+    S.dt = G.dt;
+    S.dtdx = G.dt / G.h;
+    S.kappa = M.kappa.as<float>();
+    S.d1a = M.d1a(); S.d1a1 = M.d1a1(); S.d1b = M.d1b(); S.bw1 = M.bw1();
+    S.d2a = M.d2a(); S.d2a1 = M.d2a1(); S.d2b = M.d2b(); S.bw2 = M.bw2();
+    const size_t pe = G.plane_e();
+    float *f = st.fields.as<float>() + (size_t)s * 8 * pe;
+    S.px[0] = f; S.pz[0] = f + pe; S.vx[0] = f + 2 * pe; S.vz[0] = f + 3 * pe;
+    S.px[1] = f + 4 * pe; S.pz[1] = f + 5 * pe; S.vx[1] = f + 6 * pe; S.vz[1] = f + 7 * pe;
+    if (band) {
+        S.band_row = band->d_row.as<int>();
+        float *b = st.band.as<float>() + (size_t)s * 2 * std::max(nslot, 1) * G.ldx;
+        S.pband[0] = b;
+        S.pband[1] = b + (size_t)std::max(nslot, 1) * G.ldx;
+    }
+    if (st.snap.p) {
+        float *sn = st.snap.as<float>() + (size_t)s * 3 * G.plane_i() * nt;
+        S.sp = sn; S.svx = sn + G.plane_i() * nt; S.svz = sn + 2 * G.plane_i() * nt;
+    }
+    S.xlo1 = G.npml; S.xhi1 = G.npml + G.n1 - 1;
+    S.xlo2 = G.npml; S.xhi2 = G.npml + G.n2 - 1;
+}
+
+static void set_inject(ShotDev &S, const Inject &I, const float *amp, int amp_stride, int vinj)
+{
+    S.inj_row = I.d_row.as<int>();
+    S.inj_ptr = I.d_ptr.as<int>();
+    S.inj_r = I.d_rid.as<int>();
+    S.inj_cdt = I.d_cdt.as<float>();
+    S.inj_amp = amp;
+    S.amp_stride = amp_stride;
+    S.vinj = vinj;
+}
+
+static void add_records(std::vector<RecItem> &v, const PointSet &P, int shot, int clip, int reverse, const DevBuf &coef,
+                        float *out, int nt)
+{
+    for (int r = 0; r < P.n; ++r) {
+        RecItem R;
+        R.shot = shot; R.ix1 = P.ix1[r]; R.ix2 = P.ix2[r]; R.clip = clip; R.reverse = reverse; R.pad = 0;
+        R.coef = coef.as<float>() + (size_t)r * 441;
+        R.out = out + (size_t)r * nt;
+        v.push_back(R);
+    }
+}
+
+// ------------------------------------------------------------------ model cache
+struct ModelKey {
+    std::vector<float> vp;
+    int n1, n2, npml, ofs;
+    float h, dt;
+    bool operator==(const ModelKey &o) const
+    {
+        return n1 == o.n1 && n2 == o.n2 && npml == o.npml && ofs == o.ofs && h == o.h && dt == o.dt && vp == o.vp;
+    }
+};
+static std::unique_ptr<DevModel> g_model;
+static ModelKey g_model_key;
+
+static int get_model(const DevModel *&out, const Grid &G, const float *vp)
+{
+    ModelKey k;
+    k.vp.assign(vp, vp + (size_t)G.n1 * G.n2);
+    k.n1 = G.n1; k.n2 = G.n2; k.npml = G.npml; k.ofs = G.ofs; k.h = G.h; k.dt = G.dt;
+    if (!g_model || !(k == g_model_key)) {
+        auto M = std::make_unique<DevModel>();
+        if (int rc = make_model(*M, G, vp)) return rc;
+        g_model = std::move(M);
+        g_model_key = std::move(k);
+    }
+    out = g_model.get();
+    return 0;
+}
+
+static int check_hicks()
+{
+    if (g.hicks_ofs == -1000) return fail(DAW_ERR_STATE, "daw_kwsinc2d() must be called before the propagators (main_redatum_DAW.f90:92-93)");
+    return 0;
+}
+
+// ------------------------------------------------------------------ Hessian-type two-pass operator
+// Pass 1 (kBack) back-propagates the per-shot amplitudes amp[s*amp_shot + r*nt + t]
+// and stores snapshots; pass 2 (kReinject) re-injects them time reversed
+// and records the acquisition receivers (and optionally datum receivers).
+struct HessOut {
+    float *sis = nullptr;      // per shot acquisition trace block, stride sis_shot
+    size_t sis_shot = 0;
+    int reverse = 0;
+    float *datum = nullptr;    // per shot datum block or null
+    size_t datum_shot = 0;
+    float *wf = nullptr;       // single shot only: full wavefield (Fortran layout)
+    int variant = 0;           // 0: esource_data ranges, 1: esource ranges (2..n-1)
+    int record = 1;            // type_rec == 0
+};
+
+struct Batch {
+    Grid G;
+    const DevModel *M = nullptr;
+    int nshots = 0, nt = 0;
+    PointSet src, rec, dat;
+    Inject inj_rec;
+    std::vector<Inject> inj_src;   // one per shot
+    Band band;
+    DevBuf d_rec_coef, d_dat_coef;
+    ShotStore store;
+    SweepSet W;
+};
+
+static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o)
+{
+    SweepSet &W = B.W;
+    for (int s = 0; s < B.nshots; ++s) {
+        ShotDev &S = W.host[s];
+        set_inject(S, B.inj_rec, amp + (size_t)s * amp_shot, B.nt, 0);
+        S.wf = nullptr;
+    }
+    if (int rc = refresh_shots(W)) return rc;
+    if (int rc = run_sweep(kBack, W, nullptr, 0)) return rc;
+    std::vector<RecItem> items;
+    for (int s = 0; s < B.nshots; ++s) {
+        ShotDev &S = W.host[s];
+        S.inj_row = nullptr;
+        S.wf = o.wf;
+        if (o.variant == 1) { S.xlo1 = B.G.npml + 1; S.xhi1 = B.G.npml + B.G.n1 - 2; S.xlo2 = B.G.npml + 1; S.xhi2 = B.G.npml + B.G.n2 - 2; }
+        else { S.xlo1 = B.G.npml; S.xhi1 = B.G.npml + B.G.n1 - 1; S.xlo2 = B.G.npml; S.xhi2 = B.G.npml + B.G.n2 - 1; }
+        if (o.record && o.sis) add_records(items, B.rec, s, 0, o.reverse, B.d_rec_coef, o.sis + (size_t)s * o.sis_shot, B.nt);
+        if (o.datum) add_records(items, B.dat, s, 1, 0, B.d_dat_coef, o.datum + (size_t)s * o.datum_shot, B.nt);
+    }
+    if (!o.record) for (auto &S : W.host) S.wf = nullptr;
+    DevBuf d_items;
+    if (int rc = upload(d_items, items)) return rc;
+    if (int rc = refresh_shots(W)) return rc;
+    int rc = run_sweep(kReinject, W, d_items.as<RecItem>(), (int)items.size());
+    HIPCHK(hipStreamSynchronize(g.stream));   // d_items lifetime
+    return rc;
+}
+
+// Forward sweep: sources of every shot, record acquisition (+ datum).
+static int forward(Batch &B, const float *s_wavelet, int type_src, int record, float *sis, size_t sis_shot,
+                   float *datum, size_t datum_shot, float *wf)
+{
+    SweepSet &W = B.W;
+    std::vector<RecItem> items;
+    for (int s = 0; s < B.nshots; ++s) {
+        ShotDev &S = W.host[s];
+        set_inject(S, B.inj_src[s], s_wavelet, 0, type_src == 1 ? 1 : 0);
+        if (type_src != 0 && type_src != 1) S.inj_row = nullptr;
+        S.wf = record ? wf : nullptr;
+        if (record && sis) add_records(items, B.rec, s, 0, 0, B.d_rec_coef, sis + (size_t)s * sis_shot, B.nt);
+        if (datum) add_records(items, B.dat, s, 1, 0, B.d_dat_coef, datum + (size_t)s * datum_shot, B.nt);
+    }
+    DevBuf d_items;
+    if (int rc = upload(d_items, items)) return rc;
+    if (int rc = refresh_shots(W)) return rc;
+    int rc = run_sweep(kFwd, W, d_items.as<RecItem>(), (int)items.size());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return rc;
+}
+
+// Builds the per-batch tables and storage.  src_tab: (nsrc_total,5) with
+// rows src_first..src_first+nshots-1 used; rec_tab (nrec,5); dat_tab (ndat,5) or null.
+static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int nshots, const int *src_rows5,
+                       int src_stride, const int *rec_tab, int nrec, const int *dat_tab, int ndat, bool want_snap)
+{
+    B.G = G;
+    B.M = M;
+    B.nshots = nshots;
+    B.nt = nt;
+    if (int rc = make_points(B.rec, rec_tab, nrec, G, false)) return rc;
+    if (dat_tab && ndat > 0) {
+        if (int rc = make_points(B.dat, dat_tab, ndat, G, true)) return rc;
+    }
+    if (int rc = make_inject(B.inj_rec, B.rec, G)) return rc;
+    B.inj_src.clear();
+    B.inj_src.resize(nshots);
+    if (src_rows5) {
+        for (int s = 0; s < nshots; ++s) {
+            int row[5];
+            for (int k = 0; k < 5; ++k) row[k] = src_rows5[s + k * src_stride];
+            PointSet P;
+            if (int rc = make_points(P, row, 1, G, false)) return rc;
+            if (int rc = make_inject(B.inj_src[s], P, G)) return rc;
+        }
+    }
+    std::vector<const PointSet *> sets{&B.rec};
+    if (B.dat.n) sets.push_back(&B.dat);
+    if (int rc = make_band(B.band, sets, G)) return rc;
+    if (int rc = upload(B.d_rec_coef, B.rec.coef)) return rc;
+    if (B.dat.n)
+        if (int rc = upload(B.d_dat_coef, B.dat.coef)) return rc;
+    if (int rc = alloc_store(B.store, nshots, G, B.band.nslot, nt, want_snap)) return rc;
+    B.W.nshots = nshots;
+    B.W.host.resize(nshots);
+    for (int s = 0; s < nshots; ++s) fill_shot(B.W.host[s], G, *M, B.store, s, nt, B.band.nslot, &B.band);
+    return finalize_sweepset(B.W);
+}
+
+// ------------------------------------------------------------------ MINRES over a batch
+struct MinresWork {
+    DevBuf vec;     // x, v, Y[3], Wv[3], hb: 9 vectors per shot
+    DevBuf off, len, part0, part1, scal, hist;
+};
+
+// Solves (H + fac I) x = b per shot as sublinearminres/MINRES do
+// (sublinearminres.f90:64-76, minresModule.f90:83-329).  b and x are
+// device blocks [shot][nrec][nt].
+static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim, double rtol, double *hist_host,
+                        double *scal_host)
+{
+    const int ns = B.nshots;
+    const size_t n = (size_t)B.nt * B.rec.n;
+    const int nb = minres_vec_blocks(), nsc = minres_scal_stride();
+    MinresWork w;
+    if (int rc = w.vec.alloc(8 * n * ns * sizeof(float))) return rc;
+    std::vector<size_t> off(ns);
+    std::vector<int> len(ns, (int)n);
+    for (int s = 0; s < ns; ++s) off[s] = (size_t)s * n;
+    if (int rc = upload(w.off, off)) return rc;
+    if (int rc = upload(w.len, len)) return rc;
+    if (int rc = w.part0.alloc((size_t)ns * nb * sizeof(double))) return rc;
+    if (int rc = w.part1.alloc((size_t)ns * nb * sizeof(double))) return rc;
+    if (int rc = w.scal.alloc((size_t)ns * nsc * sizeof(double))) return rc;
+    if (int rc = w.hist.alloc((size_t)ns * std::max(itnlim, 1) * sizeof(double))) return rc;
+    float *base = w.vec.as<float>();
+    const size_t blk = n * ns;
+    float *v = base, *Y[3] = {base + blk, base + 2 * blk, base + 3 * blk};
+    float *Wv[3] = {base + 4 * blk, base + 5 * blk, base + 6 * blk};
+    float *hb = base + 7 * blk;
+    const size_t *doff = w.off.as<size_t>();
+    const int *dlen = w.len.as<int>();
+    launch_fill(w.hist.as<float>(), (size_t)ns * std::max(itnlim, 1) * 2, 0.0f, g.stream);
+
+    // fac from ||b|| / ||H b|| (sublinearminres.f90:64-72)
+    HessOut o;
+    o.sis = hb; o.sis_shot = n; o.reverse = 0; o.variant = 0;
+    if (int rc = hessian(B, b, n, o)) return rc;
+    launch_dot(b, b, doff, dlen, w.part0.as<double>(), ns, g.stream);
+    launch_dot(hb, hb, doff, dlen, w.part1.as<double>(), ns, g.stream);
+    launch_minres_init(w.part0.as<double>(), w.part1.as<double>(), w.scal.as<double>(), mu, ns, g.stream);
+    // y = r1 = r2 = b; x = w = w2 = 0
+    for (int k = 0; k < 3; ++k) HIPCHK(hipMemcpyAsync(Y[k], b, blk * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
+    launch_fill(x, blk, 0.0f, g.stream);
+    for (int k = 0; k < 3; ++k) launch_fill(Wv[k], blk, 0.0f, g.stream);
+    int iy = 0, ir1 = 1, ir2 = 2;        // roles in Y
+    int iw = 0, iw1 = 1, iw2 = 2;        // roles in Wv
+    std::vector<double> scal((size_t)ns * nsc);
+    for (int itn = 1; itn <= itnlim; ++itn) {
+        HIPCHK(hipMemcpyAsync(scal.data(), w.scal.p, scal.size() * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        bool any = false;
+        for (int s = 0; s < ns; ++s)
+            if (scal[(size_t)s * nsc + kSIstop] == 0.0 && scal[(size_t)s * nsc + kSItn] >= 0.0) any = true;
+        if (!any) break;
+        launch_minres_v(Y[iy], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
+        HessOut h;
+        h.sis = Y[iy]; h.sis_shot = n; h.reverse = 1; h.variant = 0;     // blurring: time-reversed output (:394-396)
+        if (int rc = hessian(B, v, n, h)) return rc;
+        launch_minres_l1(Y[iy], v, Y[ir1], doff, dlen, w.scal.as<double>(), w.part0.as<double>(), ns, g.stream);
+        launch_minres_l2(Y[iy], Y[ir2], doff, dlen, w.scal.as<double>(), w.part1.as<double>(), ns, g.stream);
+        // r1 <- r2, r2 <- y, y <- free
+        const int old_r1 = ir1;
+        ir1 = ir2;
+        ir2 = iy;
+        iy = old_r1;
+        launch_minres_scalars(w.part1.as<double>(), w.scal.as<double>(), w.hist.as<double>(), itnlim, rtol, ns, g.stream);
+        // w1 <- w2, w2 <- w, w <- new (written over old w1)
+        launch_minres_x(x, Wv[iw1], Wv[iw], Wv[iw2], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
+        const int nw = iw1, nw2 = iw, nw1 = iw2;
+        iw = nw; iw2 = nw2; iw1 = nw1;
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (hist_host)
+        HIPCHK(hipMemcpy(hist_host, w.hist.p, (size_t)ns * std::max(itnlim, 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (scal_host) HIPCHK(hipMemcpy(scal_host, w.scal.p, (size_t)ns * nsc * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // namespace daw
+
+// ====================================================================== C-ABI
+using namespace daw;
+
+extern "C" {
+
+const char *daw_last_error(void) { return g_err.c_str(); }
+
+int daw_version(void) { return DAW_ABI_VERSION; }
+
+int daw_kwsinc2d(int ofs)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.hicks_ofs = ofs;
+    return 0;
+}
+
+int daw_deallocatekws2d(void)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.hicks_ofs = -1000;
+    g_model.reset();
+    return 0;
+}
+
+int daw_hicks1d(float *out)
+{
+    if (!out) return fail(DAW_ERR_ARG, "null output");
+    hicks1d(out);
+    return 0;
+}
+
+int daw_hicks2d_tile(int ia1, int ia2, int ip, int ofs, float *out)
+{
+    if (!out || ia1 < 1 || ia1 > kNalpha || ia2 < 1 || ia2 > kNalpha || ip < 0 || ip > kIrad)
+        return fail(DAW_ERR_ARG, "hicks2d_tile: index out of range");
+    hicks2d_tile(ia1, ia2, ip, ofs, out);
+    return 0;
+}
+
+int daw_subsourcekws(float z, float x, float h, int ofs, int *ix, int *ialpha, int *nfs)
+{
+    if (!ix || !ialpha || !nfs || !(h > 0.0f)) return fail(DAW_ERR_ARG, "subsourcekws: bad argument");
+    subsourcekws(z, x, h, ofs, ix, ialpha, nfs);
+    return 0;
+}
+
+int daw_subacqui(const float *z, const float *x, int n, float h, int npml, int ofs, int *tab)
+{
+    if ((n > 0 && (!z || !x || !tab)) || n < 0 || !(h > 0.0f)) return fail(DAW_ERR_ARG, "subacqui: bad argument");
+    for (int i = 0; i < n; ++i) {
+        int row[5];
+        acqui_row(z[i], x[i], h, npml, ofs, row);
+        for (int k = 0; k < 5; ++k) tab[i + k * n] = row[k];
+    }
+    return 0;
+}
+
+int daw_subresiduals(const float *dobs, const float *dcal, float *res, int nt, int nr)
+{
+    if (nt < 0 || nr < 0 || (nt * (size_t)nr > 0 && (!dobs || !dcal || !res))) return fail(DAW_ERR_ARG, "subresiduals: bad argument");
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = ensure_stream()) return rc;
+    const size_t n = (size_t)nt * nr;
+    if (n == 0) return 0;
+    DevBuf a, b, c;
+    if (int rc = a.alloc(n * 4)) return rc;
+    if (int rc = b.alloc(n * 4)) return rc;
+    if (int rc = c.alloc(n * 4)) return rc;
+    HIPCHK(hipMemcpy(a.p, dobs, n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.p, dcal, n * 4, hipMemcpyHostToDevice));
+    launch_residuals(a.as<float>(), b.as<float>(), c.as<float>(), nt, nr, g.stream);
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(res, c.p, n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int daw_submodeling(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt, const float *s,
+                    int type_src, int type_rec, const int *is5, const int *ir, int nr, float *sis, float *wavefield)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_hicks()) return rc;
+    if (int rc = check_grid(n1, n2, npml, nt, h, dt)) return rc;
+    if (!vp || !s || !is5 || (nr > 0 && (!ir || !sis)) || nr < 0) return fail(DAW_ERR_ARG, "submodeling: null argument");
+    if (int rc = ensure_stream()) return rc;
+    const Grid G = make_grid(n1, n2, npml, h, dt, ofs);
+    const DevModel *M;
+    if (int rc = get_model(M, G, vp)) return rc;
+    Batch B;
+    if (int rc = build_batch(B, G, M, nt, 1, is5, 1, ir, nr, nullptr, 0, false)) return rc;
+    DevBuf d_s, d_sis, d_wf;
+    if (int rc = d_s.alloc((size_t)nt * 4)) return rc;
+    HIPCHK(hipMemcpy(d_s.p, s, (size_t)nt * 4, hipMemcpyHostToDevice));
+    if (int rc = d_sis.alloc((size_t)std::max(nr, 1) * nt * 4)) return rc;
+    launch_fill(d_sis.as<float>(), (size_t)std::max(nr, 1) * nt, 0.0f, g.stream);
+    const bool rec = type_rec == 0;
+    if (wavefield && rec) {
+        if (int rc = d_wf.alloc(G.plane_i() * nt * 4)) return rc;
+    }
+    if (int rc = forward(B, d_s.as<float>(), type_src, rec ? 1 : 0, d_sis.as<float>(), (size_t)nt, nullptr, 0,
+                         d_wf.as<float>())) return rc;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (nr > 0) HIPCHK(hipMemcpy(sis, d_sis.p, (size_t)nr * nt * 4, hipMemcpyDeviceToHost));
+    if (wavefield) {
+        if (rec) HIPCHK(hipMemcpy(wavefield, d_wf.p, G.plane_i() * nt * 4, hipMemcpyDeviceToHost));
+        else std::memset(wavefield, 0, G.plane_i() * nt * 4);
+    }
+    return 0;
+}
+
+static int esource_common(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                          const float *res, int type_rec, int nr, const int *ir, float *sis, float *wavefieldp,
+                          int variant)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_hicks()) return rc;
+    if (int rc = check_grid(n1, n2, npml, nt, h, dt)) return rc;
+    if (!vp || nr < 0 || (nr > 0 && (!ir || !res || !sis))) return fail(DAW_ERR_ARG, "esource: null argument");
+    if (int rc = ensure_stream()) return rc;
+    const Grid G = make_grid(n1, n2, npml, h, dt, ofs);
+    const DevModel *M;
+    if (int rc = get_model(M, G, vp)) return rc;
+    Batch B;
+    if (int rc = build_batch(B, G, M, nt, 1, nullptr, 1, ir, nr, nullptr, 0, true)) return rc;
+    DevBuf d_res, d_sis, d_wf;
+    const size_t n = (size_t)std::max(nr, 1) * nt;
+    if (int rc = d_res.alloc(n * 4)) return rc;
+    if (int rc = d_sis.alloc(n * 4)) return rc;
+    launch_fill(d_res.as<float>(), n, 0.0f, g.stream);
+    launch_fill(d_sis.as<float>(), n, 0.0f, g.stream);
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (nr > 0) HIPCHK(hipMemcpy(d_res.p, res, (size_t)nr * nt * 4, hipMemcpyHostToDevice));
+    const bool rec = type_rec == 0;
+    if (wavefieldp && rec)
+        if (int rc = d_wf.alloc(G.plane_i() * nt * 4)) return rc;
+    HessOut o;
+    o.sis = d_sis.as<float>(); o.sis_shot = n; o.reverse = 0; o.variant = variant; o.record = rec ? 1 : 0;
+    o.wf = d_wf.as<float>();
+    if (int rc = hessian(B, d_res.as<float>(), n, o)) return rc;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (nr > 0) HIPCHK(hipMemcpy(sis, d_sis.p, (size_t)nr * nt * 4, hipMemcpyDeviceToHost));
+    if (wavefieldp) {
+        if (rec) HIPCHK(hipMemcpy(wavefieldp, d_wf.p, G.plane_i() * nt * 4, hipMemcpyDeviceToHost));
+        else std::memset(wavefieldp, 0, G.plane_i() * nt * 4);
+    }
+    return 0;
+}
+
+int daw_submodeling_adjoint_esource_data(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                                         const float *res, int type_rec, int nr, const int *ir, float *sis)
+{
+    return esource_common(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, nullptr, 0);
+}
+
+int daw_submodeling_adjoint_esource(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                                    const float *res, int type_rec, int nr, const int *ir, float *sis,
+                                    float *wavefieldp)
+{
+    return esource_common(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, wavefieldp, 1);
+}
+
+int daw_subwavefieldresample(int n1, int n2, int npml, int nt, float dt, const int *ir, int nr, float *sis,
+                             const float *wavefield)
+{
+    (void)dt;
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_hicks()) return rc;
+    if (n1 < 1 || n2 < 1 || nt < 1 || nr < 0 || (nr > 0 && (!ir || !sis || !wavefield)))
+        return fail(DAW_ERR_ARG, "subwavefieldresample: bad argument");
+    if (nr == 0) return 0;
+    if (int rc = ensure_stream()) return rc;
+    const Grid G = make_grid(n1, n2, npml, 1.0f, 1.0f, 0);
+    PointSet P;
+    if (int rc = make_points(P, ir, nr, G, true)) return rc;
+    std::vector<int> c1(nr), c2(nr);
+    for (int r = 0; r < nr; ++r) { c1[r] = P.ix1[r] - npml; c2[r] = P.ix2[r] - npml; }
+    DevBuf d_wf, d_c1, d_c2, d_coef, d_sis;
+    if (int rc = d_wf.alloc(G.plane_i() * nt * 4)) return rc;
+    HIPCHK(hipMemcpy(d_wf.p, wavefield, G.plane_i() * nt * 4, hipMemcpyHostToDevice));
+    if (int rc = upload(d_c1, c1)) return rc;
+    if (int rc = upload(d_c2, c2)) return rc;
+    if (int rc = upload(d_coef, P.coef)) return rc;
+    if (int rc = d_sis.alloc((size_t)nr * nt * 4)) return rc;
+    launch_resample(d_wf.as<float>(), n1, n2, nt, d_c1.as<int>(), d_c2.as<int>(), d_coef.as<float>(), nr,
+                    d_sis.as<float>(), g.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(sis, d_sis.p, (size_t)nr * nt * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt, int type_rec,
+                        int nrec, const int *ir, float mu, float maxiter, float error, const float *deltadr,
+                        float *deltade, double *hist, double *info)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_hicks()) return rc;
+    if (int rc = check_grid(n1, n2, npml, nt, h, dt)) return rc;
+    if (!c0 || nrec < 1 || !ir || !deltadr || !deltade) return fail(DAW_ERR_ARG, "sublinearminres: null argument");
+    if (type_rec != 0) return fail(DAW_ERR_ARG, "sublinearminres: type_rec must be 0 (the data operator records pressure)");
+    if (int rc = ensure_stream()) return rc;
+    const int itnlim = (int)maxiter;   // itnlim = grid%maxiter (REAL -> INTEGER), sublinearminres.f90:48
+    const Grid G = make_grid(n1, n2, npml, h, dt, ofs);
+    const DevModel *M;
+    if (int rc = get_model(M, G, c0)) return rc;
+    Batch B;
+    if (int rc = build_batch(B, G, M, nt, 1, nullptr, 1, ir, nrec, nullptr, 0, true)) return rc;
+    const size_t n = (size_t)nrec * nt;
+    DevBuf d_b, d_x;
+    if (int rc = d_b.alloc(n * 4)) return rc;
+    if (int rc = d_x.alloc(n * 4)) return rc;
+    HIPCHK(hipMemcpy(d_b.p, deltadr, n * 4, hipMemcpyHostToDevice));
+    std::vector<double> scal(minres_scal_stride());
+    std::vector<double> hh(std::max(itnlim, 1));
+    if (int rc = minres_batch(B, d_b.as<float>(), d_x.as<float>(), mu, itnlim, (double)error, hh.data(), scal.data()))
+        return rc;
+    HIPCHK(hipMemcpy(deltade, d_x.p, n * 4, hipMemcpyDeviceToHost));
+    if (hist) std::memcpy(hist, hh.data(), (size_t)std::max(itnlim, 1) * sizeof(double));
+    if (info) {
+        info[0] = scal[kSFac]; info[1] = scal[kSAnorm]; info[2] = scal[kSAcond]; info[3] = scal[kSRnorm];
+        info[4] = scal[kSArnorm]; info[5] = scal[kSYnorm]; info[6] = scal[kSBeta1]; info[7] = scal[kSIstop];
+        info[8] = scal[kSItn];
+    }
+    return 0;
+}
+
+int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
+                    float *dout)
+{
+    if (nr < 0 || nt < 1 || (nr > 0 && (!din || !dout))) return fail(DAW_ERR_ARG, "filmartin2d: bad argument");
+    if (filter_apply_host(din, nr, nt, dt, flow, fhig, llow, lhig, dout) != 0)
+        return fail(DAW_ERR_ARG, "filmartin2d: nt=%d exceeds the 65536-sample FFT length list", nt);
+    return 0;
+}
+
+// ------------------------------------------------------------------ batched redatuming
+int daw_redatum_batch(const daw_redatum_args *a)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (!a) return fail(DAW_ERR_ARG, "null args");
+    if (int rc = check_hicks()) return rc;
+    if (int rc = check_grid(a->n1, a->n2, a->npml, a->nt, a->h, a->dt)) return rc;
+    if (!a->c0 || !a->s_dev || !a->src_tab || !a->rec_tab || a->nrec < 1 || a->nshots < 1 || !a->dobs_dev)
+        return fail(DAW_ERR_ARG, "redatum_batch: missing input");
+    if (a->ndat > 0 && (!a->dat_tab || !a->datum_dev)) return fail(DAW_ERR_ARG, "redatum_batch: datum receivers need dat_tab and datum_dev");
+    if (int rc = ensure_stream()) return rc;
+    const Grid G = make_grid(a->n1, a->n2, a->npml, a->h, a->dt, a->ofs);
+    const DevModel *M;
+    if (int rc = get_model(M, G, a->c0)) return rc;
+    const int nt = a->nt, nrec = a->nrec, ndat = a->ndat;
+    const size_t nvec = (size_t)nrec * nt, ndv = (size_t)ndat * nt;
+    // HBM admission: snapshots dominate (3*n1*n2*nt floats per shot)
+    size_t per_shot = 3 * G.plane_i() * nt * 4 + 8 * G.plane_e() * 4 + 14 * nvec * 4 + 3 * ndv * 4 + (1u << 20);
+    int maxb = a->max_batch;
+    if (maxb <= 0) {
+        size_t freeb = 0, total = 0;
+        HIPCHK(hipMemGetInfo(&freeb, &total));
+        const double budget = a->hbm_budget_bytes > 0 ? (double)a->hbm_budget_bytes : 0.85 * (double)freeb;
+        maxb = (int)std::max(1.0, std::floor(budget / (double)per_shot));
+    }
+    maxb = std::min(maxb, a->nshots);
+    g.max_batch_used = maxb;
+    for (int s0 = 0; s0 < a->nshots; s0 += maxb) {
+        const int ns = std::min(maxb, a->nshots - s0);
+        Batch B;
+        int src_rows[5 * 1];
+        (void)src_rows;
+        std::vector<int> srows((size_t)ns * 5);
+        for (int s = 0; s < ns; ++s)
+            for (int k = 0; k < 5; ++k) srows[s + (size_t)k * ns] = a->src_tab[(s0 + s) + (size_t)k * a->nsrc_total];
+        if (int rc = build_batch(B, G, M, nt, ns, srows.data(), ns, a->rec_tab, nrec, a->dat_tab, ndat, true)) return rc;
+        DevBuf d_datr, d_datum_r, d_res, d_x, d_esc, d_datum_e;
+        if (int rc = d_datr.alloc(ns * nvec * 4)) return rc;
+        if (int rc = d_res.alloc(ns * nvec * 4)) return rc;
+        if (int rc = d_x.alloc(ns * nvec * 4)) return rc;
+        if (int rc = d_esc.alloc(ns * nvec * 4)) return rc;
+        if (ndat > 0) {
+            if (int rc = d_datum_r.alloc(ns * ndv * 4)) return rc;
+            if (int rc = d_datum_e.alloc(ns * ndv * 4)) return rc;
+        }
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g.stream));
+        // 1. background forward in c0: d^r and the datum gather of u^r (sub_redatum_DAW.f90:82-83)
+        if (int rc = forward(B, a->s_dev, a->type_src, 1, d_datr.as<float>(), nvec,
+                             ndat ? d_datum_r.as<float>() : nullptr, ndv, nullptr)) return rc;
+        // 2. reduced residual, time reversed (:103)
+        launch_residuals(a->dobs_dev + (size_t)s0 * nvec, d_datr.as<float>(), d_res.as<float>(), nt, (size_t)ns * nrec, g.stream);
+        // 3. MINRES on (H + fac I) de = dr (:115)
+        if (int rc = minres_batch(B, d_res.as<float>(), d_x.as<float>(), a->mu, a->maxiter, (double)a->rtol,
+                                  a->hist_host ? a->hist_host + (size_t)s0 * std::max(a->maxiter, 1) : nullptr, nullptr))
+            return rc;
+        // 4. source extension wavefield: dataesc and its datum gather (:131-132)
+        HessOut o;
+        o.sis = d_esc.as<float>(); o.sis_shot = nvec; o.reverse = 0; o.variant = 1;
+        o.datum = ndat ? d_datum_e.as<float>() : nullptr; o.datum_shot = ndv;
+        if (int rc = hessian(B, d_x.as<float>(), nvec, o)) return rc;
+        // 5. data-assimilated data and the redatumed gather of u^r + u^e (:143-145, :238)
+        if (a->datae_dev) launch_add(d_datr.as<float>(), d_esc.as<float>(), a->datae_dev + (size_t)s0 * nvec, ns * nvec, g.stream);
+        if (a->deltade_dev)
+            HIPCHK(hipMemcpyAsync(a->deltade_dev + (size_t)s0 * nvec, d_x.p, ns * nvec * 4, hipMemcpyDeviceToDevice, g.stream));
+        if (ndat > 0)
+            launch_add(d_datum_r.as<float>(), d_datum_e.as<float>(), a->datum_dev + (size_t)s0 * ndv, ns * ndv, g.stream);
+        HIPCHK(hipEventRecord(e1, g.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        g.sweep_ms += ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int daw_set_timing(int on)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.timing = on != 0;
+    return 0;
+}
+
+int daw_reset_stats(void)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    for (int i = 0; i < 3; ++i) { g.t_ms[i] = 0; g.t_cnt[i] = 0; }
+    g.sweep_ms = 0;
+    g.pts = 0;
+    g.launches = 0;
+    return 0;
+}
+
+int daw_get_stats(daw_stats *st)
+{
+    if (!st) return fail(DAW_ERR_ARG, "null stats");
+    std::lock_guard<std::mutex> lk(g.mu);
+    for (int i = 0; i < 3; ++i) { st->sweep_ms[i] = g.t_ms[i]; st->sweep_steps[i] = g.t_cnt[i]; }
+    st->batch_ms = g.sweep_ms;
+    st->point_updates = g.pts;
+    st->step_launches = g.launches;
+    st->max_batch_used = g.max_batch_used;
+    return 0;
+}
+
+}  // extern "C"

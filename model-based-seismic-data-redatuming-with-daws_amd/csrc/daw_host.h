@@ -1,0 +1,51 @@
+// Host-side geometry and setup for the DAW redatuming library.
+//
+// Everything here runs once per model / acquisition on the host in IEEE
+// single precision with no FP contraction, so its results are bit-identical
+// to the reference's Fortran (which is compiled for baseline x86-64, no FMA).
+// The device only ever sees the finished tables.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace daw {
+
+constexpr int kIrad = 10;          // Hicks half-width, kwsinc.f90:25
+constexpr int kWin = 2 * kIrad + 1; // 21-point support
+constexpr int kNalpha = 100;       // kwsinc.f90:26
+
+// Modified Bessel I0 in the reference's mixed precision (kwsinc.f90:612-635).
+float bessi0(float x);
+
+// 1-D Kaiser-windowed-sinc monopole table (kwsinc.f90:94-179);
+// out[(n+10) + 21*(ia-1)], n = -10..10, ia = 1..100.
+void hicks1d(float *out);
+
+// One 21x21 tile of the 2-D monopole table for (ialpha1, ialpha2, ip) as
+// kwsinc2d(ofs) leaves it (kwsinc.f90:212-378); out[(n1+10) + 21*(n2+10)].
+void hicks2d_tile(int ia1, int ia2, int ip, int ofs, float *out);
+
+// Grid index / sub-cell shift / nodes-above-free-surface for a point
+// (kwsinc.f90:582-606).
+void subsourcekws(float z, float x, float h, int ofs, int ix[2], int ialpha[2], int *nfs);
+
+// One row of the src_tab/rec_tab contract of subacqui.f90:33-58.
+void acqui_row(float z, float x, float h, int npml, int ofs, int row[5]);
+
+// Split-sponge profiles of cspongea/cspongeb (submodeling2.f90:1331-1393),
+// 0-based arrays of length n + 2*npml.
+void sponge_a(int npml, int n, std::vector<float> &damp, std::vector<float> &damp1);
+void sponge_b(int npml, int n, std::vector<float> &damp, std::vector<float> &damp1);
+
+// Band-pass weights of filmartin2d (minresModule.f90:486-586): returns the
+// FFT length nto (power of two >= nt from the reference's list, -1 if nt is
+// too large) and fills w[0..nto] with dmartin(1..nto+1).
+int filter_design(int nt, float dt, float flow, float fhig, float llow, float lhig, std::vector<double> &w);
+
+// Apply the band-pass to nr traces of length nt (trace-major, t fastest) on
+// the host with an FP64 real FFT.  Same transfer function as filmartin2d,
+// including its Nyquist bin being scaled by the DC weight.
+int filter_apply_host(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow,
+                      float lhig, float *dout);
+
+}  // namespace daw

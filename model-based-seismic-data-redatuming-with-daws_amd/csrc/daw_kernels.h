@@ -1,0 +1,43 @@
+// Launch wrappers of the gfx950 kernels (daw_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include "daw_device.h"
+
+namespace daw {
+
+void launch_step(int mode, const StepParams &P, hipStream_t st);
+void launch_fill(float *p, size_t n, float v, hipStream_t st);
+void launch_residuals(const float *dobs, const float *dcal, float *res, int nt, size_t ntr, hipStream_t st);
+void launch_add(const float *a, const float *b, float *c, size_t n, hipStream_t st);
+void launch_resample(const float *wf, int n1, int n2, int nt, const int *ix1, const int *ix2, const float *coef,
+                     int nr, float *sis, hipStream_t st);
+
+// MINRES over a batch: vectors of shot s start at off[s] and hold len[s]
+// floats; part holds minres_vec_blocks() FP64 partial sums per shot; scal
+// holds minres_scal_stride() FP64 scalars per shot.
+int minres_scal_stride();
+int minres_vec_blocks();
+void launch_dot(const float *a, const float *b, const size_t *off, const int *len, double *part, int nshots,
+                hipStream_t st);
+void launch_minres_init(const double *part0, const double *part1, double *scal, float mu, int nshots,
+                        hipStream_t st);
+void launch_minres_v(const float *y, float *v, const size_t *off, const int *len, const double *scal, int nshots,
+                     hipStream_t st);
+void launch_minres_l1(float *y, const float *v, const float *r1, const size_t *off, const int *len,
+                      const double *scal, double *part, int nshots, hipStream_t st);
+void launch_minres_l2(float *y, const float *r2, const size_t *off, const int *len, const double *scal,
+                      double *part, int nshots, hipStream_t st);
+void launch_minres_scalars(const double *part, double *scal, double *hist, int itnlim, double rtol, int nshots,
+                           hipStream_t st);
+void launch_minres_x(float *x, float *wn, const float *w2, const float *w1, const float *v, const size_t *off,
+                     const int *len, double *scal, int nshots, hipStream_t st);
+
+// indices into a shot's scalar row (see k_minres_scalars)
+enum MinresScalIndex {
+    kSBeta = 0, kSOldb, kSBeta1, kSDbar, kSEpsln, kSQrnorm, kSPhibar, kSRhs1, kSRhs2, kSTnorm2, kSYnorm2, kSCs,
+    kSSn, kSGmax, kSGmin, kSAnorm, kSYnorm, kSRnorm, kSAcond, kSAlfa, kSShift, kSOldeps, kSDelta, kSPhi,
+    kSDenom, kSIstop, kSItn, kSArnorm, kSFac
+};
+
+}  // namespace daw
