@@ -1,0 +1,227 @@
+"""Benchmark: batched DAW redatuming on one MI355X.
+
+Workload (default): 201 x 801 grid, h = 8 m, 20-cell PML, nt = 3000,
+dt = 0.8 ms, 10 Hz Ricker, 64 shots from x = 400 m at 50 m spacing, a fixed
+401-receiver spread, 30 MINRES iterations, datum at z = 400 m -- the
+reference's redatuming workflow (sub_redatum_DAW.f90 modes 1 and 2) for all
+64 shots.  Observed data are modelled beforehand in the true model with 1 %
+seeded noise (untimed input synthesis, the reference's mode 0).
+
+One step = one redatuming pass over all shots: background forward, residual,
+fac, 30 MINRES iterations (each one back-propagation + re-injection sweep
+pair), the source-extension sweep pair and the datum gathers, with inputs
+resident in HBM.  The line also reports the plain one-shot-at-a-time GPU path,
+the roofline of the dominant step kernel and the reference's own CPU path
+timed on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def _pkg():
+    import __graft_entry__ as ge
+    daw = ge._load_pkg()
+    import importlib
+    return daw, importlib.import_module("daw_amd.synth")
+
+
+def _log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def setup(daw, syn, cfg, shots=None, nt=None, maxiter=None):
+    import torch
+    w = syn.workload(cfg)
+    if nt:
+        w.nt = nt
+    if maxiter is not None:
+        w.maxiter = maxiter
+    vp, c0 = syn.models(w)
+    h, dt = np.float32(w.h), np.float32(w.dt)
+    nsh = w.nshots if shots is None else shots
+    src = daw.subacqui(w.src_z[:nsh], w.src_x[:nsh], h, w.npml, w.ofs)
+    rec = daw.subacqui(w.rec_z, w.rec_x, h, w.npml, w.ofs)
+    dat = daw.subacqui(w.dat_z, w.dat_x, h, w.npml, w.ofs)
+    s = syn.ricker(w.nt, float(dt), w.f0, float(h))
+    s_dev = torch.from_numpy(s).cuda()
+    dobs = torch.zeros((nsh, rec.shape[1], w.nt), dtype=torch.float32, device="cuda")
+    dnew = torch.zeros((nsh, dat.shape[1], w.nt), dtype=torch.float32, device="cuda")
+    daw.modeling_batch(vp, w.npml, h, w.ofs, w.nt, dt, s_dev, src, rec, dobs, dat_tab=dat, datum_dev=dnew)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2000 + cfg)
+    rms = float(torch.sqrt(torch.mean(dobs.double() ** 2)))
+    dobs += 0.01 * rms * torch.randn(dobs.shape, generator=g, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    return dict(w=w, vp=vp, c0=c0, h=h, dt=dt, src=src, rec=rec, dat=dat, s=s, s_dev=s_dev, dobs=dobs, dnew=dnew,
+                nsh=nsh)
+
+
+def run_redatum(daw, P, shots=None, max_batch=0, hbm_budget=0.0):
+    import torch
+    w = P["w"]
+    nsh = P["nsh"] if shots is None else shots
+    datum = torch.zeros((nsh, P["dat"].shape[1], w.nt), dtype=torch.float32, device="cuda")
+    hist = daw.redatum_batch(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], P["s_dev"], P["src"][:, :nsh],
+                             P["rec"], P["dat"], P["dobs"][:nsh].contiguous(), w.mu, w.maxiter, 0.0,
+                             datum_dev=datum, max_batch=max_batch, hbm_budget_bytes=hbm_budget)
+    return datum, hist
+
+
+def cpu_baseline(P, nt_sample):
+    """The reference's own CPU path (oracle/_ref, built from its sources) on
+    one shot: one submodeling_adjoint_esource_data call (back-propagation +
+    re-injection, the operator MINRES applies) truncated to nt_sample steps,
+    scaled to a redatumed shot of 7 + 2*itn sweeps at the full nt."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    w = P["w"]
+    rng = np.random.default_rng(0)
+    res = rng.standard_normal((P["rec"].shape[1], nt_sample)).astype(np.float32)
+    kind = "reference" if O.ref_available() else "port"
+    if kind == "reference":
+        O.ref_init(w.ofs)
+        fn = lambda: O.ref_esource_data(P["c0"], w.npml, P["h"], w.ofs, nt_sample, P["dt"], res, P["rec"])
+    else:
+        fn = lambda: O.esource_data(P["c0"], w.npml, P["h"], w.ofs, nt_sample, P["dt"], res, P["rec"])
+    t0 = time.perf_counter()
+    fn()
+    t = time.perf_counter() - t0
+    sweeps = 7 + 2 * w.maxiter
+    t_shot = t / (2.0 * nt_sample) * sweeps * w.nt
+    n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml
+    return dict(value=3600.0 / t_shot, unit="shots/hour", cores=1, kind=kind,
+                sample=f"one submodeling_adjoint_esource_data call (2 sweeps) of one shot at nt={nt_sample} "
+                       f"took {t:.2f} s on 1 core; scaled x{sweeps}/2 sweeps x{w.nt}/{nt_sample} steps "
+                       f"= {t_shot:.0f} s per redatumed shot",
+                gpts_per_s=n1e * n2e * 2 * nt_sample / t / 1e9)
+
+
+def calibrate_copy(nbytes=1 << 30):
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--shots", type=int, default=None)
+    ap.add_argument("--nt", type=int, default=None)
+    ap.add_argument("--maxiter", type=int, default=None)
+    ap.add_argument("--plain-shots", type=int, default=2)
+    ap.add_argument("--cpu-nt", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--hbm-budget-gb", type=float, default=0.0)
+    ap.add_argument("--max-batch", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); shots are independent and would "
+              "shard across GPUs without any collective, which this build does not exercise.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    daw, syn = _pkg()
+    _log("setup: modelling observed data")
+    P = setup(daw, syn, args.config, shots=args.shots, nt=args.nt, maxiter=args.maxiter)
+    w = P["w"]
+    budget = args.hbm_budget_gb * 1e9
+    # first-use warm-up of the library (kernel loading, stream creation): one short shot
+    Q = setup(daw, syn, args.config, shots=1, nt=64, maxiter=1)
+    run_redatum(daw, Q)
+    for i in range(args.warmup):
+        _log(f"warmup step {i + 1}/{args.warmup}")
+        run_redatum(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+    daw.reset_stats()
+    daw.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _log(f"timed step {i + 1}/{args.steps}: {P['nsh']} shots")
+        datum, hist = run_redatum(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    daw.set_timing(False)
+    st = daw.stats()
+    nsh = P["nsh"]
+    shots_per_hour = nsh * args.steps / elapsed * 3600.0
+    gpts = st["point_updates"] / elapsed / 1e9
+    n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml
+    mb = st["max_batch_used"]
+    # dominant kernel: back-propagation or re-injection step (one launch = one
+    # time step for all shots of a sub-batch)
+    modes = {1: "k_step<1> (back-propagation)", 2: "k_step<2> (re-injection)"}
+    dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
+    avg_ms = st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1)
+    bytes_per_shot_step = 36.0 * n1e * n2e + 12.0 * w.nz * w.nx
+    shots_per_launch = nsh / int(np.ceil(nsh / mb))
+    alg_bytes = bytes_per_shot_step * shots_per_launch
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    # quality: redatumed (unfiltered) gathers vs data modelled at the datum in the true model
+    err = float(torch.linalg.norm(datum - P["dnew"]) / torch.linalg.norm(P["dnew"]))
+    # plain path: one shot at a time
+    plain = None
+    if args.plain_shots > 0:
+        daw.reset_stats()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _log(f"plain path: {args.plain_shots} shots one at a time")
+        dp, _ = run_redatum(daw, P, shots=args.plain_shots, max_batch=1)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        plain = args.plain_shots / tp * 3600.0
+        same = bool(torch.equal(dp, datum[:args.plain_shots]))
+    cpu = None
+    if not args.no_cpu:
+        _log(f"cpu baseline sample nt={args.cpu_nt}")
+        cpu = cpu_baseline(P, args.cpu_nt)
+    out = {
+        "metric": "shots redatumed/hour + stencil GPts/s on 1 MI355X, batched vs plain at equal error",
+        "value": shots_per_hour,
+        "unit": "shots/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded layered model (sigma 3 smoothing), background sigma 10, Ricker, 1% seeded noise",
+        "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m", "shots": nsh, "nt": w.nt,
+                   "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
+                   "shots_per_launch": shots_per_launch, "snapshots": "fp32"},
+        "gpts_per_s": gpts,
+        "plain_shots_per_hour": plain,
+        "batched_over_plain": (shots_per_hour / plain) if plain else None,
+        "batched_equals_plain": same if plain else None,
+        "redatum_rel_l2_vs_true_datum_data": err,
+        "minres_final_rel_residual_mean": float(np.mean(hist[:, -1])) if hist is not None else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None, "kernel": modes[dom],
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,30 @@ typedef struct daw_redatum_args {
 
 int daw_redatum_batch(const daw_redatum_args *args);
 
+/* Mode-0 forward modelling of sub_redatum_DAW.f90:55-65 (submodeling per
+ * shot, submodeling2.f90:11-288) for a block of shots advanced together:
+ * dobs_dev [shot][nrec][nt]; with datum receivers also the datum gather of
+ * each shot's wavefield (what mode 2 resamples, subwavefieldresample.f90)
+ * without storing the wavefield.  Device pointers for s/dobs/datum. */
+typedef struct daw_modeling_args {
+    const float *vp;          /* host, vp(n1,n2) */
+    int n1, n2, npml;
+    float h, dt;
+    int ofs, nt, type_src, type_rec;
+    const float *s_dev;
+    int nshots, nsrc_total;
+    const int *src_tab;
+    int nrec;
+    const int *rec_tab;
+    int ndat;
+    const int *dat_tab;
+    float *dobs_dev;          /* device out [shot][nrec][nt] */
+    float *datum_dev;         /* device out [shot][ndat][nt] or NULL */
+    int max_batch;            /* <= 0: all shots at once (up to 256) */
+} daw_modeling_args;
+
+int daw_modeling_batch(const daw_modeling_args *args);
+
 /* ---- measurement ------------------------------------------------------ */
 typedef struct daw_stats {
     double sweep_ms[3];           /* time in forward / back-propagation / re-injection sweeps (timing on) */

@@ -33,7 +33,7 @@ EXPORTS = [
     "daw_subsourcekws", "daw_subacqui", "daw_submodeling", "daw_submodeling_adjoint_esource",
     "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
-    "daw_get_stats",
+    "daw_get_stats", "daw_modeling_batch",
 ]
 
 
@@ -51,6 +51,17 @@ class RedatumArgs(C.Structure):
         ("dobs_dev", C.c_void_p), ("mu", C.c_float), ("maxiter", C.c_int), ("rtol", C.c_float),
         ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
         ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
+    ]
+
+
+class ModelingArgs(C.Structure):
+    _fields_ = [
+        ("vp", C.c_void_p), ("n1", C.c_int), ("n2", C.c_int), ("npml", C.c_int),
+        ("h", C.c_float), ("dt", C.c_float),
+        ("ofs", C.c_int), ("nt", C.c_int), ("type_src", C.c_int), ("type_rec", C.c_int),
+        ("s_dev", C.c_void_p), ("nshots", C.c_int), ("nsrc_total", C.c_int), ("src_tab", C.c_void_p),
+        ("nrec", C.c_int), ("rec_tab", C.c_void_p), ("ndat", C.c_int), ("dat_tab", C.c_void_p),
+        ("dobs_dev", C.c_void_p), ("datum_dev", C.c_void_p), ("max_batch", C.c_int),
     ]
 
 
@@ -90,6 +101,7 @@ def lib():
                                       C.c_float, _f32p]
         L.daw_redatum_batch.argtypes = [C.POINTER(RedatumArgs)]
         L.daw_get_stats.argtypes = [C.POINTER(Stats)]
+        L.daw_modeling_batch.argtypes = [C.POINTER(ModelingArgs)]
         L.daw_set_timing.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -309,6 +321,41 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
     torch.cuda.synchronize()
     _check(lib().daw_redatum_batch(C.byref(a)))
     return hist
+
+
+def modeling_batch(vp, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dobs_dev, dat_tab=None, datum_dev=None,
+                   max_batch=0, type_src=0, type_rec=0):
+    """Mode-0 forward modelling (sub_redatum_DAW.f90:55-65) of every shot in
+    src_tab at once: dobs_dev [nshots, nrec, nt] and, with datum receivers,
+    datum_dev [nshots, ndat, nt] (torch float32 device tensors)."""
+    import torch
+    _ensure_hicks(ofs)
+    vp = _f32(vp)
+    n2, n1 = vp.shape
+    src_tab = _i32(src_tab)
+    rec_tab = _i32(rec_tab)
+    dat_tab = _i32(dat_tab) if dat_tab is not None else None
+    nshots = src_tab.shape[1]
+    nrec = rec_tab.shape[1]
+    ndat = 0 if dat_tab is None else dat_tab.shape[1]
+    for t, shape in ((dobs_dev, (nshots, nrec, nt)), (datum_dev, (nshots, ndat, nt))):
+        if t is not None and (t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous()
+                              or tuple(t.shape) != shape):
+            raise ValueError(f"device tensor must be contiguous float32 cuda of shape {shape}")
+    a = ModelingArgs()
+    a.vp = vp.ctypes.data
+    a.n1, a.n2, a.npml = n1, n2, int(npml)
+    a.h, a.dt = float(h), float(dt)
+    a.ofs, a.nt, a.type_src, a.type_rec = int(ofs), int(nt), int(type_src), int(type_rec)
+    a.s_dev = s_dev.data_ptr()
+    a.nshots, a.nsrc_total, a.src_tab = nshots, nshots, src_tab.ctypes.data
+    a.nrec, a.rec_tab = nrec, rec_tab.ctypes.data
+    a.ndat, a.dat_tab = ndat, (dat_tab.ctypes.data if dat_tab is not None else None)
+    a.dobs_dev = dobs_dev.data_ptr()
+    a.datum_dev = datum_dev.data_ptr() if datum_dev is not None else None
+    a.max_batch = int(max_batch)
+    torch.cuda.synchronize()
+    _check(lib().daw_modeling_batch(C.byref(a)))
 
 
 def set_timing(on):

@@ -659,7 +659,9 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
         for (int s = 0; s < ns; ++s)
             if (scal[(size_t)s * nsc + kSIstop] == 0.0 && scal[(size_t)s * nsc + kSItn] >= 0.0) any = true;
         if (!any) break;
-        launch_minres_v(Y[iy], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
+        // y of the previous iteration lives in the r2 buffer (r2 = y, :193); the
+        // free buffer Y[iy] receives the new operator output
+        launch_minres_v(Y[ir2], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
         HessOut h;
         h.sis = Y[iy]; h.sis_shot = n; h.reverse = 1; h.variant = 0;     // blurring: time-reversed output (:394-396)
         if (int rc = hessian(B, v, n, h)) return rc;
@@ -945,6 +947,10 @@ int daw_redatum_batch(const daw_redatum_args *a)
         maxb = (int)std::max(1.0, std::floor(budget / (double)per_shot));
     }
     maxb = std::min(maxb, a->nshots);
+    {   // equal-sized sub-batches
+        const int nb = (a->nshots + maxb - 1) / maxb;
+        maxb = (a->nshots + nb - 1) / nb;
+    }
     g.max_batch_used = maxb;
     for (int s0 = 0; s0 < a->nshots; s0 += maxb) {
         const int ns = std::min(maxb, a->nshots - s0);
@@ -995,6 +1001,39 @@ int daw_redatum_batch(const daw_redatum_args *a)
         g.sweep_ms += ms;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int daw_modeling_batch(const daw_modeling_args *a)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (!a) return fail(DAW_ERR_ARG, "null args");
+    if (int rc = check_hicks()) return rc;
+    if (int rc = check_grid(a->n1, a->n2, a->npml, a->nt, a->h, a->dt)) return rc;
+    if (!a->vp || !a->s_dev || !a->src_tab || a->nshots < 1 || a->nrec < 0 || (a->nrec > 0 && (!a->rec_tab || !a->dobs_dev)))
+        return fail(DAW_ERR_ARG, "modeling_batch: missing input");
+    if (a->ndat > 0 && (!a->dat_tab || !a->datum_dev)) return fail(DAW_ERR_ARG, "modeling_batch: datum receivers need dat_tab and datum_dev");
+    if (int rc = ensure_stream()) return rc;
+    const Grid G = make_grid(a->n1, a->n2, a->npml, a->h, a->dt, a->ofs);
+    const DevModel *M;
+    if (int rc = get_model(M, G, a->vp)) return rc;
+    const int nt = a->nt;
+    const size_t nvec = (size_t)a->nrec * nt, ndv = (size_t)a->ndat * nt;
+    const int maxb = std::max(1, std::min(a->max_batch > 0 ? a->max_batch : 256, a->nshots));
+    for (int s0 = 0; s0 < a->nshots; s0 += maxb) {
+        const int ns = std::min(maxb, a->nshots - s0);
+        std::vector<int> srows((size_t)ns * 5);
+        for (int s = 0; s < ns; ++s)
+            for (int k = 0; k < 5; ++k) srows[s + (size_t)k * ns] = a->src_tab[(s0 + s) + (size_t)k * a->nsrc_total];
+        Batch B;
+        if (int rc = build_batch(B, G, M, nt, ns, srows.data(), ns, a->rec_tab, a->nrec, a->dat_tab, a->ndat, false))
+            return rc;
+        if (int rc = forward(B, a->s_dev, a->type_src, a->type_rec == 0 ? 1 : 0,
+                             a->nrec > 0 ? a->dobs_dev + (size_t)s0 * nvec : nullptr, nvec,
+                             a->ndat > 0 ? a->datum_dev + (size_t)s0 * ndv : nullptr, ndv, nullptr))
+            return rc;
     }
     HIPCHK(hipStreamSynchronize(g.stream));
     return 0;

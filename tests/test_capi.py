@@ -1,0 +1,77 @@
+"""The C-ABI library on the CPU (no GPU): it loads, exports every symbol
+include/daw.h declares, its host-side geometry is bit-identical to the
+reference's (via the oracle), and argument errors come back as codes."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "daw.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(daw_\w+)\s*\(", txt, re.M)))
+
+
+def test_exports_every_declared_symbol(daw):
+    L = daw.lib()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(daw.EXPORTS)
+    assert L.daw_version() == 1
+
+
+def test_host_geometry_bitexact(daw):
+    assert np.array_equal(daw.hicks1d().view(np.uint32), O.hicks1d().view(np.uint32))
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        a, b, p, ofs = int(rng.integers(1, 101)), int(rng.integers(1, 101)), int(rng.integers(0, 11)), \
+            int(rng.integers(-1, 2))
+        assert np.array_equal(daw.hicks2d_tile(a, b, p, ofs).view(np.uint32), O.hicks2d_tile(a, b, p, ofs).view(np.uint32))
+    z = rng.uniform(0, 300, 3000).astype(np.float32)
+    x = rng.uniform(0, 6000, 3000).astype(np.float32)
+    for h in (10.0, 8.0, 6.25, 3.125):
+        for ofs in (0, 1):
+            assert np.array_equal(daw.subacqui(z, x, np.float32(h), 20, ofs), O.acqui_table(z, x, np.float32(h), 20, ofs))
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "ref_small.npz"), allow_pickle=False)
+    assert np.array_equal(daw.subacqui(gold["acqui_z"], gold["acqui_x"], gold["acqui_h"], int(gold["acqui_npml"]), 1),
+                          gold["acqui_tab_ofs1"])
+
+
+def test_filter_host(daw):
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "ref_small.npz"), allow_pickle=False)
+    d = gold["filt_in"]
+    y = daw.filmartin2d(d, 1e-3, 3.0, 30.0, 2.0, 10.0)
+    ref = gold["filt_out"]
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-6
+    y2 = daw.filmartin2d(d, 8e-4, 2.0, 700.0, 1.0, 5.0)
+    assert np.linalg.norm(y2 - gold["filt_out2"]) / np.linalg.norm(gold["filt_out2"]) < 1e-6
+    with pytest.raises(daw.DawError):
+        daw.filmartin2d(np.zeros((1, 70000), np.float32), 1e-3, 1, 2, 1, 1)
+
+
+def test_error_codes(daw):
+    L = daw.lib()
+    # propagators refuse to run before kwsinc2d, as the reference needs its table first
+    L.daw_deallocatekws2d()
+    daw._hicks_ofs = None
+    vp = np.full((10, 10), 1500.0, np.float32)
+    s = np.zeros(5, np.float32)
+    is5 = np.array([12, 12, 50, 50, 0], np.int32)
+    ir = np.array([[12], [12], [50], [50], [0]], np.int32)
+    sis = np.zeros((1, 5), np.float32)
+    rc = L.daw_submodeling(vp, 10, 10, 10, C.c_float(10.0), 1, 5, C.c_float(1e-3), s, 0, 0, is5, ir, 1, sis, None)
+    assert rc == -2 and b"kwsinc2d" in L.daw_last_error()
+    L.daw_kwsinc2d(1)
+    rc = L.daw_submodeling(vp, 10, 10, 2, C.c_float(10.0), 1, 5, C.c_float(1e-3), s, 0, 0, is5, ir, 1, sis, None)
+    assert rc == -1          # npml < 3
+    rc = L.daw_hicks2d_tile(0, 1, 0, 1, np.zeros(441, np.float32))
+    assert rc == -1
+    L.daw_deallocatekws2d()
