@@ -87,13 +87,25 @@ def cpu_baseline(P, nt_sample):
     res = rng.standard_normal((P["rec"].shape[1], nt_sample)).astype(np.float32)
     kind = "reference" if O.ref_available() else "port"
     if kind == "reference":
-        O.ref_init(w.ofs)
         fn = lambda: O.ref_esource_data(P["c0"], w.npml, P["h"], w.ofs, nt_sample, P["dt"], res, P["rec"])
     else:
         fn = lambda: O.esource_data(P["c0"], w.npml, P["h"], w.ofs, nt_sample, P["dt"], res, P["rec"])
-    t0 = time.perf_counter()
-    fn()
-    t = time.perf_counter() - t0
+    # the reference writes progress text to stdout (kwsinc.f90:224, minresModule.f90:512);
+    # keep this process's stdout for the one JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        if kind == "reference":
+            O.ref_init(w.ofs)
+        t0 = time.perf_counter()
+        fn()
+        t = time.perf_counter() - t0
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
     sweeps = 7 + 2 * w.maxiter
     t_shot = t / (2.0 * nt_sample) * sweeps * w.nt
     n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml

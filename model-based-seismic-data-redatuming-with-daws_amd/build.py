@@ -54,5 +54,38 @@ def build(verbose=False):
     return LIB
 
 
+def build_fortran(verbose=False):
+    """The iso_c_binding drop-in (fortran/daw_shim.f90 -> fortran/_build/
+    libdaw_fortran.a + .mod files) and the test driver that calls it the way
+    the reference's driver calls the reference (tests/fortran/)."""
+    fc = os.environ.get("FC", "flang")
+    fdir = os.path.join(HERE, "fortran", "_build")
+    tsrc = os.path.join(ROOT, "tests", "fortran")
+    tdir = os.path.join(tsrc, "_build")
+    os.makedirs(fdir, exist_ok=True)
+    os.makedirs(tdir, exist_ok=True)
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+    ff = ["-O2", "-fPIC"]
+    run([fc] + ff + ["-module-dir", fdir, "-c", os.path.join(HERE, "fortran", "daw_shim.f90"),
+                     "-o", os.path.join(fdir, "daw_shim.o")])
+    run(["ar", "rcs", os.path.join(fdir, "libdaw_fortran.a"), os.path.join(fdir, "daw_shim.o")])
+    run([fc] + ff + ["-module-dir", tdir, "-c", os.path.join(tsrc, "parameters_test.f90"),
+                     "-o", os.path.join(tdir, "parameters_test.o")])
+    run([fc] + ff + ["-I", fdir, "-I", tdir, "-module-dir", tdir, "-c",
+                     os.path.join(HERE, "fortran", "daw_shim_minres.f90"), "-o", os.path.join(tdir, "daw_shim_minres.o")])
+    rpath = "-Wl,-rpath,$ORIGIN/../../../" + os.path.basename(HERE)
+    run([fc] + ff + ["-I", fdir, "-I", tdir, os.path.join(tsrc, "shim_driver.f90"),
+                     os.path.join(tdir, "parameters_test.o"), os.path.join(tdir, "daw_shim_minres.o"),
+                     os.path.join(fdir, "libdaw_fortran.a"), "-L", HERE, "-ldaw", rpath,
+                     "-o", os.path.join(tdir, "shim_driver")])
+    return os.path.join(tdir, "shim_driver")
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))
