@@ -141,6 +141,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0)
     ap.add_argument("--max-batch", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0, help="16, 32, or 0 for the autotuner")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,6 +153,7 @@ def main():
         calibrate_copy()
         return
     daw, syn = _pkg()
+    daw.set_tile_rows(args.tile_rows)
     _log("setup: modelling observed data")
     P = setup(daw, syn, args.config, shots=args.shots, nt=args.nt, maxiter=args.maxiter)
     w = P["w"]
@@ -220,7 +222,8 @@ def main():
         "data": "synthetic: seeded layered model (sigma 3 smoothing), background sigma 10, Ricker, 1% seeded noise",
         "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m", "shots": nsh, "nt": w.nt,
                    "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
-                   "shots_per_launch": shots_per_launch, "snapshots": "fp32"},
+                   "shots_per_launch": shots_per_launch, "snapshots": "fp32",
+                   "tile": [daw.get_tile_rows(), 64]},
         "gpts_per_s": gpts,
         "plain_shots_per_hour": plain,
         "batched_over_plain": (shots_per_hour / plain) if plain else None,

@@ -164,6 +164,11 @@ typedef struct daw_stats {
 } daw_stats;
 
 int daw_set_timing(int on);
+/* Rows per step-kernel tile: 16 or 32 (tiles are 64 columns wide), or 0 to
+ * let daw_redatum_batch time both on a short probe sweep per batch shape and
+ * keep the faster (the default).  No reference counterpart. */
+int daw_set_tile_rows(int tz);
+int daw_get_tile_rows(void);
 int daw_reset_stats(void);
 int daw_get_stats(daw_stats *st);
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "daw_subsourcekws", "daw_subacqui", "daw_submodeling", "daw_submodeling_adjoint_esource",
     "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
-    "daw_get_stats", "daw_modeling_batch",
+    "daw_get_stats", "daw_modeling_batch", "daw_set_tile_rows", "daw_get_tile_rows",
 ]
 
 
@@ -360,6 +360,15 @@ def modeling_batch(vp, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dobs_dev, 
 
 def set_timing(on):
     _check(lib().daw_set_timing(1 if on else 0))
+
+
+def set_tile_rows(tz):
+    """16 or 32 rows per step-kernel tile, or 0 for the autotuner (default)."""
+    _check(lib().daw_set_tile_rows(int(tz)))
+
+
+def get_tile_rows():
+    return int(lib().daw_get_tile_rows())
 
 
 def reset_stats():

@@ -22,12 +22,10 @@ enum SweepMode { kFwd = 0, kBack = 1, kReinject = 2 };
 // stencil needs (two cells each side for the derivative, and one more
 // pressure ring recomputed locally so the velocity half-step needs no
 // second launch).
+// TX columns (one wavefront width) by TZ rows; TZ is 16 or 32, a launch
+// parameter the tile autotuner picks.
 constexpr int TX = 64;
-constexpr int TZ = 16;
 constexpr int kBlock = 256;
-constexpr int PW = TX + 3, PH = TZ + 3;       // pressure rect: x in [x0-1, x0+TX+2), z in [z0-1, z0+TZ+2)
-constexpr int VXW = TX + 6, VXH = TZ + 3;     // vx: x in [x0-3, x0+TX+3)
-constexpr int VZW = TX + 3, VZH = TZ + 6;     // vz: z in [z0-3, z0+TZ+3)
 
 struct ShotDev {
     // grid

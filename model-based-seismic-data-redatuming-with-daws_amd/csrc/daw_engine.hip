@@ -57,6 +57,9 @@ struct Global {
     long long pts = 0;         // point-updates executed (sum n1e*n2e per shot per step)
     long long launches = 0;
     int max_batch_used = 0;
+    int tz = 16;               // tile rows of k_step (16 or 32)
+    bool tz_auto = true;       // pick tz per batch shape with a timed probe
+    std::map<std::vector<int>, int> tz_cache;
 };
 static Global g;
 
@@ -316,7 +319,7 @@ static int finalize_sweepset(SweepSet &W)
     W.pts_per_step = 0;
     for (int s = 0; s < W.nshots; ++s) {
         const ShotDev &S = W.host[s];
-        tps[s] = ((S.n1e + TZ - 1) / TZ) * ((S.n2e + TX - 1) / TX);
+        tps[s] = ((S.n1e + g.tz - 1) / g.tz) * ((S.n2e + TX - 1) / TX);
         W.nt_max = std::max(W.nt_max, S.nt);
         W.pts_per_step += (long long)S.n1e * S.n2e;
     }
@@ -333,7 +336,7 @@ static int refresh_shots(SweepSet &W)
 }
 
 // Runs one sweep of nt_max steps over the batch and the trailing record.
-static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec)
+static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int nsteps_limit = 0)
 {
     // zero the read half of the double-buffered state
     for (int s = 0; s < W.nshots; ++s) {
@@ -356,8 +359,9 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec)
         HIPCHK(hipEventRecord(e0, g.stream));
     }
     int cur = 0;
-    for (int it = 0; it <= W.nt_max; ++it) {
-        P.ntiles = (it < W.nt_max) ? W.ntiles : 0;
+    const int nsteps = nsteps_limit > 0 ? std::min(nsteps_limit, W.nt_max) : W.nt_max;
+    for (int it = 0; it <= nsteps; ++it) {
+        P.ntiles = (it < nsteps) ? W.ntiles : 0;
         P.it = it;
         P.cur = cur;
         const bool rec_prev = it > 0 && nrec > 0;
@@ -365,7 +369,7 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec)
         P.nrec_blocks = rec_prev ? rec_blocks : 0;
         P.rec_it = it - 1;
         P.rec_slot = (it - 1) & 1;
-        launch_step(mode, P, g.stream);
+        launch_step(mode, P, g.tz, g.stream);
         cur ^= 1;
     }
     HIPCHK(hipGetLastError());
@@ -375,12 +379,12 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec)
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         g.t_ms[mode] += ms;
-        g.t_cnt[mode] += W.nt_max;
+        g.t_cnt[mode] += nsteps;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
-    g.pts += W.pts_per_step * W.nt_max;
-    g.launches += W.nt_max + 1;
+    g.pts += W.pts_per_step * nsteps;
+    g.launches += nsteps + 1;
     return 0;
 }
 
@@ -513,6 +517,54 @@ struct Batch {
     ShotStore store;
     SweepSet W;
 };
+
+// Tile autotuner: time a short back-propagation probe (zero residuals) at
+// every candidate tile height and keep the fastest for this batch shape.
+static int autotune_tiles(Batch &B)
+{
+    if (!g.tz_auto) return 0;
+    const std::vector<int> key{B.G.n1e, B.G.n2e, B.nshots};
+    auto hit = g.tz_cache.find(key);
+    if (hit != g.tz_cache.end()) {
+        if (hit->second != g.tz) { g.tz = hit->second; return finalize_sweepset(B.W); }
+        return 0;
+    }
+    if (!B.store.snap.p) return 0;
+    DevBuf zeros;
+    if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
+    launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
+    for (auto &S : B.W.host) set_inject(S, B.inj_rec, zeros.as<float>(), 0, 0);
+    const bool timing = g.timing;
+    const double pts = g.pts;
+    const long long launches = g.launches;
+    g.timing = false;
+    const int probe = std::min(24, B.nt);
+    int best = 16;
+    float best_ms = 1e30f;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int tz : {16, 32}) {
+        g.tz = tz;
+        if (int rc = finalize_sweepset(B.W)) return rc;
+        if (int rc = run_sweep(kBack, B.W, nullptr, 0, 4)) return rc;       // warm
+        HIPCHK(hipEventRecord(e0, g.stream));
+        if (int rc = run_sweep(kBack, B.W, nullptr, 0, probe)) return rc;
+        HIPCHK(hipEventRecord(e1, g.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = tz; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g.timing = timing;
+    g.pts = pts;
+    g.launches = launches;
+    g.tz_cache[key] = best;
+    g.tz = best;
+    return finalize_sweepset(B.W);
+}
 
 static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o)
 {
@@ -961,6 +1013,7 @@ int daw_redatum_batch(const daw_redatum_args *a)
         for (int s = 0; s < ns; ++s)
             for (int k = 0; k < 5; ++k) srows[s + (size_t)k * ns] = a->src_tab[(s0 + s) + (size_t)k * a->nsrc_total];
         if (int rc = build_batch(B, G, M, nt, ns, srows.data(), ns, a->rec_tab, nrec, a->dat_tab, ndat, true)) return rc;
+        if (int rc = autotune_tiles(B)) return rc;
         DevBuf d_datr, d_datum_r, d_res, d_x, d_esc, d_datum_e;
         if (int rc = d_datr.alloc(ns * nvec * 4)) return rc;
         if (int rc = d_res.alloc(ns * nvec * 4)) return rc;
@@ -1037,6 +1090,21 @@ int daw_modeling_batch(const daw_modeling_args *a)
     }
     HIPCHK(hipStreamSynchronize(g.stream));
     return 0;
+}
+
+int daw_set_tile_rows(int tz)
+{
+    if (tz != 0 && tz != 16 && tz != 32) return fail(DAW_ERR_ARG, "tile rows must be 0 (autotune), 16 or 32");
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.tz_auto = tz == 0;
+    if (tz) g.tz = tz;
+    return 0;
+}
+
+int daw_get_tile_rows(void)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    return g.tz;
 }
 
 int daw_set_timing(int on)

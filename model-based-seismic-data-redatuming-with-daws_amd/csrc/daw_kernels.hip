@@ -19,33 +19,151 @@ __device__ __forceinline__ bool in_box(int z, int x, int lo, int hiz, int hix)
 }
 
 // ---------------------------------------------------------------------------
-// Trailing record blocks: sis(it, r) = sum over the Hicks window of c * p,
-// i1 outer / i2 inner as submodeling2.f90:214-219 (extended-grid clip) and
-// subwavefieldresample.f90:32-37 (interior clip) accumulate it.
+// Record blocks: sis(it, r) = sum over the Hicks window of c * p, one
+// sequential chain i1 outer / i2 inner exactly as submodeling2.f90:214-219
+// (extended-grid clip) and subwavefieldresample.f90:32-37 (interior clip)
+// accumulate it.  Each 21-point window row is loaded first and summed after,
+// so a row costs one memory latency instead of 21.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void record_item(const StepParams &P, int k)
 {
     const RecItem R = P.rec[k];
-    const ShotDev &S = P.shots[R.shot];
-    if (P.rec_it >= S.nt) return;
+    const ShotDev *__restrict__ S = P.shots + R.shot;
+    const int nt = S->nt;
+    if (P.rec_it >= nt) return;
+    const int npml = S->npml, n1e = S->n1e, n2e = S->n2e, ldx = S->ldx;
     int lo1, hi1, lo2, hi2;
     if (R.clip == 0) {
-        lo1 = -R.ix1; hi1 = S.n1e - 1 - R.ix1;
-        lo2 = -R.ix2; hi2 = S.n2e - 1 - R.ix2;
+        lo1 = -R.ix1; hi1 = n1e - 1 - R.ix1;
+        lo2 = -R.ix2; hi2 = n2e - 1 - R.ix2;
     } else {
-        lo1 = S.npml - R.ix1; hi1 = S.npml + S.n1 - 1 - R.ix1;
-        lo2 = S.npml - R.ix2; hi2 = S.npml + S.n2 - 1 - R.ix2;
+        lo1 = npml - R.ix1; hi1 = npml + S->n1 - 1 - R.ix1;
+        lo2 = npml - R.ix2; hi2 = npml + S->n2 - 1 - R.ix2;
     }
     lo1 = lo1 > -10 ? lo1 : -10; hi1 = hi1 < 10 ? hi1 : 10;
     lo2 = lo2 > -10 ? lo2 : -10; hi2 = hi2 < 10 ? hi2 : 10;
-    const float *band = S.pband[P.rec_slot];
+    const float *__restrict__ band = S->pband[P.rec_slot];
+    const int *__restrict__ brow = S->band_row;
+    const float *__restrict__ coef = R.coef;
     float acc = 0.0f;
-    for (int i1 = lo1; i1 <= hi1; ++i1) {
-        const float *row = band + (size_t)S.band_row[R.ix1 + i1] * S.ldx + R.ix2;
-        const float *c = R.coef + (i1 + 10);
-        for (int i2 = lo2; i2 <= hi2; ++i2) acc = acc + c[21 * (i2 + 10)] * row[i2];
+    if (lo2 == -10 && hi2 == 10) {
+        for (int i1 = lo1; i1 <= hi1; ++i1) {
+            const float *row = band + (size_t)brow[R.ix1 + i1] * ldx + (R.ix2 - 10);
+            const float *c = coef + (i1 + 10);
+            float v[21], w[21];
+#pragma unroll
+            for (int j = 0; j < 21; ++j) { v[j] = row[j]; w[j] = c[21 * j]; }
+#pragma unroll
+            for (int j = 0; j < 21; ++j) acc = acc + w[j] * v[j];
+        }
+    } else {
+        for (int i1 = lo1; i1 <= hi1; ++i1) {
+            const float *row = band + (size_t)brow[R.ix1 + i1] * ldx + R.ix2;
+            const float *c = coef + (i1 + 10);
+            for (int i2 = lo2; i2 <= hi2; ++i2) acc = acc + c[21 * (i2 + 10)] * row[i2];
+        }
     }
-    R.out[R.reverse ? (S.nt - 1 - P.rec_it) : P.rec_it] = acc;
+    R.out[R.reverse ? (nt - 1 - P.rec_it) : P.rec_it] = acc;
+}
+
+// Pressure half-step at one point of the pressure rectangle (the reference's
+// statement order: split-PML update on the stencil box, Hicks injection,
+// snapshot re-injection, p = px + pz, free surface).  vr points at vx(gz,gx)
+// in LDS, vc at vz(gz,gx).  Returns p; writes px/pz when `owned`.
+// Flat per-launch view of one shot: every field a scalar or a pointer, the
+// double-buffer halves already selected, so it lives in registers.
+struct View {
+    int n1, n2, npml, n1e, n2e, ldx, nt, ofs, amp_stride, vinj;
+    int xlo1, xhi1, xlo2, xhi2;
+    float dt, dtdx;
+    const float *kappa, *d1a, *d1a1, *d1b, *bw1, *d2a, *d2a1, *d2b, *bw2;
+    const float *pxo, *pzo, *vxo, *vzo;
+    float *pxn, *pzn, *vxn, *vzn;
+    const int *inj_row, *inj_ptr, *inj_r;
+    const float *inj_cdt, *inj_amp;
+    const int *band_row;
+    float *pband;
+    float *sp, *svx, *svz, *wf;
+};
+
+__device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur, int it)
+{
+    View v;
+    v.n1 = S->n1; v.n2 = S->n2; v.npml = S->npml; v.n1e = S->n1e; v.n2e = S->n2e; v.ldx = S->ldx;
+    v.nt = S->nt; v.ofs = S->ofs; v.amp_stride = S->amp_stride; v.vinj = S->vinj;
+    v.xlo1 = S->xlo1; v.xhi1 = S->xhi1; v.xlo2 = S->xlo2; v.xhi2 = S->xhi2;
+    v.dt = S->dt; v.dtdx = S->dtdx;
+    v.kappa = S->kappa; v.d1a = S->d1a; v.d1a1 = S->d1a1; v.d1b = S->d1b; v.bw1 = S->bw1;
+    v.d2a = S->d2a; v.d2a1 = S->d2a1; v.d2b = S->d2b; v.bw2 = S->bw2;
+    const bool c1 = cur != 0;
+    v.pxo = c1 ? S->px[1] : S->px[0]; v.pzo = c1 ? S->pz[1] : S->pz[0];
+    v.vxo = c1 ? S->vx[1] : S->vx[0]; v.vzo = c1 ? S->vz[1] : S->vz[0];
+    v.pxn = c1 ? S->px[0] : S->px[1]; v.pzn = c1 ? S->pz[0] : S->pz[1];
+    v.vxn = c1 ? S->vx[0] : S->vx[1]; v.vzn = c1 ? S->vz[0] : S->vz[1];
+    v.inj_row = S->inj_row; v.inj_ptr = S->inj_ptr; v.inj_r = S->inj_r; v.inj_cdt = S->inj_cdt; v.inj_amp = S->inj_amp;
+    v.band_row = S->band_row;
+    v.pband = (it & 1) ? S->pband[1] : S->pband[0];
+    v.sp = S->sp; v.svx = S->svx; v.svz = S->svz; v.wf = S->wf;
+    return v;
+}
+
+template <int MODE, int VZW_>
+__device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, int gx, float pxv, float pzv,
+                                         float kap, const float *vr, const float *vc, float d1a, float d1a1,
+                                         float d2a, float d2a1, int injq, bool owned, int lo, int hiz, int hix,
+                                         float *__restrict__ pxn, float *__restrict__ pzn)
+{
+    const float a0 = 1.125f;
+    const float a1 = -(1.0f / 24.0f);
+    if (in_box(gz, gx, lo, hiz, hix)) {
+        const float der2 = a0 * (vr[0] - vr[-1]) + a1 * (vr[1] - vr[-2]);
+        const float der1 = a0 * (vc[0] - vc[-VZW_]) + a1 * (vc[VZW_] - vc[-2 * VZW_]);
+        const float tk = S.dtdx * kap;                      // kappaw = dtdx*kappa*damp1 (:105-106)
+        const float kw2 = tk * d2a1;
+        const float kw1 = tk * d1a1;
+        if (MODE == kBack) {
+            pxv = d2a * pxv - kw2 * der2;
+            pzv = d1a * pzv - kw1 * der1;
+        } else {
+            pxv = d2a * pxv + kw2 * der2;
+            pzv = d1a * pzv + kw1 * der1;
+        }
+    }
+    if (MODE != kReinject && injq >= 0) {
+        const int *ptr = S.inj_ptr + (size_t)injq * (S.n2e + 1);
+        const int kb = ptr[gx], ke = ptr[gx + 1];
+        for (int k = kb; k < ke; ++k) {
+            const float term = (S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + it]) * kap;
+            if (MODE == kBack) pxv = pxv - term;   // :1042
+            else pxv = pxv + term;                 // :175
+        }
+    }
+    if (MODE == kReinject) {
+        if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2) {
+            const float sv = S.sp[(size_t)ic * ((size_t)S.n1 * S.n2) + (size_t)(gz - S.npml) * S.n2 + (gx - S.npml)];
+            pxv = pxv + ((-S.dt) * sv) * kap;          // ss*dt*sp*kappa, ss = -1 (:1123)
+        }
+    }
+    float p = pxv + pzv;
+    if (S.ofs == 1 && gz <= S.npml) p = 0.0f;           // p(1:npml+1,:) = 0
+    if (owned) {
+        const size_t g = (size_t)gz * S.ldx + gx;
+        pxn[g] = pxv;
+        pzn[g] = pzv;
+        const int npml = S.npml;
+        const bool interior = gz >= npml && gz < npml + S.n1 && gx >= npml && gx < npml + S.n2;
+        if (MODE == kBack && interior)
+            S.sp[(size_t)it * ((size_t)S.n1 * S.n2) + (size_t)(gz - npml) * S.n2 + (gx - npml)] = p;
+        if (MODE != kBack) {
+            if (S.band_row != nullptr) {
+                const int slot = S.band_row[gz];
+                if (slot >= 0) S.pband[(size_t)slot * S.ldx + gx] = p;
+            }
+            if (S.wf != nullptr && interior)
+                S.wf[(size_t)it * ((size_t)S.n1 * S.n2) + (size_t)(gx - npml) * S.n1 + (gz - npml)] = p;
+        }
+    }
+    return p;
 }
 
 }  // namespace
@@ -53,36 +171,44 @@ __device__ __forceinline__ void record_item(const StepParams &P, int k)
 // ---------------------------------------------------------------------------
 // One time step of one sweep for every active shot of a batch.
 //
-// Per owned tile (TZ x TX points):
-//   1. stage old vx (x halo 3/3) and vz (z halo 3/3) in LDS;
-//   2. over the tile plus a one/two-cell pressure ring: split-PML pressure
-//      update, Hicks injection gather (kFwd source, kBack residuals) or the
-//      time-reversed snapshot re-injection (kReinject), p = px + pz, free
-//      surface; p of the ring stays in LDS, owned px/pz/p go to HBM;
-//   3. velocity update of the owned points from LDS p, vz source or
-//      snapshot re-injection, vz(npml) = vz(npml+1) mirror; write vx/vz
-//      (and the kBack velocity snapshots).
-// Blocks past ntiles run the Hicks record of the previous step.
+// Block layout: the first nrec_blocks workgroups run the Hicks record of the
+// previous step (so its latency-bound chains overlap the stencil work); the
+// rest each own a TZ x 64 tile.  Wavefront w of a tile block owns rows
+// w, w+4, ... with lane = column, so every global access is one aligned
+// 256-byte row segment and per-row / per-column PML profiles stay in
+// registers.  Per tile:
+//   1. stage old vx (x halo 3/3) and vz (z halo 3/3) in LDS and load
+//      px, pz, kappa of the pressure rectangle into registers;
+//   2. pressure half-step on the tile plus the one/two-cell ring the
+//      velocity stencil needs (the ring's p stays in LDS);
+//   3. velocity half-step of the owned points, vz source or snapshot
+//      re-injection, the vz(npml) = vz(npml+1) mirror, writes.
 // ---------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, int TZ>
 __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
 {
-    const int b = blockIdx.x;
-    const int tid = threadIdx.x;
-    if (b >= P.ntiles) {
-        const int k = (b - P.ntiles) * kBlock + tid;
+    constexpr int NW = kBlock / 64;
+    constexpr int PW_ = TX + 3, PH_ = TZ + 3;
+    constexpr int VXW_ = TX + 6, VXH_ = TZ + 3;
+    constexpr int VZW_ = TX + 3, VZH_ = TZ + 6;
+    constexpr int PROWS = (PH_ + NW - 1) / NW;
+    constexpr int VROWS = TZ / NW;
+    static_assert(TZ % NW == 0, "tile rows must split evenly over the wavefronts");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if ((int)blockIdx.x < P.nrec_blocks) {
+        const int k = blockIdx.x * kBlock + tid;
         if (k < P.nrec) record_item(P, k);
         return;
     }
-    __shared__ float s_vx[VXH * VXW];
-    __shared__ float s_vz[VZH * VZW];
-    __shared__ float s_p[PH * PW];
+    __shared__ float s_vx[VXH_ * VXW_];
+    __shared__ float s_vz[VZH_ * VZW_];
+    __shared__ float s_p[PH_ * PW_];
 
-    const int2 t = P.tiles[b];
-    const ShotDev &S = P.shots[t.x];
+    const int2 t = P.tiles[blockIdx.x - P.nrec_blocks];
     const int it = P.it;
+    const View S = make_view(P.shots + t.x, P.cur, it);   // registers: stores below cannot alias it
     if (it >= S.nt) return;
-    const int n1e = S.n1e, n2e = S.n2e, ldx = S.ldx, npml = S.npml;
+    const int n1e = S.n1e, n2e = S.n2e, ldx = S.ldx;
     const int ntx = (n2e + TX - 1) / TX;
     const int z0 = (t.y / ntx) * TZ;
     const int x0 = (t.y % ntx) * TX;
@@ -91,114 +217,126 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const int lo = 2;
     const int hiz = (MODE == kFwd) ? n1e - 3 : n1e - 4;
     const int hix = (MODE == kFwd) ? n2e - 3 : n2e - 4;
-    const int cur = P.cur, nxt = P.cur ^ 1;
-    const float *__restrict__ vxo = S.vx[cur];
-    const float *__restrict__ vzo = S.vz[cur];
-    const float *__restrict__ pxo = S.px[cur];
-    const float *__restrict__ pzo = S.pz[cur];
-    float *__restrict__ pxn = S.px[nxt];
-    float *__restrict__ pzn = S.pz[nxt];
-    float *__restrict__ vxn = S.vx[nxt];
-    float *__restrict__ vzn = S.vz[nxt];
+    const float *__restrict__ vxo = S.vxo;
+    const float *__restrict__ vzo = S.vzo;
+    const float *__restrict__ pxo = S.pxo;
+    const float *__restrict__ pzo = S.pzo;
+    float *__restrict__ pxn = S.pxn;
+    float *__restrict__ pzn = S.pzn;
+    float *__restrict__ vxn = S.vxn;
+    float *__restrict__ vzn = S.vzn;
+    const int ic = S.nt - 1 - it;   // time-reversed snapshot index (icount = nt-it+1)
+    const int gxl = x0 + lane;       // this lane's column
+    const bool colok = gxl < n2e;
+
+    // ---- 1. stage the velocity halos, preload the pressure rectangle
+    for (int r = wv; r < VXH_; r += NW) {
+        const int gz = z0 - 1 + r;
+        s_vx[r * VXW_ + lane + 3] = (gz >= 0 && gz < n1e && colok) ? vxo[(size_t)gz * ldx + gxl] : 0.0f;
+    }
+    for (int r = wv; r < VZH_; r += NW) {
+        const int gz = z0 - 3 + r;
+        s_vz[r * VZW_ + lane + 1] = (gz >= 0 && gz < n1e && colok) ? vzo[(size_t)gz * ldx + gxl] : 0.0f;
+    }
+    for (int i = tid; i < VXH_ * 6; i += kBlock) {
+        const int r = i / 6, k = i - 6 * r;
+        const int c = k < 3 ? k : TX + k;
+        const int gz = z0 - 1 + r, gx = x0 - 3 + c;
+        s_vx[r * VXW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? vxo[(size_t)gz * ldx + gx] : 0.0f;
+    }
+    for (int i = tid; i < VZH_ * 3; i += kBlock) {
+        const int r = i / 3, k = i - 3 * r;
+        const int c = k == 0 ? 0 : TX + k;
+        const int gz = z0 - 3 + r, gx = x0 - 1 + c;
+        s_vz[r * VZW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? vzo[(size_t)gz * ldx + gx] : 0.0f;
+    }
+    float pxr[PROWS], pzr[PROWS], kar[PROWS];
+#pragma unroll
+    for (int j = 0; j < PROWS; ++j) {
+        const int r = wv + NW * j;
+        const int gz = z0 - 1 + r;
+        pxr[j] = 0.0f; pzr[j] = 0.0f; kar[j] = 0.0f;
+        if (r < PH_ && gz >= 0 && gz < n1e && colok) {
+            const size_t g = (size_t)gz * ldx + gxl;
+            pxr[j] = pxo[g];
+            pzr[j] = pzo[g];
+            kar[j] = S.kappa[g];
+        }
+    }
+    // side points of the ring: columns x0-1, x0+TX, x0+TX+1 on the tile rows,
+    // taken by the highest threads (the wavefront with the fewest core rows)
+    const int sid = kBlock - 1 - tid;
+    int s_r = 0, s_c = 0, s_gz = -1, s_gx = -1;
+    float s_px = 0.0f, s_pz = 0.0f, s_ka = 0.0f;
+    const bool side = sid < 3 * TZ;
+    if (side) {
+        const int k = sid / TZ;
+        s_r = 1 + (sid - k * TZ);
+        s_c = k == 0 ? 0 : TX + k;
+        s_gz = z0 - 1 + s_r;
+        s_gx = x0 - 1 + s_c;
+        if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
+            const size_t g = (size_t)s_gz * ldx + s_gx;
+            s_px = pxo[g];
+            s_pz = pzo[g];
+            s_ka = S.kappa[g];
+        }
+    }
+    const float c_d2a = colok ? S.d2a[gxl] : 1.0f, c_d2a1 = colok ? S.d2a1[gxl] : 1.0f;
+    __syncthreads();
+
+    // ---- 2. pressure half-step
+#pragma unroll
+    for (int j = 0; j < PROWS; ++j) {
+        const int r = wv + NW * j;
+        if (r < PH_) {
+            const int gz = z0 - 1 + r;
+            float p = 0.0f;
+            if (gz >= 0 && gz < n1e && colok) {
+                const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[gz] : -1;
+                p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + lane + 3],
+                                        &s_vz[(r + 2) * VZW_ + lane + 1], S.d1a[gz], S.d1a1[gz], c_d2a, c_d2a1,
+                                        injq, r >= 1 && r <= TZ, lo, hiz, hix, pxn, pzn);
+            }
+            s_p[r * PW_ + lane + 1] = p;
+        }
+    }
+    if (side) {
+        float p = 0.0f;
+        if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
+            const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[s_gz] : -1;
+            p = p_point<MODE, VZW_>(S, it, ic, s_gz, s_gx, s_px, s_pz, s_ka, &s_vx[s_r * VXW_ + s_c + 2],
+                                    &s_vz[(s_r + 2) * VZW_ + s_c], S.d1a[s_gz], S.d1a1[s_gz], S.d2a[s_gx],
+                                    S.d2a1[s_gx], injq, false, lo, hiz, hix, pxn, pzn);
+        }
+        s_p[s_r * PW_ + s_c] = p;
+    }
+    __syncthreads();
+
+    // ---- 3. velocity half-step of the owned points
+    if (!colok) return;
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
-
-    for (int i = tid; i < VXH * VXW; i += kBlock) {
-        const int r = i / VXW, c = i - r * VXW;
-        const int gz = z0 - 1 + r, gx = x0 - 3 + c;
-        s_vx[i] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? vxo[(size_t)gz * ldx + gx] : 0.0f;
-    }
-    for (int i = tid; i < VZH * VZW; i += kBlock) {
-        const int r = i / VZW, c = i - r * VZW;
-        const int gz = z0 - 3 + r, gx = x0 - 1 + c;
-        s_vz[i] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? vzo[(size_t)gz * ldx + gx] : 0.0f;
-    }
-    __syncthreads();
-
-    const int n1 = S.n1, n2 = S.n2;
+    const float c_d2b = S.d2b[gxl], c_bw2 = S.bw2[gxl];
+    const int npml = S.npml, n1 = S.n1, n2 = S.n2;
     const size_t plane = (size_t)n1 * n2;
-    const int ic = S.nt - 1 - it;   // time-reversed snapshot index (icount = nt-it+1)
-
-    for (int i = tid; i < PH * PW; i += kBlock) {
-        const int r = i / PW, c = i - r * PW;
-        const int gz = z0 - 1 + r, gx = x0 - 1 + c;
-        float p = 0.0f;
-        if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) {
-            const size_t g = (size_t)gz * ldx + gx;
-            float pxv = pxo[g], pzv = pzo[g];
-            const float kap = S.kappa[g];
-            if (in_box(gz, gx, lo, hiz, hix)) {
-                const float *vr = &s_vx[r * VXW + c + 2];          // vx(gz, gx)
-                const float *vc = &s_vz[(r + 2) * VZW + c];        // vz(gz, gx)
-                const float der2 = a0 * (vr[0] - vr[-1]) + a1 * (vr[1] - vr[-2]);
-                const float der1 = a0 * (vc[0] - vc[-VZW]) + a1 * (vc[VZW] - vc[-2 * VZW]);
-                const float tk = S.dtdx * kap;                      // kappaw = dtdx*kappa*damp1 (:105-106)
-                const float kw2 = tk * S.d2a1[gx];
-                const float kw1 = tk * S.d1a1[gz];
-                if (MODE == kBack) {
-                    pxv = S.d2a[gx] * pxv - kw2 * der2;
-                    pzv = S.d1a[gz] * pzv - kw1 * der1;
-                } else {
-                    pxv = S.d2a[gx] * pxv + kw2 * der2;
-                    pzv = S.d1a[gz] * pzv + kw1 * der1;
-                }
-            }
-            if (MODE != kReinject && S.inj_row != nullptr && !S.vinj) {
-                const int q = S.inj_row[gz];
-                if (q >= 0) {
-                    const int *ptr = S.inj_ptr + (size_t)q * (n2e + 1);
-                    const int kb = ptr[gx], ke = ptr[gx + 1];
-                    for (int k = kb; k < ke; ++k) {
-                        const float term = (S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + it]) * kap;
-                        if (MODE == kBack) pxv = pxv - term;   // :1042
-                        else pxv = pxv + term;                 // :175
-                    }
-                }
-            }
-            if (MODE == kReinject) {
-                if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2) {
-                    const float sv = S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml)];
-                    pxv = pxv + ((-S.dt) * sv) * kap;          // ss*dt*sp*kappa, ss = -1 (:1123)
-                }
-            }
-            p = pxv + pzv;
-            if (S.ofs == 1 && gz <= npml) p = 0.0f;           // p(1:npml+1,:) = 0
-            if (r >= 1 && r <= TZ && c >= 1 && c <= TX) {
-                pxn[g] = pxv;
-                pzn[g] = pzv;
-                const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
-                if (MODE == kBack && interior)
-                    S.sp[(size_t)it * plane + (size_t)(gz - npml) * n2 + (gx - npml)] = p;
-                if (MODE != kBack) {
-                    if (S.band_row != nullptr) {
-                        const int slot = S.band_row[gz];
-                        if (slot >= 0) S.pband[it & 1][(size_t)slot * ldx + gx] = p;
-                    }
-                    if (S.wf != nullptr && interior)
-                        S.wf[(size_t)it * plane + (size_t)(gx - npml) * n1 + (gz - npml)] = p;
-                }
-            }
-        }
-        s_p[i] = p;
-    }
-    __syncthreads();
-
-    for (int i = tid; i < TZ * TX; i += kBlock) {
-        const int r = i / TX, c = i - r * TX;
-        const int gz = z0 + r, gx = x0 + c;
-        if (gz >= n1e || gx >= n2e) continue;
+#pragma unroll
+    for (int j = 0; j < VROWS; ++j) {
+        const int r = wv + NW * j;
+        const int gz = z0 + r, gx = gxl;
+        if (gz >= n1e) break;
         const size_t g = (size_t)gz * ldx + gx;
-        float vxv = s_vx[(r + 1) * VXW + c + 3];
-        float vzv = s_vz[(r + 3) * VZW + c + 1];
+        float vxv = s_vx[(r + 1) * VXW_ + lane + 3];
+        float vzv = s_vz[(r + 3) * VZW_ + lane + 1];
         if (in_box(gz, gx, lo, hiz, hix)) {
-            const float *pp = &s_p[(r + 1) * PW + c + 1];         // p(gz, gx)
+            const float *pp = &s_p[(r + 1) * PW_ + lane + 1];         // p(gz, gx)
             const float der2 = a0 * (pp[1] - pp[0]) + a1 * (pp[2] - pp[-1]);
-            const float der1 = a0 * (pp[PW] - pp[0]) + a1 * (pp[2 * PW] - pp[-PW]);
+            const float der1 = a0 * (pp[PW_] - pp[0]) + a1 * (pp[2 * PW_] - pp[-PW_]);
             if (MODE == kBack) {
-                vxv = S.d2b[gx] * vxv - S.bw2[gx] * der2;
+                vxv = c_d2b * vxv - c_bw2 * der2;
                 vzv = S.d1b[gz] * vzv - S.bw1[gz] * der1;
             } else {
-                vxv = S.d2b[gx] * vxv + S.bw2[gx] * der2;
+                vxv = c_d2b * vxv + c_bw2 * der2;
                 vzv = S.d1b[gz] * vzv + S.bw1[gz] * der1;
             }
         }
@@ -231,17 +369,21 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     }
 }
 
-template __global__ void k_step<kFwd>(StepParams);
-template __global__ void k_step<kBack>(StepParams);
-template __global__ void k_step<kReinject>(StepParams);
+template <int MODE>
+static void launch_mode(const StepParams &P, int tz, hipStream_t st)
+{
+    const int grid = P.ntiles + P.nrec_blocks;
+    if (tz == 32) hipLaunchKernelGGL((k_step<MODE, 32>), dim3(grid), dim3(kBlock), 0, st, P);
+    else hipLaunchKernelGGL((k_step<MODE, 16>), dim3(grid), dim3(kBlock), 0, st, P);
+}
 
-void launch_step(int mode, const StepParams &P, hipStream_t st)
+void launch_step(int mode, const StepParams &P, int tz, hipStream_t st)
 {
     const int grid = P.ntiles + P.nrec_blocks;
     if (grid == 0) return;
-    if (mode == kFwd) hipLaunchKernelGGL(k_step<kFwd>, dim3(grid), dim3(kBlock), 0, st, P);
-    else if (mode == kBack) hipLaunchKernelGGL(k_step<kBack>, dim3(grid), dim3(kBlock), 0, st, P);
-    else hipLaunchKernelGGL(k_step<kReinject>, dim3(grid), dim3(kBlock), 0, st, P);
+    if (mode == kFwd) launch_mode<kFwd>(P, tz, st);
+    else if (mode == kBack) launch_mode<kBack>(P, tz, st);
+    else launch_mode<kReinject>(P, tz, st);
 }
 
 // ---------------------------------------------------------------------------
