@@ -9,7 +9,8 @@ seeded noise (untimed input synthesis, the reference's mode 0).
 
 One step = one redatuming pass over all shots: background forward, residual,
 fac, 30 MINRES iterations (each one back-propagation + re-injection sweep
-pair), the source-extension sweep pair and the datum gathers, with inputs
+pair), the source-extension sweep pair, the datum gathers and their
+band-pass (filmartin2d, as mode 2 writes dataobs_redatum), with inputs
 resident in HBM.  The line also reports the plain one-shot-at-a-time GPU path,
 the roofline of the dominant step kernel and the reference's own CPU path
 timed on a bounded sample on this host.
@@ -55,6 +56,9 @@ def setup(daw, syn, cfg, shots=None, nt=None, maxiter=None):
     dobs = torch.zeros((nsh, rec.shape[1], w.nt), dtype=torch.float32, device="cuda")
     dnew = torch.zeros((nsh, dat.shape[1], w.nt), dtype=torch.float32, device="cuda")
     daw.modeling_batch(vp, w.npml, h, w.ofs, w.nt, dt, s_dev, src, rec, dobs, dat_tab=dat, datum_dev=dnew)
+    # data recorded at the datum in the true model, band-passed like the redatumed gathers
+    dnew2 = dnew.view(-1, w.nt)
+    daw.filmartin2d_dev(dnew2, dnew2, float(dt), *w.band)
     g = torch.Generator(device="cuda")
     g.manual_seed(2000 + cfg)
     rms = float(torch.sqrt(torch.mean(dobs.double() ** 2)))
@@ -71,7 +75,7 @@ def run_redatum(daw, P, shots=None, max_batch=0, hbm_budget=0.0):
     datum = torch.zeros((nsh, P["dat"].shape[1], w.nt), dtype=torch.float32, device="cuda")
     hist = daw.redatum_batch(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], P["s_dev"], P["src"][:, :nsh],
                              P["rec"], P["dat"], P["dobs"][:nsh].contiguous(), w.mu, w.maxiter, 0.0,
-                             datum_dev=datum, max_batch=max_batch, hbm_budget_bytes=hbm_budget)
+                             datum_dev=datum, max_batch=max_batch, hbm_budget_bytes=hbm_budget, band=w.band)
     return datum, hist
 
 

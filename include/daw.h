@@ -99,6 +99,11 @@ int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int 
 int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
                     float *dout);
 
+/* filmartin2d on device traces din/dout [nr][nt] (may alias); FFT lengths
+ * up to 8192 run on the GPU, longer ones on the host. */
+int daw_filmartin2d_dev(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
+                        float *dout);
+
 /* ---- batched redatuming (no reference counterpart: it replaces the
  * para_range shot loop of sub_redatum_DAW.f90:30-155 plus mode 2
  * :230-238 for a block of shots advanced together on one GPU) ---------- */
@@ -119,12 +124,14 @@ typedef struct daw_redatum_args {
     float mu;
     int maxiter;
     float rtol;               /* grid%error */
-    float *datum_dev;         /* device out, redatumed gathers [shot][ndat][nt] (unfiltered) */
+    float *datum_dev;         /* device out, redatumed gathers [shot][ndat][nt] (band-passed when filter = 1) */
     float *datae_dev;         /* device out or NULL: datar + dataesc [shot][nrec][nt] */
     float *deltade_dev;       /* device out or NULL: MINRES solutions [shot][nrec][nt] */
     double *hist_host;        /* host out or NULL: qrnorm/beta1 [shot][maxiter] */
     int max_batch;            /* shots advanced together; <= 0: admit by the HBM budget */
     double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
+    int filter;               /* 1: band-pass the gathers as mode 2 does (filmartin2d) */
+    float flow, fhig, llow, lhig;
 } daw_redatum_args;
 
 int daw_redatum_batch(const daw_redatum_args *args);

@@ -33,7 +33,7 @@ EXPORTS = [
     "daw_subsourcekws", "daw_subacqui", "daw_submodeling", "daw_submodeling_adjoint_esource",
     "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
-    "daw_get_stats", "daw_modeling_batch", "daw_set_tile_rows", "daw_get_tile_rows",
+    "daw_get_stats", "daw_modeling_batch", "daw_set_tile_rows", "daw_get_tile_rows", "daw_filmartin2d_dev",
 ]
 
 
@@ -51,6 +51,7 @@ class RedatumArgs(C.Structure):
         ("dobs_dev", C.c_void_p), ("mu", C.c_float), ("maxiter", C.c_int), ("rtol", C.c_float),
         ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
         ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
+        ("filter", C.c_int), ("flow", C.c_float), ("fhig", C.c_float), ("llow", C.c_float), ("lhig", C.c_float),
     ]
 
 
@@ -275,13 +276,14 @@ def filmartin2d(din, dt, flow, fhig, llow, lhig):
 
 def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, dobs_dev, mu, maxiter, rtol=0.0,
                   datum_dev=None, datae_dev=None, deltade_dev=None, want_hist=True, max_batch=0,
-                  hbm_budget_bytes=0.0, type_src=0):
+                  hbm_budget_bytes=0.0, type_src=0, band=None):
     """Mode-1 DAW plus the mode-2 datum gather of sub_redatum_DAW.f90:82-155,
     :238 for all shots of src_tab, advanced together on the GPU.
 
     s_dev, dobs_dev and the outputs are torch device tensors (float32):
     dobs_dev [nshots, nrec, nt]; datum_dev [nshots, ndat, nt] receives the
-    redatumed (unfiltered) gathers.  Returns the MINRES histories
+    redatumed gathers, band-passed as mode 2 does when band = (flow, fhig,
+    llow, lhig) is given.  Returns the MINRES histories
     (nshots, maxiter) when want_hist."""
     import torch
     _ensure_hicks(ofs)
@@ -318,6 +320,9 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
     a.hist_host = hist.ctypes.data if hist is not None else None
     a.max_batch = int(max_batch)
     a.hbm_budget_bytes = float(hbm_budget_bytes)
+    if band is not None:
+        a.filter = 1
+        a.flow, a.fhig, a.llow, a.lhig = (float(v) for v in band)
     torch.cuda.synchronize()
     _check(lib().daw_redatum_batch(C.byref(a)))
     return hist
@@ -360,6 +365,16 @@ def modeling_batch(vp, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dobs_dev, 
 
 def set_timing(on):
     _check(lib().daw_set_timing(1 if on else 0))
+
+
+def filmartin2d_dev(din_dev, dout_dev, dt, flow, fhig, llow, lhig):
+    """filmartin2d on torch device traces [nr, nt] (dout may be din)."""
+    import torch
+    nr, nt = din_dev.shape
+    torch.cuda.synchronize()
+    _check(lib().daw_filmartin2d_dev(C.c_void_p(din_dev.data_ptr()), nr, nt, C.c_float(dt), C.c_float(flow),
+                                     C.c_float(fhig), C.c_float(llow), C.c_float(lhig),
+                                     C.c_void_p(dout_dev.data_ptr())))
 
 
 def set_tile_rows(tz):

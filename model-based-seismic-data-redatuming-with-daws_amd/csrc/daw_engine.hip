@@ -655,6 +655,40 @@ static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int n
     return finalize_sweepset(B.W);
 }
 
+// ------------------------------------------------------------------ band-pass on device traces
+// filmartin2d (minresModule.f90:486-614) for ntr traces [tr][t] in HBM.  FFT
+// lengths up to 8192 run in LDS on the GPU; longer ones on the host.
+static int filter_device(const float *din, float *dout, int ntr, int nt, float dt, float flow, float fhig, float llow,
+                         float lhig)
+{
+    std::vector<double> w;
+    const int nto = filter_design(nt, dt, flow, fhig, llow, lhig, w);
+    if (nto < 0) return fail(DAW_ERR_ARG, "filmartin2d: nt=%d exceeds the 65536-sample FFT length list", nt);
+    if (ntr <= 0) return 0;
+    if (nto > 8192) {
+        std::vector<float> h((size_t)ntr * nt), o((size_t)ntr * nt);
+        HIPCHK(hipStreamSynchronize(g.stream));
+        HIPCHK(hipMemcpy(h.data(), din, h.size() * 4, hipMemcpyDeviceToHost));
+        filter_apply_host(h.data(), ntr, nt, dt, flow, fhig, llow, lhig, o.data());
+        HIPCHK(hipMemcpy(dout, o.data(), o.size() * 4, hipMemcpyHostToDevice));
+        return 0;
+    }
+    std::vector<double> tw((size_t)nto);   // nto/2 complex twiddles
+    for (int m = 0; m < nto / 2; ++m) {
+        const double ang = 2.0 * M_PI * (double)m / (double)nto;
+        tw[2 * m] = std::cos(ang);
+        tw[2 * m + 1] = -std::sin(ang);
+    }
+    w.resize(nto / 2 + 1);
+    DevBuf d_w, d_tw;
+    if (int rc = upload(d_w, w)) return rc;
+    if (int rc = upload(d_tw, tw)) return rc;
+    launch_filter(din, dout, ntr, nt, nto, d_w.as<double>(), d_tw.as<double2>(), g.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
 // ------------------------------------------------------------------ MINRES over a batch
 struct MinresWork {
     DevBuf vec;     // x, v, Y[3], Wv[3], hb: 9 vectors per shot
@@ -1045,8 +1079,13 @@ int daw_redatum_batch(const daw_redatum_args *a)
         if (a->datae_dev) launch_add(d_datr.as<float>(), d_esc.as<float>(), a->datae_dev + (size_t)s0 * nvec, ns * nvec, g.stream);
         if (a->deltade_dev)
             HIPCHK(hipMemcpyAsync(a->deltade_dev + (size_t)s0 * nvec, d_x.p, ns * nvec * 4, hipMemcpyDeviceToDevice, g.stream));
-        if (ndat > 0)
-            launch_add(d_datum_r.as<float>(), d_datum_e.as<float>(), a->datum_dev + (size_t)s0 * ndv, ns * ndv, g.stream);
+        if (ndat > 0) {
+            float *dd = a->datum_dev + (size_t)s0 * ndv;
+            launch_add(d_datum_r.as<float>(), d_datum_e.as<float>(), dd, ns * ndv, g.stream);
+            // mode 2 band-pass of the redatumed gathers (sub_redatum_DAW.f90:240)
+            if (a->filter)
+                if (int rc = filter_device(dd, dd, ns * ndat, nt, a->dt, a->flow, a->fhig, a->llow, a->lhig)) return rc;
+        }
         HIPCHK(hipEventRecord(e1, g.stream));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
@@ -1090,6 +1129,15 @@ int daw_modeling_batch(const daw_modeling_args *a)
     }
     HIPCHK(hipStreamSynchronize(g.stream));
     return 0;
+}
+
+int daw_filmartin2d_dev(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
+                        float *dout)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (nr < 0 || nt < 1 || (nr > 0 && (!din || !dout))) return fail(DAW_ERR_ARG, "filmartin2d_dev: bad argument");
+    if (int rc = ensure_stream()) return rc;
+    return filter_device(din, dout, nr, nt, dt, flow, fhig, llow, lhig);
 }
 
 int daw_set_tile_rows(int tz)

@@ -13,6 +13,11 @@ void launch_add(const float *a, const float *b, float *c, size_t n, hipStream_t 
 void launch_resample(const float *wf, int n1, int n2, int nt, const int *ix1, const int *ix2, const float *coef,
                      int nr, float *sis, hipStream_t st);
 
+// Band-pass (filmartin2d) of ntr device traces of nt samples with FFT length
+// nto <= 8192; wgt = dmartin(1..nto/2+1), tw[m] = exp(-2 pi i m / nto).
+int launch_filter(const float *din, float *dout, int ntr, int nt, int nto, const double *wgt, const double2 *tw,
+                  hipStream_t st);
+
 // MINRES over a batch: vectors of shot s start at off[s] and hold len[s]
 // floats; part holds minres_vec_blocks() FP64 partial sums per shot; scal
 // holds minres_scal_stride() FP64 scalars per shot.

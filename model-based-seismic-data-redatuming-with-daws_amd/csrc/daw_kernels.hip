@@ -111,7 +111,7 @@ template <int MODE, int VZW_>
 __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, int gx, float pxv, float pzv,
                                          float kap, const float *vr, const float *vc, float d1a, float d1a1,
                                          float d2a, float d2a1, int injq, bool owned, int lo, int hiz, int hix,
-                                         float *__restrict__ pxn, float *__restrict__ pzn)
+                                         float spv, float *__restrict__ pxn, float *__restrict__ pzn)
 {
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
@@ -139,10 +139,8 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
         }
     }
     if (MODE == kReinject) {
-        if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2) {
-            const float sv = S.sp[(size_t)ic * ((size_t)S.n1 * S.n2) + (size_t)(gz - S.npml) * S.n2 + (gx - S.npml)];
-            pxv = pxv + ((-S.dt) * sv) * kap;          // ss*dt*sp*kappa, ss = -1 (:1123)
-        }
+        if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2)
+            pxv = pxv + ((-S.dt) * spv) * kap;         // ss*dt*sp*kappa, ss = -1 (:1123)
     }
     float p = pxv + pzv;
     if (S.ofs == 1 && gz <= S.npml) p = 0.0f;           // p(1:npml+1,:) = 0
@@ -229,45 +227,62 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const int gxl = x0 + lane;       // this lane's column
     const bool colok = gxl < n2e;
 
-    // ---- 1. stage the velocity halos, preload the pressure rectangle
-    for (int r = wv; r < VXH_; r += NW) {
-        const int gz = z0 - 1 + r;
-        s_vx[r * VXW_ + lane + 3] = (gz >= 0 && gz < n1e && colok) ? vxo[(size_t)gz * ldx + gxl] : 0.0f;
+    // ---- 1. every global load of the step is issued here, into registers,
+    // before anything waits on one: velocity halos, the pressure rectangle,
+    // and (re-injection) the snapshot values
+    constexpr int XR = (VXH_ + NW - 1) / NW, ZR = (VZH_ + NW - 1) / NW;
+    const int npml = S.npml, n1 = S.n1, n2 = S.n2;
+    const size_t plane = (size_t)n1 * n2;
+    float vxl[XR], vzl[ZR];
+#pragma unroll
+    for (int j = 0; j < XR; ++j) {
+        const int r = wv + NW * j, gz = z0 - 1 + r;
+        vxl[j] = (r < VXH_ && gz >= 0 && gz < n1e && colok) ? vxo[(size_t)gz * ldx + gxl] : 0.0f;
     }
-    for (int r = wv; r < VZH_; r += NW) {
-        const int gz = z0 - 3 + r;
-        s_vz[r * VZW_ + lane + 1] = (gz >= 0 && gz < n1e && colok) ? vzo[(size_t)gz * ldx + gxl] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) {
+        const int r = wv + NW * j, gz = z0 - 3 + r;
+        vzl[j] = (r < VZH_ && gz >= 0 && gz < n1e && colok) ? vzo[(size_t)gz * ldx + gxl] : 0.0f;
     }
-    for (int i = tid; i < VXH_ * 6; i += kBlock) {
-        const int r = i / 6, k = i - 6 * r;
+    float hx = 0.0f, hz = 0.0f;
+    int hx_i = -1, hz_i = -1;
+    if (tid < VXH_ * 6) {
+        const int r = tid / 6, k = tid - 6 * r;
         const int c = k < 3 ? k : TX + k;
         const int gz = z0 - 1 + r, gx = x0 - 3 + c;
-        s_vx[r * VXW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? vxo[(size_t)gz * ldx + gx] : 0.0f;
+        hx_i = r * VXW_ + c;
+        if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hx = vxo[(size_t)gz * ldx + gx];
     }
-    for (int i = tid; i < VZH_ * 3; i += kBlock) {
-        const int r = i / 3, k = i - 3 * r;
-        const int c = k == 0 ? 0 : TX + k;
-        const int gz = z0 - 3 + r, gx = x0 - 1 + c;
-        s_vz[r * VZW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? vzo[(size_t)gz * ldx + gx] : 0.0f;
+    {
+        const int t2 = kBlock - 1 - tid;
+        if (t2 < VZH_ * 3) {
+            const int r = t2 / 3, k = t2 - 3 * r;
+            const int c = k == 0 ? 0 : TX + k;
+            const int gz = z0 - 3 + r, gx = x0 - 1 + c;
+            hz_i = r * VZW_ + c;
+            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hz = vzo[(size_t)gz * ldx + gx];
+        }
     }
-    float pxr[PROWS], pzr[PROWS], kar[PROWS];
+    float pxr[PROWS], pzr[PROWS], kar[PROWS], spr[PROWS];
 #pragma unroll
     for (int j = 0; j < PROWS; ++j) {
         const int r = wv + NW * j;
         const int gz = z0 - 1 + r;
-        pxr[j] = 0.0f; pzr[j] = 0.0f; kar[j] = 0.0f;
+        pxr[j] = 0.0f; pzr[j] = 0.0f; kar[j] = 0.0f; spr[j] = 0.0f;
         if (r < PH_ && gz >= 0 && gz < n1e && colok) {
             const size_t g = (size_t)gz * ldx + gxl;
             pxr[j] = pxo[g];
             pzr[j] = pzo[g];
             kar[j] = S.kappa[g];
+            if (MODE == kReinject && gz >= S.xlo1 && gz <= S.xhi1 && gxl >= S.xlo2 && gxl <= S.xhi2)
+                spr[j] = S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gxl - npml)];
         }
     }
     // side points of the ring: columns x0-1, x0+TX, x0+TX+1 on the tile rows,
     // taken by the highest threads (the wavefront with the fewest core rows)
     const int sid = kBlock - 1 - tid;
     int s_r = 0, s_c = 0, s_gz = -1, s_gx = -1;
-    float s_px = 0.0f, s_pz = 0.0f, s_ka = 0.0f;
+    float s_px = 0.0f, s_pz = 0.0f, s_ka = 0.0f, s_sp = 0.0f;
     const bool side = sid < 3 * TZ;
     if (side) {
         const int k = sid / TZ;
@@ -280,8 +295,33 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             s_px = pxo[g];
             s_pz = pzo[g];
             s_ka = S.kappa[g];
+            if (MODE == kReinject && s_gz >= S.xlo1 && s_gz <= S.xhi1 && s_gx >= S.xlo2 && s_gx <= S.xhi2)
+                s_sp = S.sp[(size_t)ic * plane + (size_t)(s_gz - npml) * n2 + (s_gx - npml)];
         }
     }
+    float svxr[VROWS], svzr[VROWS];
+#pragma unroll
+    for (int j = 0; j < VROWS; ++j) {
+        const int gz = z0 + wv + NW * j;
+        svxr[j] = 0.0f; svzr[j] = 0.0f;
+        if (MODE == kReinject && colok && gz >= npml && gz < npml + n1 && gxl >= npml && gxl < npml + n2) {
+            const size_t sidx = (size_t)ic * plane + (size_t)(gz - npml) * n2 + (gxl - npml);
+            svxr[j] = S.svx[sidx];
+            svzr[j] = S.svz[sidx];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < XR; ++j) {
+        const int r = wv + NW * j;
+        if (r < VXH_) s_vx[r * VXW_ + lane + 3] = vxl[j];
+    }
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) {
+        const int r = wv + NW * j;
+        if (r < VZH_) s_vz[r * VZW_ + lane + 1] = vzl[j];
+    }
+    if (hx_i >= 0) s_vx[hx_i] = hx;
+    if (hz_i >= 0) s_vz[hz_i] = hz;
     const float c_d2a = colok ? S.d2a[gxl] : 1.0f, c_d2a1 = colok ? S.d2a1[gxl] : 1.0f;
     __syncthreads();
 
@@ -296,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
                 const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[gz] : -1;
                 p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + lane + 3],
                                         &s_vz[(r + 2) * VZW_ + lane + 1], S.d1a[gz], S.d1a1[gz], c_d2a, c_d2a1,
-                                        injq, r >= 1 && r <= TZ, lo, hiz, hix, pxn, pzn);
+                                        injq, r >= 1 && r <= TZ, lo, hiz, hix, spr[j], pxn, pzn);
             }
             s_p[r * PW_ + lane + 1] = p;
         }
@@ -307,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[s_gz] : -1;
             p = p_point<MODE, VZW_>(S, it, ic, s_gz, s_gx, s_px, s_pz, s_ka, &s_vx[s_r * VXW_ + s_c + 2],
                                     &s_vz[(s_r + 2) * VZW_ + s_c], S.d1a[s_gz], S.d1a1[s_gz], S.d2a[s_gx],
-                                    S.d2a1[s_gx], injq, false, lo, hiz, hix, pxn, pzn);
+                                    S.d2a1[s_gx], injq, false, lo, hiz, hix, s_sp, pxn, pzn);
         }
         s_p[s_r * PW_ + s_c] = p;
     }
@@ -318,8 +358,6 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
     const float c_d2b = S.d2b[gxl], c_bw2 = S.bw2[gxl];
-    const int npml = S.npml, n1 = S.n1, n2 = S.n2;
-    const size_t plane = (size_t)n1 * n2;
 #pragma unroll
     for (int j = 0; j < VROWS; ++j) {
         const int r = wv + NW * j;
@@ -350,9 +388,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         }
         const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
         if (MODE == kReinject && interior) {
-            const size_t s = (size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml);
-            vxv = vxv + (-S.dt) * S.svx[s];                        // * bue (= 1), :1177
-            vzv = vzv + (-S.dt) * S.svz[s];                        // :1184
+            vxv = vxv + (-S.dt) * svxr[j];                         // * bue (= 1), :1177
+            vzv = vzv + (-S.dt) * svzr[j];                         // :1184
         }
         vxn[g] = vxv;
         if (S.ofs == 1 && gz == npml - 1) {
@@ -759,6 +796,81 @@ void launch_minres_x(float *x, float *wn, const float *w2, const float *w1, cons
 {
     hipLaunchKernelGGL(k_minres_x, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, x, wn, w2, w1, v, off, len, scal);
     hipLaunchKernelGGL(k_minres_close, dim3((nshots + 63) / 64), dim3(64), 0, st, scal, nshots);
+}
+
+// ---------------------------------------------------------------------------
+// Band-pass of filmartin2d (minresModule.f90:486-614) on device traces.
+// The transfer function W (real, W[nto-k] = W[k]) is applied in FP64 like the
+// reference's FFT.  Because W is real and even, one complex FFT of length nto
+// filters two real traces at once: x1 + i x2 -> IFFT(W FFT(x1 + i x2)) =
+// y1 + i y2.  One workgroup per trace pair, the whole transform in LDS
+// (nto <= 8192: 128 KB).  tw[m] = exp(-2 pi i m / nto), m < nto/2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int bitrev(int v, int bits) { return (int)(__brev((unsigned)v) >> (32 - bits)); }
+
+__device__ void lds_fft(double2 *a, const double2 *__restrict__ tw, int nto, int bits, bool inverse)
+{
+    for (int len = 2; len <= nto; len <<= 1) {
+        const int half = len >> 1, stride = nto / len;
+        for (int j = threadIdx.x; j < nto / 2; j += blockDim.x) {
+            const int grp = j / half, k = j - grp * half;
+            const int i0 = grp * len + k, i1 = i0 + half;
+            double2 w = tw[k * stride];
+            if (inverse) w.y = -w.y;
+            const double2 u = a[i0], v = a[i1];
+            const double2 vw = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+            a[i0] = make_double2(u.x + vw.x, u.y + vw.y);
+            a[i1] = make_double2(u.x - vw.x, u.y - vw.y);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_filter(const float *__restrict__ din, float *__restrict__ dout, int ntr,
+                                                int nt, int nto, int bits, const double *__restrict__ wgt,
+                                                const double2 *__restrict__ tw)
+{
+    extern __shared__ double2 a[];
+    const int t1 = 2 * blockIdx.x, t2 = t1 + 1;
+    const bool has2 = t2 < ntr;
+    for (int i = threadIdx.x; i < nto; i += blockDim.x) {
+        const double x1 = i < nt ? (double)din[(size_t)t1 * nt + i] : 0.0;
+        const double x2 = (i < nt && has2) ? (double)din[(size_t)t2 * nt + i] : 0.0;
+        a[bitrev(i, bits)] = make_double2(x1, x2);
+    }
+    __syncthreads();
+    lds_fft(a, tw, nto, bits, false);
+    for (int k = threadIdx.x; k < nto; k += blockDim.x) {
+        const int kk = k <= nto / 2 ? k : nto - k;
+        // bin kk < nto/2 takes dmartin(kk+1); the Nyquist bin takes dmartin(1)
+        const double g = kk == nto / 2 ? wgt[0] : wgt[kk];
+        const double2 v = a[k];
+        a[k] = make_double2(v.x * g, v.y * g);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nto; i += blockDim.x) {
+        const int r = bitrev(i, bits);
+        if (i < r) { const double2 tmp = a[i]; a[i] = a[r]; a[r] = tmp; }
+    }
+    __syncthreads();
+    lds_fft(a, tw, nto, bits, true);
+    const double sc = 1.0 / (double)nto;
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+        dout[(size_t)t1 * nt + i] = (float)(a[i].x * sc);
+        if (has2) dout[(size_t)t2 * nt + i] = (float)(a[i].y * sc);
+    }
+}
+
+int launch_filter(const float *din, float *dout, int ntr, int nt, int nto, const double *wgt, const double2 *tw,
+                  hipStream_t st)
+{
+    if (ntr <= 0) return 0;
+    int bits = 0;
+    while ((1 << bits) < nto) ++bits;
+    const size_t lds = (size_t)nto * sizeof(double2);
+    hipFuncSetAttribute((const void *)k_filter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_filter, dim3((ntr + 1) / 2), dim3(256), lds, st, din, dout, ntr, nt, nto, bits, wgt, tw);
+    return 0;
 }
 
 int minres_scal_stride() { return kScal; }

@@ -534,5 +534,105 @@ done:
     return rc;
 }
 
+
+/*
+ * Exact discrete adjoint of submodeling (no reference counterpart: the
+ * reference back-propagates with a sign-flipped forward scheme,
+ * submodeling2.f90:657-717, which is not the transpose).  For the linear map
+ * F: s(nt) -> sis(nt,nr) of the forward scheme with type_src = type_rec = 0,
+ * this returns g = F^T d, the transpose applied step by step in reverse time:
+ * mirror, velocity update, record, free surface, source injection, pressure
+ * update.  The dot test <F s, d> = <s, F^T d> pins it.
+ */
+int SFX(oracle_forward_adjoint)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                                const R *d, const int *is5, const int *ir, int nr, R *g)
+{
+    SFX(setup_t) S;
+    SFX(setup_make)(&S, vp, n1, n2, npml, h, dt);
+    const int n1e = S.n1e, n2e = S.n2e;
+    const size_t ne = (size_t)n1e * n2e;
+    const R a0 = (R)1.125, a1 = -((R)1 / (R)24);
+    R *qpx = calloc(ne, sizeof(R)), *qpz = calloc(ne, sizeof(R)), *qvx = calloc(ne, sizeof(R));
+    R *qvz = calloc(ne, sizeof(R)), *qp = calloc(ne, sizeof(R)), *ux = calloc(ne, sizeof(R)), *uz = calloc(ne, sizeof(R));
+    float stile[441];
+    oracle_hicks2d_tile(is5[2], is5[3], is5[4], ofs, stile);
+    float *rt = SFX(tiles_for)(ir, nr, ofs);
+    const int lo = 3, hi1 = n1e - 2, hi2 = n2e - 2;     /* forward stencil box */
+    for (int it = nt; it >= 1; it--) {
+        /* mirror vz(npml,:) = vz(npml+1,:) */
+        if (ofs == 1)
+            for (int i2 = 1; i2 <= n2e; i2++) {
+                E(qvz, npml + 1, i2) = E(qvz, npml + 1, i2) + E(qvz, npml, i2);
+                E(qvz, npml, i2) = 0;
+            }
+        /* velocity update: v' = damp_b v + buw D+ p on the box */
+        memset(qp, 0, ne * sizeof(R));
+        memset(ux, 0, ne * sizeof(R));
+        memset(uz, 0, ne * sizeof(R));
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                E(ux, i1, i2) = E(S.buw2, i1, i2) * E(qvx, i1, i2);
+                E(uz, i1, i2) = E(S.buw1, i1, i2) * E(qvz, i1, i2);
+                E(qvx, i1, i2) = S.damp2b[i2] * E(qvx, i1, i2);
+                E(qvz, i1, i2) = S.damp1b[i1] * E(qvz, i1, i2);
+            }
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                R u = E(ux, i1, i2), w = E(uz, i1, i2);
+                E(qp, i1, i2 + 1) += a0 * u; E(qp, i1, i2) -= a0 * u; E(qp, i1, i2 + 2) += a1 * u; E(qp, i1, i2 - 1) -= a1 * u;
+                E(qp, i1 + 1, i2) += a0 * w; E(qp, i1, i2) -= a0 * w; E(qp, i1 + 2, i2) += a1 * w; E(qp, i1 - 1, i2) -= a1 * w;
+            }
+        /* record sis(it,r) = sum c p  ->  qp += c d(it,r) */
+        for (int r = 0; r < nr; r++) {
+            const int ix1 = ir[r], ix2 = ir[r + nr];
+            const int r1min = 1 - ix1 > -10 ? 1 - ix1 : -10, r1max = n1e - ix1 < 10 ? n1e - ix1 : 10;
+            const int r2min = 1 - ix2 > -10 ? 1 - ix2 : -10, r2max = n2e - ix2 < 10 ? n2e - ix2 : 10;
+            const R dv = d[(size_t)(it - 1) + (size_t)r * nt];
+            for (int i1 = r1min; i1 <= r1max; i1++)
+                for (int i2 = r2min; i2 <= r2max; i2++)
+                    E(qp, ix1 + i1, ix2 + i2) += (R)rt[(size_t)r * 441 + (i1 + 10) + 21 * (i2 + 10)] * dv;
+        }
+        /* p = M (px + pz) */
+        for (int i2 = 1; i2 <= n2e; i2++)
+            for (int i1 = 1; i1 <= n1e; i1++) {
+                R q = (ofs == 1 && i1 <= npml + 1) ? (R)0 : E(qp, i1, i2);
+                E(qpx, i1, i2) += q;
+                E(qpz, i1, i2) += q;
+            }
+        /* source injection px += c dt s(it) kappa  ->  g(it) */
+        {
+            const int ix1 = is5[0], ix2 = is5[1];
+            const int r1min = 1 - ix1 > -10 ? 1 - ix1 : -10, r1max = n1e - ix1 < 10 ? n1e - ix1 : 10;
+            const int r2min = 1 - ix2 > -10 ? 1 - ix2 : -10, r2max = n2e - ix2 < 10 ? n2e - ix2 : 10;
+            R acc = 0;
+            for (int i1 = r1min; i1 <= r1max; i1++)
+                for (int i2 = r2min; i2 <= r2max; i2++)
+                    acc += (R)stile[(i1 + 10) + 21 * (i2 + 10)] * S.dt * E(S.kappa, ix1 + i1, ix2 + i2) *
+                           E(qpx, ix1 + i1, ix2 + i2);
+            g[it - 1] = acc;
+        }
+        /* pressure update: p' = damp_a p + kappaw D- v on the box */
+        memset(ux, 0, ne * sizeof(R));
+        memset(uz, 0, ne * sizeof(R));
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                E(ux, i1, i2) = E(S.kappaw2, i1, i2) * E(qpx, i1, i2);
+                E(uz, i1, i2) = E(S.kappaw1, i1, i2) * E(qpz, i1, i2);
+                E(qpx, i1, i2) = S.damp2a[i2] * E(qpx, i1, i2);
+                E(qpz, i1, i2) = S.damp1a[i1] * E(qpz, i1, i2);
+            }
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                R u = E(ux, i1, i2), w = E(uz, i1, i2);
+                E(qvx, i1, i2) += a0 * u; E(qvx, i1, i2 - 1) -= a0 * u; E(qvx, i1, i2 + 1) += a1 * u; E(qvx, i1, i2 - 2) -= a1 * u;
+                E(qvz, i1, i2) += a0 * w; E(qvz, i1 - 1, i2) -= a0 * w; E(qvz, i1 + 1, i2) += a1 * w; E(qvz, i1 - 2, i2) -= a1 * w;
+            }
+    }
+    free(rt);
+    free(qpx); free(qpz); free(qvx); free(qvz); free(qp); free(ux); free(uz);
+    SFX(setup_free)(&S);
+    return 0;
+}
+
 #undef E
 #undef I3

@@ -63,6 +63,10 @@ def lib():
             f = getattr(L, "oracle_resample_" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, C.c_int, C.c_int, rp, rp]
             f.restype = C.c_int
+            f = getattr(L, "oracle_forward_adjoint_" + sfx)
+            f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, rp, _i32p,
+                          _i32p, C.c_int, rp]
+            f.restype = C.c_int
             f = getattr(L, "oracle_linearminres_" + sfx)
             f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
                           C.c_int, C.c_int, _i32p, C.c_float, C.c_int, C.c_float, rp, rp, rp, rp, _i32p]
@@ -140,6 +144,21 @@ def forward(vp, npml, h, ofs, nt, dt, s, src_row, rec_tab, type_src=0, type_rec=
     if rc != 0:
         raise RuntimeError("oracle forward failed")
     return (sis, wf) if want_wavefield else sis
+
+
+def forward_adjoint(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab, prec="f64"):
+    """Exact discrete adjoint of forward (type_src = type_rec = 0): g = F^T d,
+    g has nt samples (the gradient with respect to the source wavelet)."""
+    vp = _c(vp, np.float32)
+    n2, n1 = vp.shape
+    rec_tab = _c(rec_tab, np.int32)
+    nr = rec_tab.shape[1]
+    g = np.zeros(nt, _rdt(prec))
+    rc = getattr(lib(), "oracle_forward_adjoint_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, _c(d, _rdt(prec)),
+                                                           _c(src_row, np.int32), rec_tab, nr, g)
+    if rc != 0:
+        raise RuntimeError("oracle forward_adjoint failed")
+    return g
 
 
 def esource_data(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, prec="f32"):

@@ -70,3 +70,19 @@ def test_redatum_batch_pipeline(gpu_daw):
             print(s, prec, "deltade/datae/datum rel", e, flush=True)
             assert max(e) <= tol, (s, prec, e)
             assert np.allclose(hist[s], h, rtol=1e-4)
+
+
+@pytest.mark.parametrize("nt,dt", [(300, 1e-3), (3000, 8e-4), (6000, 5e-4), (9000, 4e-4)])
+def test_filter_device(gpu_daw, nt, dt):
+    """filmartin2d on device traces: FFT lengths up to 8192 on the GPU, longer on the host."""
+    import torch
+    rng = np.random.default_rng(nt)
+    d = rng.standard_normal((7, nt)).astype(np.float32)
+    band = (2.0, 25.0, 1.0, 5.0)
+    x = torch.from_numpy(d).cuda()
+    y = torch.empty_like(x)
+    gpu_daw.filmartin2d_dev(x, y, dt, *band)
+    ref = O.filmartin2d(d, dt, *band)
+    assert rel_l2(y.cpu().numpy(), ref) < 1e-6
+    gpu_daw.filmartin2d_dev(x, x, dt, *band)          # in place
+    assert rel_l2(x.cpu().numpy(), ref) < 1e-6
