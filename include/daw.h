@@ -75,6 +75,20 @@ int daw_submodeling_adjoint_esource(const float *vp, int n1, int n2, int npml, f
 int daw_submodeling_adjoint_esource_data(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt,
                                          float dt, const float *res, int type_rec, int nr, const int *ir, float *sis);
 
+/* Exact discrete adjoint of daw_submodeling (type_src = type_rec = 0) with
+ * respect to the source wavelet: g(nt) = F^T d for data d(nt,nr).  No
+ * reference counterpart -- the reference back-propagates with a
+ * sign-flipped forward scheme (submodeling2.f90:657-717), not the transpose;
+ * <F s, d> = <s, F^T d> holds here to rounding. */
+int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                              const float *d, const int *is5, const int *ir, int nr, float *g);
+
+/* Hessian of the MINRES solve: 0 (default) = the reference's operator
+ * (sign-flipped back-propagation then re-injection, submodeling2.f90:880-1217);
+ * 1 = exact: the back-propagation pass is replaced by the exact transpose of
+ * the re-injection pass, which makes the MINRES operator symmetric. */
+int daw_set_hessian(int exact);
+
 /* subwavefieldresample(n1,n2,npml,nt,dt,ir,nr,sis,wavefield),
  * subwavefieldresample.f90:1-41. */
 int daw_subwavefieldresample(int n1, int n2, int npml, int nt, float dt, const int *ir, int nr, float *sis,

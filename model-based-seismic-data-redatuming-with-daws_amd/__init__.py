@@ -34,6 +34,7 @@ EXPORTS = [
     "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
     "daw_get_stats", "daw_modeling_batch", "daw_set_tile_rows", "daw_get_tile_rows", "daw_filmartin2d_dev",
+    "daw_submodeling_transpose", "daw_set_hessian",
 ]
 
 
@@ -104,6 +105,8 @@ def lib():
         L.daw_get_stats.argtypes = [C.POINTER(Stats)]
         L.daw_modeling_batch.argtypes = [C.POINTER(ModelingArgs)]
         L.daw_set_timing.argtypes = [C.c_int]
+        L.daw_submodeling_transpose.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                                C.c_float, _f32p, _i32p, _i32p, C.c_int, _f32p]
         _lib = L
     return _lib
 
@@ -192,6 +195,24 @@ def submodeling(vp, npml, h, ofs, nt, dt, s, src_row, rec_tab, type_src=0, type_
                                  None if wf is None else wf.ctypes.data))
     sis = sis[:nr]
     return (sis, wf) if want_wavefield else sis
+
+
+def submodeling_transpose(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab):
+    """Exact discrete adjoint of submodeling with respect to the source
+    wavelet: g (nt,) = F^T d for data d (nr, nt) (no reference counterpart)."""
+    _ensure_hicks(ofs)
+    vp = _f32(vp)
+    n2, n1 = vp.shape
+    rec_tab = _i32(rec_tab)
+    g = np.zeros(nt, np.float32)
+    _check(lib().daw_submodeling_transpose(vp, n1, n2, int(npml), float(h), int(ofs), int(nt), float(dt), _f32(d),
+                                           _i32(src_row), rec_tab, rec_tab.shape[1], g))
+    return g
+
+
+def set_hessian(exact):
+    """0: the reference's Hessian-type operator; 1: exact (symmetric) variant."""
+    _check(lib().daw_set_hessian(1 if exact else 0))
 
 
 def submodeling_adjoint_esource(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, want_wavefield=False):

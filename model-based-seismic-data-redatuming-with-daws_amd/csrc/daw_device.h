@@ -46,6 +46,7 @@ struct ShotDev {
     const float *inj_cdt;
     const float *inj_amp;               // amplitude of receiver r at step it: inj_amp[r*amp_stride + it]
     int amp_stride;
+    int amp_rev;                        // amplitude time index nt-1-it (adjoint of a time-reversed input)
     int vinj;                           // 1: the source is a vz source (type_src = 1), applied in the velocity phase
     // record band: band_row[z] = slot >= 0 for rows some record window touches
     const int *band_row;

@@ -59,6 +59,7 @@ struct Global {
     int max_batch_used = 0;
     int tz = 16;               // tile rows of k_step (16 or 32)
     bool tz_auto = true;       // pick tz per batch shape with a timed probe
+    bool hess_exact = false;   // Hessian pass 1 = exact transpose of pass 2
     std::map<std::vector<int>, int> tz_cache;
 };
 static Global g;
@@ -388,6 +389,43 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
     return 0;
 }
 
+// Reverse-time sweep of the exact adjoint step (k_adj), it = nt_max-1 .. 0;
+// the trailing record of each launch reads the band of the step before it.
+static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int nrec)
+{
+    for (int s = 0; s < W.nshots; ++s) {
+        const ShotDev &S = W.host[s];
+        const size_t pe = (size_t)S.n1e * S.ldx;
+        launch_fill(S.px[0], pe, 0.0f, g.stream);
+        launch_fill(S.pz[0], pe, 0.0f, g.stream);
+        launch_fill(S.vx[0], pe, 0.0f, g.stream);
+        launch_fill(S.vz[0], pe, 0.0f, g.stream);
+    }
+    StepParams P;
+    P.shots = W.d_shots.as<ShotDev>();
+    P.tiles = W.d_tiles.as<int2>();
+    P.rec = d_rec;
+    const int rec_blocks = (nrec + kBlock - 1) / kBlock;
+    int cur = 0;
+    for (int k = 0; k <= W.nt_max; ++k) {
+        const int it = W.nt_max - 1 - k;
+        P.ntiles = (k < W.nt_max) ? W.ntiles : 0;
+        P.it = it < 0 ? 0 : it;
+        P.cur = cur;
+        const bool rec_prev = k > 0 && nrec > 0;
+        P.nrec = rec_prev ? nrec : 0;
+        P.nrec_blocks = rec_prev ? rec_blocks : 0;
+        P.rec_it = it + 1;
+        P.rec_slot = (it + 1) & 1;
+        launch_adjoint(hess, P, g.tz, g.stream);
+        cur ^= 1;
+    }
+    HIPCHK(hipGetLastError());
+    g.pts += W.pts_per_step * W.nt_max;
+    g.launches += W.nt_max + 1;
+    return 0;
+}
+
 // Per-shot device storage of one batch.
 struct ShotStore {
     DevBuf fields;   // 8 state arrays per shot
@@ -511,6 +549,7 @@ struct Batch {
     int nshots = 0, nt = 0;
     PointSet src, rec, dat;
     Inject inj_rec;
+    Inject inj_rec_c;              // receivers with coefficient c (adjoint data spreading, R^T)
     std::vector<Inject> inj_src;   // one per shot
     Band band;
     DevBuf d_rec_coef, d_dat_coef;
@@ -569,17 +608,33 @@ static int autotune_tiles(Batch &B)
 static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o)
 {
     SweepSet &W = B.W;
+    const bool exact = g.hess_exact;
     for (int s = 0; s < B.nshots; ++s) {
         ShotDev &S = W.host[s];
-        set_inject(S, B.inj_rec, amp + (size_t)s * amp_shot, B.nt, 0);
         S.wf = nullptr;
+        if (exact) {
+            // pass 1 = transpose of pass 2 applied to the time-reversed input
+            // (the reference's res is time reversed, subresiduals.f90:14)
+            set_inject(S, B.inj_rec_c, amp + (size_t)s * amp_shot, B.nt, 0);
+            S.amp_rev = 1;
+            if (o.variant == 1) { S.xlo1 = B.G.npml + 1; S.xhi1 = B.G.npml + B.G.n1 - 2; S.xlo2 = B.G.npml + 1; S.xhi2 = B.G.npml + B.G.n2 - 2; }
+            else { S.xlo1 = B.G.npml; S.xhi1 = B.G.npml + B.G.n1 - 1; S.xlo2 = B.G.npml; S.xhi2 = B.G.npml + B.G.n2 - 1; }
+        } else {
+            set_inject(S, B.inj_rec, amp + (size_t)s * amp_shot, B.nt, 0);
+            S.amp_rev = 0;
+        }
     }
     if (int rc = refresh_shots(W)) return rc;
-    if (int rc = run_sweep(kBack, W, nullptr, 0)) return rc;
+    if (exact) {
+        if (int rc = run_adjoint_sweep(true, W, nullptr, 0)) return rc;
+    } else {
+        if (int rc = run_sweep(kBack, W, nullptr, 0)) return rc;
+    }
     std::vector<RecItem> items;
     for (int s = 0; s < B.nshots; ++s) {
         ShotDev &S = W.host[s];
         S.inj_row = nullptr;
+        S.amp_rev = 0;
         S.wf = o.wf;
         if (o.variant == 1) { S.xlo1 = B.G.npml + 1; S.xhi1 = B.G.npml + B.G.n1 - 2; S.xlo2 = B.G.npml + 1; S.xhi2 = B.G.npml + B.G.n2 - 2; }
         else { S.xlo1 = B.G.npml; S.xhi1 = B.G.npml + B.G.n1 - 1; S.xlo2 = B.G.npml; S.xhi2 = B.G.npml + B.G.n2 - 1; }
@@ -631,6 +686,11 @@ static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int n
         if (int rc = make_points(B.dat, dat_tab, ndat, G, true)) return rc;
     }
     if (int rc = make_inject(B.inj_rec, B.rec, G)) return rc;
+    {
+        Grid G1 = G;
+        G1.dt = 1.0f;              // c*dt with dt = 1 is c
+        if (int rc = make_inject(B.inj_rec_c, B.rec, G1)) return rc;
+    }
     B.inj_src.clear();
     B.inj_src.resize(nshots);
     if (src_rows5) {
@@ -1138,6 +1198,53 @@ int daw_filmartin2d_dev(const float *din, int nr, int nt, float dt, float flow, 
     if (nr < 0 || nt < 1 || (nr > 0 && (!din || !dout))) return fail(DAW_ERR_ARG, "filmartin2d_dev: bad argument");
     if (int rc = ensure_stream()) return rc;
     return filter_device(din, dout, nr, nt, dt, flow, fhig, llow, lhig);
+}
+
+int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                              const float *d, const int *is5, const int *ir, int nr, float *gsrc)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_hicks()) return rc;
+    if (int rc = check_grid(n1, n2, npml, nt, h, dt)) return rc;
+    if (!vp || !is5 || !gsrc || nr < 1 || !ir || !d) return fail(DAW_ERR_ARG, "submodeling_transpose: null argument");
+    if (int rc = ensure_stream()) return rc;
+    const Grid G = make_grid(n1, n2, npml, h, dt, ofs);
+    const DevModel *M;
+    if (int rc = get_model(M, G, vp)) return rc;
+    Batch B;
+    if (int rc = build_batch(B, G, M, nt, 1, nullptr, 1, ir, nr, nullptr, 0, false)) return rc;
+    // the source's Hicks tile and band carry B^T (the source gradient)
+    PointSet src;
+    if (int rc = make_points(src, is5, 1, G, false)) return rc;
+    Band sband;
+    if (int rc = make_band(sband, {&src}, G)) return rc;
+    DevBuf d_scoef, d_d, d_g, d_band, d_items;
+    if (int rc = upload(d_scoef, src.coef)) return rc;
+    if (int rc = d_d.alloc((size_t)nr * nt * 4)) return rc;
+    HIPCHK(hipMemcpy(d_d.p, d, (size_t)nr * nt * 4, hipMemcpyHostToDevice));
+    if (int rc = d_g.alloc((size_t)nt * 4)) return rc;
+    if (int rc = d_band.alloc((size_t)2 * std::max(sband.nslot, 1) * G.ldx * 4)) return rc;
+    ShotDev &S = B.W.host[0];
+    set_inject(S, B.inj_rec_c, d_d.as<float>(), nt, 0);
+    S.amp_rev = 0;
+    S.band_row = sband.d_row.as<int>();
+    S.pband[0] = d_band.as<float>();
+    S.pband[1] = d_band.as<float>() + (size_t)std::max(sband.nslot, 1) * G.ldx;
+    std::vector<RecItem> items;
+    add_records(items, src, 0, 0, 0, d_scoef, d_g.as<float>(), nt);
+    if (int rc = upload(d_items, items)) return rc;
+    if (int rc = refresh_shots(B.W)) return rc;
+    if (int rc = run_adjoint_sweep(false, B.W, d_items.as<RecItem>(), (int)items.size())) return rc;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(gsrc, d_g.p, (size_t)nt * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int daw_set_hessian(int exact)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.hess_exact = exact != 0;
+    return 0;
 }
 
 int daw_set_tile_rows(int tz)

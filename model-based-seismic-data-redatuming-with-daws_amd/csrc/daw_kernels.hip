@@ -73,7 +73,7 @@ __device__ __forceinline__ void record_item(const StepParams &P, int k)
 // Flat per-launch view of one shot: every field a scalar or a pointer, the
 // double-buffer halves already selected, so it lives in registers.
 struct View {
-    int n1, n2, npml, n1e, n2e, ldx, nt, ofs, amp_stride, vinj;
+    int n1, n2, npml, n1e, n2e, ldx, nt, ofs, amp_stride, amp_rev, vinj;
     int xlo1, xhi1, xlo2, xhi2;
     float dt, dtdx;
     const float *kappa, *d1a, *d1a1, *d1b, *bw1, *d2a, *d2a1, *d2b, *bw2;
@@ -90,7 +90,7 @@ __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur
 {
     View v;
     v.n1 = S->n1; v.n2 = S->n2; v.npml = S->npml; v.n1e = S->n1e; v.n2e = S->n2e; v.ldx = S->ldx;
-    v.nt = S->nt; v.ofs = S->ofs; v.amp_stride = S->amp_stride; v.vinj = S->vinj;
+    v.nt = S->nt; v.ofs = S->ofs; v.amp_stride = S->amp_stride; v.amp_rev = S->amp_rev; v.vinj = S->vinj;
     v.xlo1 = S->xlo1; v.xhi1 = S->xhi1; v.xlo2 = S->xlo2; v.xhi2 = S->xhi2;
     v.dt = S->dt; v.dtdx = S->dtdx;
     v.kappa = S->kappa; v.d1a = S->d1a; v.d1a1 = S->d1a1; v.d1b = S->d1b; v.bw1 = S->bw1;
@@ -403,6 +403,185 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             S.svx[s] = vxv;
             S.svz[s] = vzv;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact discrete adjoint of one forward time step (reverse time).  The state
+// buffers hold the adjoint fields (qpx, qpz, qvx, qvz) of the step's outputs
+// and receive those of its inputs.  With x the forward step
+//   A  px1 = damp2a px + kappaw2 D-x vx,  pz1 = damp1a pz + kappaw1 D-z vz (box)
+//   B  px2 = px1 + source;  C  p = M (px2 + pz1);  D  record R p
+//   E  vx1 = damp2b vx + buw2 D+x p,  vz1 = damp1b vz + buw1 D+z p (box)
+//   F  vz2 = vz1 with row npml taking row npml+1 (1-based)
+// the kernel applies F^T, E^T, D^T, C^T, B^T, A^T.  Per tile it has the same
+// halo structure as k_step: F^T is applied while staging qvz, the adjoint
+// pressure qp = (D+x)^T(buw2 qvx) + (D+z)^T(buw1 qvz) + R^T d is formed on the
+// tile plus its ring, and (D-)^T(kappaw q) on the ring feeds qvx, qvz.
+//
+// HESS = false: transpose of submodeling (stencil box 3..n-2); the source
+//   gradient dt*kappa*qpx2 goes to the record band (B^T, summed by the
+//   record blocks with the source's Hicks tile).
+// HESS = true: transpose of the re-injection pass (box 3..n-3); the adjoint of
+//   its extended sources, -dt*kappa*qpx2 on the pressure range and -dt*qvx,
+//   -dt*qvz on the interior, is stored as the snapshot of index nt-1-it.
+// ---------------------------------------------------------------------------
+template <int TZ, bool HESS>
+__global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
+{
+    constexpr int NW = kBlock / 64;
+    constexpr int PW_ = TX + 3, PH_ = TZ + 3;
+    constexpr int VXW_ = TX + 6, VXH_ = TZ + 3;
+    constexpr int VZW_ = TX + 3, VZH_ = TZ + 6;
+    constexpr int PROWS = (PH_ + NW - 1) / NW;
+    constexpr int VROWS = TZ / NW;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if ((int)blockIdx.x < P.nrec_blocks) {
+        const int k = blockIdx.x * kBlock + tid;
+        if (k < P.nrec) record_item(P, k);
+        return;
+    }
+    __shared__ float s_qvx[VXH_ * VXW_];
+    __shared__ float s_qvz[VZH_ * VZW_];
+    __shared__ float s_wx[PH_ * PW_];
+    __shared__ float s_wz[PH_ * PW_];
+
+    const int2 t = P.tiles[blockIdx.x - P.nrec_blocks];
+    const int it = P.it;
+    const View S = make_view(P.shots + t.x, P.cur, it);
+    if (it >= S.nt) return;
+    const int n1e = S.n1e, n2e = S.n2e, ldx = S.ldx, npml = S.npml, n1 = S.n1, n2 = S.n2;
+    const int ntx = (n2e + TX - 1) / TX;
+    const int z0 = (t.y / ntx) * TZ;
+    const int x0 = (t.y % ntx) * TX;
+    const int lo = 2, hiz = HESS ? n1e - 4 : n1e - 3, hix = HESS ? n2e - 4 : n2e - 3;
+    const int gxl = x0 + lane;
+    const bool colok = gxl < n2e;
+    const int ic = S.nt - 1 - it;
+    const size_t plane = (size_t)n1 * n2;
+    const float a0 = 1.125f;
+    const float a1 = -(1.0f / 24.0f);
+    const bool fs = S.ofs == 1;
+    // F^T on load: the adjoint of vz row npml-1 (0-based) moves onto row npml
+    auto qvz_at = [&](int gz, int gx) -> float {
+        if (gz < 0 || gz >= n1e || gx < 0 || gx >= n2e) return 0.0f;
+        if (fs && gz == npml - 1) return 0.0f;
+        const float v = S.vzo[(size_t)gz * ldx + gx];
+        return (fs && gz == npml) ? v + S.vzo[(size_t)(gz - 1) * ldx + gx] : v;
+    };
+    for (int r = wv; r < VXH_; r += NW) {
+        const int gz = z0 - 1 + r;
+        s_qvx[r * VXW_ + lane + 3] = (gz >= 0 && gz < n1e && colok) ? S.vxo[(size_t)gz * ldx + gxl] : 0.0f;
+    }
+    for (int r = wv; r < VZH_; r += NW) s_qvz[r * VZW_ + lane + 1] = qvz_at(z0 - 3 + r, gxl);
+    for (int i = tid; i < VXH_ * 6; i += kBlock) {
+        const int r = i / 6, k = i - 6 * r;
+        const int c = k < 3 ? k : TX + k;
+        const int gz = z0 - 1 + r, gx = x0 - 3 + c;
+        s_qvx[r * VXW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? S.vxo[(size_t)gz * ldx + gx] : 0.0f;
+    }
+    for (int i = tid; i < VZH_ * 3; i += kBlock) {
+        const int r = i / 3, k = i - 3 * r;
+        const int c = k == 0 ? 0 : TX + k;
+        s_qvz[r * VZW_ + c] = qvz_at(z0 - 3 + r, x0 - 1 + c);
+    }
+    __syncthreads();
+
+    // ring point: qp, then the adjoint pressures and the (D-)^T weights
+    auto ring = [&](int r, int c, bool owned) {
+        const int gz = z0 - 1 + r, gx = x0 - 1 + c;
+        float wx = 0.0f, wz = 0.0f;
+        if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) {
+            const size_t g = (size_t)gz * ldx + gx;
+            const float kap = S.kappa[g];
+            // U = buw * qv on the box, zero outside
+            auto ux = [&](int xx) -> float {
+                return in_box(gz, xx, lo, hiz, hix) ? S.bw2[xx] * s_qvx[r * VXW_ + (xx - (x0 - 3))] : 0.0f;
+            };
+            auto uz = [&](int zz) -> float {
+                return in_box(zz, gx, lo, hiz, hix) ? S.bw1[zz] * s_qvz[(zz - (z0 - 3)) * VZW_ + c] : 0.0f;
+            };
+            float qp = a0 * ux(gx - 1) - a0 * ux(gx) + a1 * ux(gx - 2) - a1 * ux(gx + 1);
+            qp += a0 * uz(gz - 1) - a0 * uz(gz) + a1 * uz(gz - 2) - a1 * uz(gz + 1);
+            if (S.inj_row != nullptr) {              // R^T d: adjoint data spread with the Hicks tiles
+                const int q = S.inj_row[gz];
+                if (q >= 0) {
+                    const int *ptr = S.inj_ptr + (size_t)q * (n2e + 1);
+                    const int ia = S.amp_rev ? S.nt - 1 - it : it;
+                    for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
+                        qp += S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + ia];
+                }
+            }
+            if (fs && gz <= npml) qp = 0.0f;          // C^T: M
+            const float qpx = S.pxo[g] + qp, qpz = S.pzo[g] + qp;
+            const bool box = in_box(gz, gx, lo, hiz, hix);
+            const float tk = S.dtdx * kap;
+            wx = box ? (tk * S.d2a1[gx]) * qpx : 0.0f;
+            wz = box ? (tk * S.d1a1[gz]) * qpz : 0.0f;
+            if (owned) {
+                S.pxn[g] = box ? S.d2a[gx] * qpx : qpx;
+                S.pzn[g] = box ? S.d1a[gz] * qpz : qpz;
+                const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
+                if (HESS) {
+                    if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2)
+                        S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml)] = -S.dt * kap * qpx;
+                    else if (interior)
+                        S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml)] = 0.0f;
+                } else if (S.band_row != nullptr) {
+                    const int slot = S.band_row[gz];
+                    if (slot >= 0) S.pband[(size_t)slot * ldx + gx] = S.dt * kap * qpx;
+                }
+            }
+        }
+        s_wx[r * PW_ + c] = wx;
+        s_wz[r * PW_ + c] = wz;
+    };
+    for (int r = wv; r < PH_; r += NW) {
+        if (colok) ring(r, lane + 1, r >= 1 && r <= TZ);
+        else { s_wx[r * PW_ + lane + 1] = 0.0f; s_wz[r * PW_ + lane + 1] = 0.0f; }
+    }
+    {
+        const int sid = kBlock - 1 - tid;
+        if (sid < 3 * TZ) {
+            const int k = sid / TZ;
+            ring(1 + (sid - k * TZ), k == 0 ? 0 : TX + k, false);
+        }
+    }
+    __syncthreads();
+
+    if (!colok) return;
+    for (int j = 0; j < VROWS; ++j) {
+        const int r = wv + NW * j;
+        const int gz = z0 + r, gx = gxl;
+        if (gz >= n1e) break;
+        const size_t g = (size_t)gz * ldx + gx;
+        const float qvx = s_qvx[(r + 1) * VXW_ + lane + 3];
+        const float qvz = s_qvz[(r + 3) * VZW_ + lane + 1];
+        const bool box = in_box(gz, gx, lo, hiz, hix);
+        const float *wxr = &s_wx[(r + 1) * PW_ + lane + 1];
+        const float *wzc = &s_wz[(r + 1) * PW_ + lane + 1];
+        const float ax = a0 * wxr[0] - a0 * wxr[1] + a1 * wxr[-1] - a1 * wxr[2];          // (D-x)^T
+        const float az = a0 * wzc[0] - a0 * wzc[PW_] + a1 * wzc[-PW_] - a1 * wzc[2 * PW_]; // (D-z)^T
+        S.vxn[g] = (box ? S.d2b[gx] * qvx : qvx) + ax;
+        S.vzn[g] = (box ? S.d1b[gz] * qvz : qvz) + az;
+        if (HESS && gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2) {
+            const size_t sidx = (size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml);
+            S.svx[sidx] = -S.dt * qvx;
+            S.svz[sidx] = -S.dt * qvz;
+        }
+    }
+}
+
+void launch_adjoint(bool hess, const StepParams &P, int tz, hipStream_t st)
+{
+    const int grid = P.ntiles + P.nrec_blocks;
+    if (grid == 0) return;
+    if (hess) {
+        if (tz == 32) hipLaunchKernelGGL((k_adj<32, true>), dim3(grid), dim3(kBlock), 0, st, P);
+        else hipLaunchKernelGGL((k_adj<16, true>), dim3(grid), dim3(kBlock), 0, st, P);
+    } else {
+        if (tz == 32) hipLaunchKernelGGL((k_adj<32, false>), dim3(grid), dim3(kBlock), 0, st, P);
+        else hipLaunchKernelGGL((k_adj<16, false>), dim3(grid), dim3(kBlock), 0, st, P);
     }
 }
 

@@ -291,9 +291,12 @@ int SFX(oracle_forward)(const float *vp, int n1, int n2, int npml, float h, int 
  * wavefieldp (:793-797).  The snapshot arrays are owned here.  If
  * snap_cost is non-NULL it receives sum(0.5*sp**2) (:734-742).
  */
+static void SFX(exact_pass1)(SFX(setup_t) *S, int ofs, int nt, const R *res, int nr, const int *ir,
+                             const float *rt, int variant, R *sp, R *svx, R *svz);
+
 static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt,
                                     float dt, const R *res, int type_rec, int nr, const int *ir, R *sis,
-                                    R *wavefieldp, int variant, R *snap_cost)
+                                    R *wavefieldp, int variant, R *snap_cost, int exact)
 {
     SFX(setup_t) S;
     SFX(setup_make)(&S, vp, n1, n2, npml, h, dt);
@@ -306,7 +309,8 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
     SFX(fields_alloc)(&F, ne);
     float *rt = SFX(tiles_for)(ir, nr, ofs);
     /* pass 1 */
-    for (int it = 1; it <= nt; it++) {
+    if (exact) SFX(exact_pass1)(&S, ofs, nt, res, nr, ir, rt, variant, sp, svx, svz);
+    for (int it = 1; it <= (exact ? 0 : nt); it++) {
         SFX(p_update)(&S, F.px, F.pz, F.vx, F.vz, 3, n1e - 3, n2e - 3, -1);
         for (int r = 0; r < nr; r++)
             SFX(hicks_inject)(&S, F.px, ir + r, nr, rt + (size_t)r * 441, res[(size_t)(it - 1) + (size_t)r * nt], 1);
@@ -366,13 +370,22 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
 int SFX(oracle_esource_data)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
                              const R *res, int type_rec, int nr, const int *ir, R *sis)
 {
-    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, NULL, 0, NULL);
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, NULL, 0, NULL, 0);
+}
+
+/* Exact-Hessian variant (no reference counterpart): pass 1 is the transpose
+ * of pass 2 applied to the time-reversed input, so that
+ * Rev o esource_data_exact is symmetric. */
+int SFX(oracle_esource_data_exact)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                                   const R *res, int nr, const int *ir, R *sis)
+{
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, 0, nr, ir, sis, NULL, 0, NULL, 1);
 }
 
 int SFX(oracle_esource)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
                         const R *res, int type_rec, int nr, const int *ir, R *sis, R *wavefieldp, R *cost)
 {
-    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, wavefieldp, 1, cost);
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, wavefieldp, 1, cost, 0);
 }
 
 /* subwavefieldresample (subwavefieldresample.f90:1-41): window clipped to the
@@ -632,6 +645,89 @@ int SFX(oracle_forward_adjoint)(const float *vp, int n1, int n2, int npml, float
     free(qpx); free(qpz); free(qvx); free(qvz); free(qp); free(ux); free(uz);
     SFX(setup_free)(&S);
     return 0;
+}
+
+/* Transpose of the re-injection pass (box 3..n-3), reverse time, with the
+ * adjoint data d(it) = res(nt-it+1) spread by the Hicks tiles; the adjoints
+ * of its extended sources become the snapshots sp, svx, svz of index
+ * icount = nt-it+1. */
+static void SFX(exact_pass1)(SFX(setup_t) *S, int ofs, int nt, const R *res, int nr, const int *ir,
+                             const float *rt, int variant, R *sp, R *svx, R *svz)
+{
+    const int n1e = S->n1e, n2e = S->n2e, n1 = S->n1, n2 = S->n2, npml = S->npml;
+    const size_t ne = (size_t)n1e * n2e;
+    const R a0 = (R)1.125, a1 = -((R)1 / (R)24);
+    R *qpx = calloc(ne, sizeof(R)), *qpz = calloc(ne, sizeof(R)), *qvx = calloc(ne, sizeof(R));
+    R *qvz = calloc(ne, sizeof(R)), *qp = calloc(ne, sizeof(R)), *ux = calloc(ne, sizeof(R)), *uz = calloc(ne, sizeof(R));
+    const int lo = 3, hi1 = n1e - 3, hi2 = n2e - 3;
+    const int plo = variant == 1 ? 2 : 1, phi1 = variant == 1 ? n1 - 1 : n1, phi2 = variant == 1 ? n2 - 1 : n2;
+    for (int it = nt; it >= 1; it--) {
+        const int ic = nt - it + 1;
+        if (ofs == 1)
+            for (int i2 = 1; i2 <= n2e; i2++) {
+                E(qvz, npml + 1, i2) = E(qvz, npml + 1, i2) + E(qvz, npml, i2);
+                E(qvz, npml, i2) = 0;
+            }
+        /* vx += -dt svx(ic), vz += -dt svz(ic) on the interior */
+        for (int i2 = 1; i2 <= n2; i2++)
+            for (int i1 = 1; i1 <= n1; i1++) {
+                I3(svx, i1, i2, ic) = -S->dt * E(qvx, i1 + npml, i2 + npml);
+                I3(svz, i1, i2, ic) = -S->dt * E(qvz, i1 + npml, i2 + npml);
+            }
+        memset(qp, 0, ne * sizeof(R));
+        memset(ux, 0, ne * sizeof(R));
+        memset(uz, 0, ne * sizeof(R));
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                E(ux, i1, i2) = E(S->buw2, i1, i2) * E(qvx, i1, i2);
+                E(uz, i1, i2) = E(S->buw1, i1, i2) * E(qvz, i1, i2);
+                E(qvx, i1, i2) = S->damp2b[i2] * E(qvx, i1, i2);
+                E(qvz, i1, i2) = S->damp1b[i1] * E(qvz, i1, i2);
+            }
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                R u = E(ux, i1, i2), w = E(uz, i1, i2);
+                E(qp, i1, i2 + 1) += a0 * u; E(qp, i1, i2) -= a0 * u; E(qp, i1, i2 + 2) += a1 * u; E(qp, i1, i2 - 1) -= a1 * u;
+                E(qp, i1 + 1, i2) += a0 * w; E(qp, i1, i2) -= a0 * w; E(qp, i1 + 2, i2) += a1 * w; E(qp, i1 - 1, i2) -= a1 * w;
+            }
+        for (int r = 0; r < nr; r++) {
+            const int ix1 = ir[r], ix2 = ir[r + nr];
+            const int r1min = 1 - ix1 > -10 ? 1 - ix1 : -10, r1max = n1e - ix1 < 10 ? n1e - ix1 : 10;
+            const int r2min = 1 - ix2 > -10 ? 1 - ix2 : -10, r2max = n2e - ix2 < 10 ? n2e - ix2 : 10;
+            const R dv = res[(size_t)(nt - it) + (size_t)r * nt];
+            for (int i1 = r1min; i1 <= r1max; i1++)
+                for (int i2 = r2min; i2 <= r2max; i2++)
+                    E(qp, ix1 + i1, ix2 + i2) += (R)rt[(size_t)r * 441 + (i1 + 10) + 21 * (i2 + 10)] * dv;
+        }
+        for (int i2 = 1; i2 <= n2e; i2++)
+            for (int i1 = 1; i1 <= n1e; i1++) {
+                R q = (ofs == 1 && i1 <= npml + 1) ? (R)0 : E(qp, i1, i2);
+                E(qpx, i1, i2) += q;
+                E(qpz, i1, i2) += q;
+            }
+        /* px += -dt sp(ic) kappa on the pressure range */
+        for (int i2 = 1; i2 <= n2; i2++)
+            for (int i1 = 1; i1 <= n1; i1++) {
+                const int in = i1 >= plo && i1 <= phi1 && i2 >= plo && i2 <= phi2;
+                I3(sp, i1, i2, ic) = in ? -S->dt * E(S->kappa, i1 + npml, i2 + npml) * E(qpx, i1 + npml, i2 + npml) : (R)0;
+            }
+        memset(ux, 0, ne * sizeof(R));
+        memset(uz, 0, ne * sizeof(R));
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                E(ux, i1, i2) = E(S->kappaw2, i1, i2) * E(qpx, i1, i2);
+                E(uz, i1, i2) = E(S->kappaw1, i1, i2) * E(qpz, i1, i2);
+                E(qpx, i1, i2) = S->damp2a[i2] * E(qpx, i1, i2);
+                E(qpz, i1, i2) = S->damp1a[i1] * E(qpz, i1, i2);
+            }
+        for (int i2 = lo; i2 <= hi2; i2++)
+            for (int i1 = lo; i1 <= hi1; i1++) {
+                R u = E(ux, i1, i2), w = E(uz, i1, i2);
+                E(qvx, i1, i2) += a0 * u; E(qvx, i1, i2 - 1) -= a0 * u; E(qvx, i1, i2 + 1) += a1 * u; E(qvx, i1, i2 - 2) -= a1 * u;
+                E(qvz, i1, i2) += a0 * w; E(qvz, i1 - 1, i2) -= a0 * w; E(qvz, i1 + 1, i2) += a1 * w; E(qvz, i1 - 2, i2) -= a1 * w;
+            }
+    }
+    free(qpx); free(qpz); free(qvx); free(qvz); free(qp); free(ux); free(uz);
 }
 
 #undef E

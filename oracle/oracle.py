@@ -63,6 +63,10 @@ def lib():
             f = getattr(L, "oracle_resample_" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, C.c_int, C.c_int, rp, rp]
             f.restype = C.c_int
+            f = getattr(L, "oracle_esource_data_exact_" + sfx)
+            f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, rp, C.c_int,
+                          _i32p, rp]
+            f.restype = C.c_int
             f = getattr(L, "oracle_forward_adjoint_" + sfx)
             f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, rp, _i32p,
                           _i32p, C.c_int, rp]
@@ -172,6 +176,20 @@ def esource_data(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, prec="f32")
                                                         rec_tab, sis)
     if rc != 0:
         raise RuntimeError("oracle esource_data failed (out of memory?)")
+    return sis
+
+
+def esource_data_exact(vp, npml, h, ofs, nt, dt, res, rec_tab, prec="f64"):
+    """Hessian-type operator with pass 1 the exact transpose of pass 2."""
+    vp = _c(vp, np.float32)
+    n2, n1 = vp.shape
+    rec_tab = _c(rec_tab, np.int32)
+    nr = rec_tab.shape[1]
+    sis = np.zeros((nr, nt), _rdt(prec))
+    rc = getattr(lib(), "oracle_esource_data_exact_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, _c(res, _rdt(prec)), nr,
+                                                              rec_tab, sis)
+    if rc != 0:
+        raise RuntimeError("oracle esource_data_exact failed")
     return sis
 
 

@@ -210,3 +210,22 @@ def test_oracle_equals_reference_random():
         a = O.subsourcekws(z, x, h, 1)
         b = O.ref_subsourcekws(z, x, h, 1)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_exact_adjoint_dot_product():
+    """FP64 dot test of the exact transpose of the forward scheme, and symmetry
+    of the exact Hessian (time reversal o esource_data_exact)."""
+    for ofs in (1, 0):
+        c = Case(nz=26, nx=50, nt=120, ofs=ofs, nrec=9)
+        rng = np.random.default_rng(0)
+        s = rng.standard_normal(c.nt).astype(np.float32)
+        d = rng.standard_normal((c.rec.shape[1], c.nt))
+        Fs = O.forward(c.vp, c.npml, c.h, ofs, c.nt, c.dt, s, c.src, c.rec, prec="f64")
+        g = O.forward_adjoint(c.vp, c.npml, c.h, ofs, c.nt, c.dt, d, c.src, c.rec, prec="f64")
+        lhs, rhs = (Fs * d).sum(), (s.astype(np.float64) * g).sum()
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        x = rng.standard_normal(d.shape)
+        A = lambda v: O.esource_data_exact(c.vp0, c.npml, c.h, ofs, c.nt, c.dt, v, c.rec)[:, ::-1]
+        Ax, Ad = A(x), A(d)
+        assert abs((x * Ad).sum() - (Ax * d).sum()) <= 1e-12 * abs((x * Ad).sum())
+        assert (x * Ax).sum() > 0
