@@ -185,11 +185,12 @@ typedef struct daw_stats {
 } daw_stats;
 
 int daw_set_timing(int on);
-/* Rows per step-kernel tile: 16 or 32 (tiles are 64 columns wide), or 0 to
- * let daw_redatum_batch time both on a short probe sweep per batch shape and
- * keep the faster (the default).  No reference counterpart. */
-int daw_set_tile_rows(int tz);
-int daw_get_tile_rows(void);
+/* Step-kernel tile (rows x columns): 8x128, 16x64, 16x128, 32x64, 32x128 or
+ * 64x64; rows = 0 lets daw_redatum_batch time every shape on a short probe
+ * sweep per batch shape and keep the fastest (the default).  No reference
+ * counterpart: the launch shape of the autotuner. */
+int daw_set_tile(int rows, int cols);
+int daw_get_tile(int *rows, int *cols);
 int daw_reset_stats(void);
 int daw_get_stats(daw_stats *st);
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "daw_subsourcekws", "daw_subacqui", "daw_submodeling", "daw_submodeling_adjoint_esource",
     "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
-    "daw_get_stats", "daw_modeling_batch", "daw_set_tile_rows", "daw_get_tile_rows", "daw_filmartin2d_dev",
+    "daw_get_stats", "daw_modeling_batch", "daw_set_tile", "daw_get_tile", "daw_filmartin2d_dev",
     "daw_submodeling_transpose", "daw_set_hessian",
 ]
 
@@ -398,13 +398,16 @@ def filmartin2d_dev(din_dev, dout_dev, dt, flow, fhig, llow, lhig):
                                      C.c_void_p(dout_dev.data_ptr())))
 
 
-def set_tile_rows(tz):
-    """16 or 32 rows per step-kernel tile, or 0 for the autotuner (default)."""
-    _check(lib().daw_set_tile_rows(int(tz)))
+def set_tile(rows, cols=64):
+    """Step-kernel tile rows x cols (8x128, 16x64, 16x128, 32x64, 32x128,
+    64x64), or rows = 0 for the autotuner (default)."""
+    _check(lib().daw_set_tile(int(rows), int(cols)))
 
 
-def get_tile_rows():
-    return int(lib().daw_get_tile_rows())
+def get_tile():
+    r, c = C.c_int(), C.c_int()
+    _check(lib().daw_get_tile(C.byref(r), C.byref(c)))
+    return r.value, c.value
 
 
 def reset_stats():

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -57,10 +58,11 @@ struct Global {
     long long pts = 0;         // point-updates executed (sum n1e*n2e per shot per step)
     long long launches = 0;
     int max_batch_used = 0;
-    int tz = 16;               // tile rows of k_step (16 or 32)
+    int tz = 16;               // tile rows of k_step
+    int txw = 1;               // tile width in wavefronts (64 columns each)
     bool tz_auto = true;       // pick tz per batch shape with a timed probe
     bool hess_exact = false;   // Hessian pass 1 = exact transpose of pass 2
-    std::map<std::vector<int>, int> tz_cache;
+    std::map<std::vector<int>, std::pair<int, int>> tz_cache;
 };
 static Global g;
 
@@ -320,7 +322,7 @@ static int finalize_sweepset(SweepSet &W)
     W.pts_per_step = 0;
     for (int s = 0; s < W.nshots; ++s) {
         const ShotDev &S = W.host[s];
-        tps[s] = ((S.n1e + g.tz - 1) / g.tz) * ((S.n2e + TX - 1) / TX);
+        tps[s] = ((S.n1e + g.tz - 1) / g.tz) * ((S.n2e + TX * g.txw - 1) / (TX * g.txw));
         W.nt_max = std::max(W.nt_max, S.nt);
         W.pts_per_step += (long long)S.n1e * S.n2e;
     }
@@ -370,7 +372,7 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
         P.nrec_blocks = rec_prev ? rec_blocks : 0;
         P.rec_it = it - 1;
         P.rec_slot = (it - 1) & 1;
-        launch_step(mode, P, g.tz, g.stream);
+        launch_step(mode, P, g.tz, g.txw, g.stream);
         cur ^= 1;
     }
     HIPCHK(hipGetLastError());
@@ -417,7 +419,7 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
         P.nrec_blocks = rec_prev ? rec_blocks : 0;
         P.rec_it = it + 1;
         P.rec_slot = (it + 1) & 1;
-        launch_adjoint(hess, P, g.tz, g.stream);
+        launch_adjoint(hess, P, g.tz, g.txw, g.stream);
         cur ^= 1;
     }
     HIPCHK(hipGetLastError());
@@ -558,33 +560,41 @@ struct Batch {
 };
 
 // Tile autotuner: time a short back-propagation probe (zero residuals) at
-// every candidate tile height and keep the fastest for this batch shape.
+// every compiled tile shape and keep the fastest for this batch shape.
+// DAW_TUNE_LOG=1 prints the probe times to stderr.
 static int autotune_tiles(Batch &B)
 {
     if (!g.tz_auto) return 0;
     const std::vector<int> key{B.G.n1e, B.G.n2e, B.nshots};
     auto hit = g.tz_cache.find(key);
     if (hit != g.tz_cache.end()) {
-        if (hit->second != g.tz) { g.tz = hit->second; return finalize_sweepset(B.W); }
+        if (hit->second.first != g.tz || hit->second.second != g.txw) {
+            g.tz = hit->second.first;
+            g.txw = hit->second.second;
+            return finalize_sweepset(B.W);
+        }
         return 0;
     }
     if (!B.store.snap.p) return 0;
     DevBuf zeros;
     if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
     launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
-    for (auto &S : B.W.host) set_inject(S, B.inj_rec, zeros.as<float>(), 0, 0);
+    for (auto &S : B.W.host) { set_inject(S, B.inj_rec, zeros.as<float>(), 0, 0); S.amp_rev = 0; }
+    if (int rc = refresh_shots(B.W)) return rc;
     const bool timing = g.timing;
-    const double pts = g.pts;
-    const long long launches = g.launches;
+    const long long pts = g.pts, launches = g.launches;
     g.timing = false;
     const int probe = std::min(24, B.nt);
-    int best = 16;
+    std::pair<int, int> best{16, 1};
     float best_ms = 1e30f;
+    const bool log = getenv("DAW_TUNE_LOG") != nullptr;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    for (int tz : {16, 32}) {
-        g.tz = tz;
+    static const int shapes[][2] = {{8, 2}, {16, 1}, {16, 2}, {32, 1}, {32, 2}, {64, 1}};
+    for (const auto &sh : shapes) {
+        g.tz = sh[0];
+        g.txw = sh[1];
         if (int rc = finalize_sweepset(B.W)) return rc;
         if (int rc = run_sweep(kBack, B.W, nullptr, 0, 4)) return rc;       // warm
         HIPCHK(hipEventRecord(e0, g.stream));
@@ -593,7 +603,9 @@ static int autotune_tiles(Batch &B)
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = tz; }
+        if (log) fprintf(stderr, "[daw tune] %d shots %dx%d grid: tile %dx%d  %.1f us/step\n", B.nshots, B.G.n1e,
+                         B.G.n2e, sh[0], 64 * sh[1], 1000.0f * ms / (probe + 1));
+        if (ms < best_ms) { best_ms = ms; best = {sh[0], sh[1]}; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -601,7 +613,8 @@ static int autotune_tiles(Batch &B)
     g.pts = pts;
     g.launches = launches;
     g.tz_cache[key] = best;
-    g.tz = best;
+    g.tz = best.first;
+    g.txw = best.second;
     return finalize_sweepset(B.W);
 }
 
@@ -1247,19 +1260,24 @@ int daw_set_hessian(int exact)
     return 0;
 }
 
-int daw_set_tile_rows(int tz)
+int daw_set_tile(int rows, int cols)
 {
-    if (tz != 0 && tz != 16 && tz != 32) return fail(DAW_ERR_ARG, "tile rows must be 0 (autotune), 16 or 32");
     std::lock_guard<std::mutex> lk(g.mu);
-    g.tz_auto = tz == 0;
-    if (tz) g.tz = tz;
+    if (rows == 0) { g.tz_auto = true; return 0; }
+    if (cols % 64 != 0 || !tile_shape_ok(rows, cols / 64))
+        return fail(DAW_ERR_ARG, "tile %dx%d has no compiled kernel (8x128, 16x64, 16x128, 32x64, 32x128, 64x64)", rows, cols);
+    g.tz_auto = false;
+    g.tz = rows;
+    g.txw = cols / 64;
     return 0;
 }
 
-int daw_get_tile_rows(void)
+int daw_get_tile(int *rows, int *cols)
 {
     std::lock_guard<std::mutex> lk(g.mu);
-    return g.tz;
+    if (rows) *rows = g.tz;
+    if (cols) *cols = 64 * g.txw;
+    return 0;
 }
 
 int daw_set_timing(int on)

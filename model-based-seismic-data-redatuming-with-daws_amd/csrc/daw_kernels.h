@@ -6,10 +6,12 @@
 
 namespace daw {
 
-void launch_step(int mode, const StepParams &P, int tz, hipStream_t st);
+void launch_step(int mode, const StepParams &P, int tz, int txw, hipStream_t st);
+// Tile shape (rows tz, 64*txw columns) has compiled kernels.
+bool tile_shape_ok(int tz, int txw);
 // Exact adjoint of one step (reverse time); hess selects the transpose of
 // the re-injection pass (snapshots out) over that of the forward sweep.
-void launch_adjoint(bool hess, const StepParams &P, int tz, hipStream_t st);
+void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st);
 void launch_fill(float *p, size_t n, float v, hipStream_t st);
 void launch_residuals(const float *dobs, const float *dcal, float *res, int nt, size_t ntr, hipStream_t st);
 void launch_add(const float *a, const float *b, float *c, size_t n, hipStream_t st);

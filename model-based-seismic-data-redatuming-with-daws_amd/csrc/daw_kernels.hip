@@ -182,16 +182,18 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
 //   3. velocity half-step of the owned points, vz source or snapshot
 //      re-injection, the vz(npml) = vz(npml+1) mirror, writes.
 // ---------------------------------------------------------------------------
-template <int MODE, int TZ>
+template <int MODE, int TZ, int TXW>
 __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
 {
     constexpr int NW = kBlock / 64;
-    constexpr int PW_ = TX + 3, PH_ = TZ + 3;
-    constexpr int VXW_ = TX + 6, VXH_ = TZ + 3;
-    constexpr int VZW_ = TX + 3, VZH_ = TZ + 6;
-    constexpr int PROWS = (PH_ + NW - 1) / NW;
-    constexpr int VROWS = TZ / NW;
-    static_assert(TZ % NW == 0, "tile rows must split evenly over the wavefronts");
+    constexpr int TXT = 64 * TXW;          // tile width: TXW wavefronts side by side
+    constexpr int NWZ = NW / TXW;          // wavefront rows
+    constexpr int PW_ = TXT + 3, PH_ = TZ + 3;
+    constexpr int VXW_ = TXT + 6, VXH_ = TZ + 3;
+    constexpr int VZW_ = TXT + 3, VZH_ = TZ + 6;
+    constexpr int PROWS = (PH_ + NWZ - 1) / NWZ;
+    constexpr int VROWS = TZ / NWZ;
+    static_assert(TZ % NWZ == 0 && 3 * TZ <= kBlock, "tile rows must split evenly over the wavefront rows");
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if ((int)blockIdx.x < P.nrec_blocks) {
         const int k = blockIdx.x * kBlock + tid;
@@ -207,9 +209,9 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const View S = make_view(P.shots + t.x, P.cur, it);   // registers: stores below cannot alias it
     if (it >= S.nt) return;
     const int n1e = S.n1e, n2e = S.n2e, ldx = S.ldx;
-    const int ntx = (n2e + TX - 1) / TX;
+    const int ntx = (n2e + TXT - 1) / TXT;
     const int z0 = (t.y / ntx) * TZ;
-    const int x0 = (t.y % ntx) * TX;
+    const int x0 = (t.y % ntx) * TXT;
     // stencil bounds: 3..n-2 forward (submodeling2.f90:148-149), 3..n-3 for
     // both passes of the Hessian-type sweeps (:1014-1015, :1108-1109)
     const int lo = 2;
@@ -224,49 +226,61 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     float *__restrict__ vxn = S.vxn;
     float *__restrict__ vzn = S.vzn;
     const int ic = S.nt - 1 - it;   // time-reversed snapshot index (icount = nt-it+1)
-    const int gxl = x0 + lane;       // this lane's column
+    const int cl = (wv % TXW) * 64 + lane;   // this lane's column in the tile
+    const int rz = wv / TXW;                  // this wavefront's first row
+    const int gxl = x0 + cl;
     const bool colok = gxl < n2e;
 
     // ---- 1. every global load of the step is issued here, into registers,
     // before anything waits on one: velocity halos, the pressure rectangle,
     // and (re-injection) the snapshot values
-    constexpr int XR = (VXH_ + NW - 1) / NW, ZR = (VZH_ + NW - 1) / NW;
+    constexpr int XR = (VXH_ + NWZ - 1) / NWZ, ZR = (VZH_ + NWZ - 1) / NWZ;
     const int npml = S.npml, n1 = S.n1, n2 = S.n2;
     const size_t plane = (size_t)n1 * n2;
     float vxl[XR], vzl[ZR];
 #pragma unroll
     for (int j = 0; j < XR; ++j) {
-        const int r = wv + NW * j, gz = z0 - 1 + r;
+        const int r = rz + NWZ * j, gz = z0 - 1 + r;
         vxl[j] = (r < VXH_ && gz >= 0 && gz < n1e && colok) ? vxo[(size_t)gz * ldx + gxl] : 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < ZR; ++j) {
-        const int r = wv + NW * j, gz = z0 - 3 + r;
+        const int r = rz + NWZ * j, gz = z0 - 3 + r;
         vzl[j] = (r < VZH_ && gz >= 0 && gz < n1e && colok) ? vzo[(size_t)gz * ldx + gxl] : 0.0f;
     }
-    float hx = 0.0f, hz = 0.0f;
-    int hx_i = -1, hz_i = -1;
-    if (tid < VXH_ * 6) {
-        const int r = tid / 6, k = tid - 6 * r;
-        const int c = k < 3 ? k : TX + k;
-        const int gz = z0 - 1 + r, gx = x0 - 3 + c;
-        hx_i = r * VXW_ + c;
-        if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hx = vxo[(size_t)gz * ldx + gx];
+    constexpr int HXN = (VXH_ * 6 + kBlock - 1) / kBlock, HZN = (VZH_ * 3 + kBlock - 1) / kBlock;
+    float hx[HXN], hz[HZN];
+    int hx_i[HXN], hz_i[HZN];
+#pragma unroll
+    for (int q = 0; q < HXN; ++q) {
+        const int i = tid + q * kBlock;
+        hx[q] = 0.0f;
+        hx_i[q] = -1;
+        if (i < VXH_ * 6) {
+            const int r = i / 6, k = i - 6 * r;
+            const int c = k < 3 ? k : TXT + k;
+            const int gz = z0 - 1 + r, gx = x0 - 3 + c;
+            hx_i[q] = r * VXW_ + c;
+            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hx[q] = vxo[(size_t)gz * ldx + gx];
+        }
     }
-    {
-        const int t2 = kBlock - 1 - tid;
-        if (t2 < VZH_ * 3) {
-            const int r = t2 / 3, k = t2 - 3 * r;
-            const int c = k == 0 ? 0 : TX + k;
+#pragma unroll
+    for (int q = 0; q < HZN; ++q) {
+        const int i = (kBlock - 1 - tid) + q * kBlock;
+        hz[q] = 0.0f;
+        hz_i[q] = -1;
+        if (i < VZH_ * 3) {
+            const int r = i / 3, k = i - 3 * r;
+            const int c = k == 0 ? 0 : TXT + k;
             const int gz = z0 - 3 + r, gx = x0 - 1 + c;
-            hz_i = r * VZW_ + c;
-            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hz = vzo[(size_t)gz * ldx + gx];
+            hz_i[q] = r * VZW_ + c;
+            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hz[q] = vzo[(size_t)gz * ldx + gx];
         }
     }
     float pxr[PROWS], pzr[PROWS], kar[PROWS], spr[PROWS];
 #pragma unroll
     for (int j = 0; j < PROWS; ++j) {
-        const int r = wv + NW * j;
+        const int r = rz + NWZ * j;
         const int gz = z0 - 1 + r;
         pxr[j] = 0.0f; pzr[j] = 0.0f; kar[j] = 0.0f; spr[j] = 0.0f;
         if (r < PH_ && gz >= 0 && gz < n1e && colok) {
@@ -287,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     if (side) {
         const int k = sid / TZ;
         s_r = 1 + (sid - k * TZ);
-        s_c = k == 0 ? 0 : TX + k;
+        s_c = k == 0 ? 0 : TXT + k;
         s_gz = z0 - 1 + s_r;
         s_gx = x0 - 1 + s_c;
         if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
@@ -302,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     float svxr[VROWS], svzr[VROWS];
 #pragma unroll
     for (int j = 0; j < VROWS; ++j) {
-        const int gz = z0 + wv + NW * j;
+        const int gz = z0 + rz + NWZ * j;
         svxr[j] = 0.0f; svzr[j] = 0.0f;
         if (MODE == kReinject && colok && gz >= npml && gz < npml + n1 && gxl >= npml && gxl < npml + n2) {
             const size_t sidx = (size_t)ic * plane + (size_t)(gz - npml) * n2 + (gxl - npml);
@@ -312,33 +326,37 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     }
 #pragma unroll
     for (int j = 0; j < XR; ++j) {
-        const int r = wv + NW * j;
-        if (r < VXH_) s_vx[r * VXW_ + lane + 3] = vxl[j];
+        const int r = rz + NWZ * j;
+        if (r < VXH_) s_vx[r * VXW_ + cl + 3] = vxl[j];
     }
 #pragma unroll
     for (int j = 0; j < ZR; ++j) {
-        const int r = wv + NW * j;
-        if (r < VZH_) s_vz[r * VZW_ + lane + 1] = vzl[j];
+        const int r = rz + NWZ * j;
+        if (r < VZH_) s_vz[r * VZW_ + cl + 1] = vzl[j];
     }
-    if (hx_i >= 0) s_vx[hx_i] = hx;
-    if (hz_i >= 0) s_vz[hz_i] = hz;
+#pragma unroll
+    for (int q = 0; q < HXN; ++q)
+        if (hx_i[q] >= 0) s_vx[hx_i[q]] = hx[q];
+#pragma unroll
+    for (int q = 0; q < HZN; ++q)
+        if (hz_i[q] >= 0) s_vz[hz_i[q]] = hz[q];
     const float c_d2a = colok ? S.d2a[gxl] : 1.0f, c_d2a1 = colok ? S.d2a1[gxl] : 1.0f;
     __syncthreads();
 
     // ---- 2. pressure half-step
 #pragma unroll
     for (int j = 0; j < PROWS; ++j) {
-        const int r = wv + NW * j;
+        const int r = rz + NWZ * j;
         if (r < PH_) {
             const int gz = z0 - 1 + r;
             float p = 0.0f;
             if (gz >= 0 && gz < n1e && colok) {
                 const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[gz] : -1;
-                p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + lane + 3],
-                                        &s_vz[(r + 2) * VZW_ + lane + 1], S.d1a[gz], S.d1a1[gz], c_d2a, c_d2a1,
+                p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + cl + 3],
+                                        &s_vz[(r + 2) * VZW_ + cl + 1], S.d1a[gz], S.d1a1[gz], c_d2a, c_d2a1,
                                         injq, r >= 1 && r <= TZ, lo, hiz, hix, spr[j], pxn, pzn);
             }
-            s_p[r * PW_ + lane + 1] = p;
+            s_p[r * PW_ + cl + 1] = p;
         }
     }
     if (side) {
@@ -360,14 +378,14 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const float c_d2b = S.d2b[gxl], c_bw2 = S.bw2[gxl];
 #pragma unroll
     for (int j = 0; j < VROWS; ++j) {
-        const int r = wv + NW * j;
+        const int r = rz + NWZ * j;
         const int gz = z0 + r, gx = gxl;
         if (gz >= n1e) break;
         const size_t g = (size_t)gz * ldx + gx;
-        float vxv = s_vx[(r + 1) * VXW_ + lane + 3];
-        float vzv = s_vz[(r + 3) * VZW_ + lane + 1];
+        float vxv = s_vx[(r + 1) * VXW_ + cl + 3];
+        float vzv = s_vz[(r + 3) * VZW_ + cl + 1];
         if (in_box(gz, gx, lo, hiz, hix)) {
-            const float *pp = &s_p[(r + 1) * PW_ + lane + 1];         // p(gz, gx)
+            const float *pp = &s_p[(r + 1) * PW_ + cl + 1];         // p(gz, gx)
             const float der2 = a0 * (pp[1] - pp[0]) + a1 * (pp[2] - pp[-1]);
             const float der1 = a0 * (pp[PW_] - pp[0]) + a1 * (pp[2 * PW_] - pp[-PW_]);
             if (MODE == kBack) {
@@ -426,15 +444,16 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
 //   its extended sources, -dt*kappa*qpx2 on the pressure range and -dt*qvx,
 //   -dt*qvz on the interior, is stored as the snapshot of index nt-1-it.
 // ---------------------------------------------------------------------------
-template <int TZ, bool HESS>
+template <int TZ, int TXW, bool HESS>
 __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
 {
     constexpr int NW = kBlock / 64;
-    constexpr int PW_ = TX + 3, PH_ = TZ + 3;
-    constexpr int VXW_ = TX + 6, VXH_ = TZ + 3;
-    constexpr int VZW_ = TX + 3, VZH_ = TZ + 6;
-    constexpr int PROWS = (PH_ + NW - 1) / NW;
-    constexpr int VROWS = TZ / NW;
+    constexpr int TXT = 64 * TXW;
+    constexpr int NWZ = NW / TXW;
+    constexpr int PW_ = TXT + 3, PH_ = TZ + 3;
+    constexpr int VXW_ = TXT + 6, VXH_ = TZ + 3;
+    constexpr int VZW_ = TXT + 3, VZH_ = TZ + 6;
+    constexpr int VROWS = TZ / NWZ;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if ((int)blockIdx.x < P.nrec_blocks) {
         const int k = blockIdx.x * kBlock + tid;
@@ -451,11 +470,13 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     const View S = make_view(P.shots + t.x, P.cur, it);
     if (it >= S.nt) return;
     const int n1e = S.n1e, n2e = S.n2e, ldx = S.ldx, npml = S.npml, n1 = S.n1, n2 = S.n2;
-    const int ntx = (n2e + TX - 1) / TX;
+    const int ntx = (n2e + TXT - 1) / TXT;
     const int z0 = (t.y / ntx) * TZ;
-    const int x0 = (t.y % ntx) * TX;
+    const int x0 = (t.y % ntx) * TXT;
     const int lo = 2, hiz = HESS ? n1e - 4 : n1e - 3, hix = HESS ? n2e - 4 : n2e - 3;
-    const int gxl = x0 + lane;
+    const int cl = (wv % TXW) * 64 + lane;
+    const int rz = wv / TXW;
+    const int gxl = x0 + cl;
     const bool colok = gxl < n2e;
     const int ic = S.nt - 1 - it;
     const size_t plane = (size_t)n1 * n2;
@@ -469,20 +490,20 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         const float v = S.vzo[(size_t)gz * ldx + gx];
         return (fs && gz == npml) ? v + S.vzo[(size_t)(gz - 1) * ldx + gx] : v;
     };
-    for (int r = wv; r < VXH_; r += NW) {
+    for (int r = rz; r < VXH_; r += NWZ) {
         const int gz = z0 - 1 + r;
-        s_qvx[r * VXW_ + lane + 3] = (gz >= 0 && gz < n1e && colok) ? S.vxo[(size_t)gz * ldx + gxl] : 0.0f;
+        s_qvx[r * VXW_ + cl + 3] = (gz >= 0 && gz < n1e && colok) ? S.vxo[(size_t)gz * ldx + gxl] : 0.0f;
     }
-    for (int r = wv; r < VZH_; r += NW) s_qvz[r * VZW_ + lane + 1] = qvz_at(z0 - 3 + r, gxl);
+    for (int r = rz; r < VZH_; r += NWZ) s_qvz[r * VZW_ + cl + 1] = qvz_at(z0 - 3 + r, gxl);
     for (int i = tid; i < VXH_ * 6; i += kBlock) {
         const int r = i / 6, k = i - 6 * r;
-        const int c = k < 3 ? k : TX + k;
+        const int c = k < 3 ? k : TXT + k;
         const int gz = z0 - 1 + r, gx = x0 - 3 + c;
         s_qvx[r * VXW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? S.vxo[(size_t)gz * ldx + gx] : 0.0f;
     }
     for (int i = tid; i < VZH_ * 3; i += kBlock) {
         const int r = i / 3, k = i - 3 * r;
-        const int c = k == 0 ? 0 : TX + k;
+        const int c = k == 0 ? 0 : TXT + k;
         s_qvz[r * VZW_ + c] = qvz_at(z0 - 3 + r, x0 - 1 + c);
     }
     __syncthreads();
@@ -536,30 +557,30 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         s_wx[r * PW_ + c] = wx;
         s_wz[r * PW_ + c] = wz;
     };
-    for (int r = wv; r < PH_; r += NW) {
-        if (colok) ring(r, lane + 1, r >= 1 && r <= TZ);
-        else { s_wx[r * PW_ + lane + 1] = 0.0f; s_wz[r * PW_ + lane + 1] = 0.0f; }
+    for (int r = rz; r < PH_; r += NWZ) {
+        if (colok) ring(r, cl + 1, r >= 1 && r <= TZ);
+        else { s_wx[r * PW_ + cl + 1] = 0.0f; s_wz[r * PW_ + cl + 1] = 0.0f; }
     }
     {
         const int sid = kBlock - 1 - tid;
         if (sid < 3 * TZ) {
             const int k = sid / TZ;
-            ring(1 + (sid - k * TZ), k == 0 ? 0 : TX + k, false);
+            ring(1 + (sid - k * TZ), k == 0 ? 0 : TXT + k, false);
         }
     }
     __syncthreads();
 
     if (!colok) return;
     for (int j = 0; j < VROWS; ++j) {
-        const int r = wv + NW * j;
+        const int r = rz + NWZ * j;
         const int gz = z0 + r, gx = gxl;
         if (gz >= n1e) break;
         const size_t g = (size_t)gz * ldx + gx;
-        const float qvx = s_qvx[(r + 1) * VXW_ + lane + 3];
-        const float qvz = s_qvz[(r + 3) * VZW_ + lane + 1];
+        const float qvx = s_qvx[(r + 1) * VXW_ + cl + 3];
+        const float qvz = s_qvz[(r + 3) * VZW_ + cl + 1];
         const bool box = in_box(gz, gx, lo, hiz, hix);
-        const float *wxr = &s_wx[(r + 1) * PW_ + lane + 1];
-        const float *wzc = &s_wz[(r + 1) * PW_ + lane + 1];
+        const float *wxr = &s_wx[(r + 1) * PW_ + cl + 1];
+        const float *wzc = &s_wz[(r + 1) * PW_ + cl + 1];
         const float ax = a0 * wxr[0] - a0 * wxr[1] + a1 * wxr[-1] - a1 * wxr[2];          // (D-x)^T
         const float az = a0 * wzc[0] - a0 * wzc[PW_] + a1 * wzc[-PW_] - a1 * wzc[2 * PW_]; // (D-z)^T
         S.vxn[g] = (box ? S.d2b[gx] * qvx : qvx) + ax;
@@ -572,34 +593,51 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     }
 }
 
-void launch_adjoint(bool hess, const StepParams &P, int tz, hipStream_t st)
+// Tile shapes the autotuner may pick: (rows, wavefronts across x).
+#define DAW_TILE_SHAPES(X) X(8, 2) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(64, 1)
+
+void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st)
 {
     const int grid = P.ntiles + P.nrec_blocks;
     if (grid == 0) return;
-    if (hess) {
-        if (tz == 32) hipLaunchKernelGGL((k_adj<32, true>), dim3(grid), dim3(kBlock), 0, st, P);
-        else hipLaunchKernelGGL((k_adj<16, true>), dim3(grid), dim3(kBlock), 0, st, P);
-    } else {
-        if (tz == 32) hipLaunchKernelGGL((k_adj<32, false>), dim3(grid), dim3(kBlock), 0, st, P);
-        else hipLaunchKernelGGL((k_adj<16, false>), dim3(grid), dim3(kBlock), 0, st, P);
+#define DAW_ADJ(Z, W)                                                                                          \
+    if (tz == Z && txw == W) {                                                                                 \
+        if (hess) hipLaunchKernelGGL((k_adj<Z, W, true>), dim3(grid), dim3(kBlock), 0, st, P);                 \
+        else hipLaunchKernelGGL((k_adj<Z, W, false>), dim3(grid), dim3(kBlock), 0, st, P);                     \
+        return;                                                                                                \
     }
+    DAW_TILE_SHAPES(DAW_ADJ)
+#undef DAW_ADJ
 }
 
 template <int MODE>
-static void launch_mode(const StepParams &P, int tz, hipStream_t st)
+static void launch_mode(const StepParams &P, int tz, int txw, hipStream_t st)
 {
     const int grid = P.ntiles + P.nrec_blocks;
-    if (tz == 32) hipLaunchKernelGGL((k_step<MODE, 32>), dim3(grid), dim3(kBlock), 0, st, P);
-    else hipLaunchKernelGGL((k_step<MODE, 16>), dim3(grid), dim3(kBlock), 0, st, P);
+#define DAW_STEP(Z, W)                                                                                         \
+    if (tz == Z && txw == W) {                                                                                 \
+        hipLaunchKernelGGL((k_step<MODE, Z, W>), dim3(grid), dim3(kBlock), 0, st, P);                          \
+        return;                                                                                                \
+    }
+    DAW_TILE_SHAPES(DAW_STEP)
+#undef DAW_STEP
 }
 
-void launch_step(int mode, const StepParams &P, int tz, hipStream_t st)
+bool tile_shape_ok(int tz, int txw)
+{
+#define DAW_OK_SHAPE(Z, W) if (tz == Z && txw == W) return true;
+    DAW_TILE_SHAPES(DAW_OK_SHAPE)
+#undef DAW_OK_SHAPE
+    return false;
+}
+
+void launch_step(int mode, const StepParams &P, int tz, int txw, hipStream_t st)
 {
     const int grid = P.ntiles + P.nrec_blocks;
     if (grid == 0) return;
-    if (mode == kFwd) launch_mode<kFwd>(P, tz, st);
-    else if (mode == kBack) launch_mode<kBack>(P, tz, st);
-    else launch_mode<kReinject>(P, tz, st);
+    if (mode == kFwd) launch_mode<kFwd>(P, tz, txw, st);
+    else if (mode == kBack) launch_mode<kBack>(P, tz, txw, st);
+    else launch_mode<kReinject>(P, tz, txw, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -55,3 +55,19 @@ def test_exact_hessian(gpu_daw):
     # the default operator is the reference's one again
     Hd = gpu_daw.submodeling_adjoint_esource_data(c.vp0, c.npml, c.h, 1, c.nt, c.dt, x, c.rec)
     assert np.array_equal(Hd, O.esource_data(c.vp0, c.npml, c.h, 1, c.nt, c.dt, x, c.rec))
+
+
+def test_transpose_tile_shapes(gpu_daw):
+    """The adjoint step gives the same result for every tile shape."""
+    c = Case(nz=45, nx=150, nt=120, nrec=11)
+    rng = np.random.default_rng(2)
+    d = rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
+    outs = []
+    try:
+        for tile in [(8, 128), (16, 64), (16, 128), (32, 64), (32, 128), (64, 64)]:
+            gpu_daw.set_tile(*tile)
+            outs.append(gpu_daw.submodeling_transpose(c.vp, c.npml, c.h, 1, c.nt, c.dt, d, c.src, c.rec))
+    finally:
+        gpu_daw.set_tile(0)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])

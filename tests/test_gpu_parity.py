@@ -92,16 +92,16 @@ def test_sublinearminres(gpu_daw):
     assert abs(info["fac"] / i64["fac"] - 1) < 1e-5
 
 
-@pytest.mark.parametrize("tz", [32, 16])
-def test_tile_heights_bitexact(gpu_daw, tz):
-    """Both step-kernel tile heights give the reference's fields bit for bit."""
+@pytest.mark.parametrize("tile", [(8, 128), (16, 64), (16, 128), (32, 64), (32, 128), (64, 64)])
+def test_tile_shapes_bitexact(gpu_daw, tile):
+    """Every step-kernel tile shape gives the reference's fields bit for bit."""
     c = Case(nz=45, nx=150, nt=160)
-    gpu_daw.set_tile_rows(tz)
+    gpu_daw.set_tile(*tile)
     try:
         res = c.rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
         a = gpu_daw.submodeling(c.vp, c.npml, c.h, 1, c.nt, c.dt, c.s, c.src, c.rec)
         b = gpu_daw.submodeling_adjoint_esource_data(c.vp0, c.npml, c.h, 1, c.nt, c.dt, res, c.rec)
     finally:
-        gpu_daw.set_tile_rows(0)
+        gpu_daw.set_tile(0)
     assert np.array_equal(a, O.forward(c.vp, c.npml, c.h, 1, c.nt, c.dt, c.s, c.src, c.rec))
     assert np.array_equal(b, O.esource_data(c.vp0, c.npml, c.h, 1, c.nt, c.dt, res, c.rec))
