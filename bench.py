@@ -120,6 +120,38 @@ def cpu_baseline(P, nt_sample):
                 gpts_per_s=n1e * n2e * 2 * nt_sample / t / 1e9)
 
 
+def f64_check(P, datum, hist, nshots):
+    """Per-shot relative L2 error of the redatumed gathers and of the MINRES
+    residual history against the FP64 oracle of the plain method; the
+    oracle's gathers go through the same band-pass (filmartin2d) as the
+    timed path's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    w = P["w"]
+    O.set_progress(True)
+    dobs = P["dobs"].cpu().numpy()
+    e_dat, e_hist = [], []
+    for s in range(nshots):
+        _log(f"fp64 oracle shot {s + 1}/{nshots}")
+        row = P["src"][:, s].copy()
+        dr, wfr = O.forward(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], P["s"], row, P["rec"],
+                            want_wavefield=True, prec="f64")
+        res = np.ascontiguousarray((dobs[s].astype(np.float64) - dr)[:, ::-1])
+        _log(f"fp64 oracle shot {s + 1}: MINRES")
+        x, h, _ = O.linearminres(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], res, P["rec"], w.mu, w.maxiter,
+                                 prec="f64")
+        _log(f"fp64 oracle shot {s + 1}: source extension")
+        esc, wfe = O.esource(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], x, P["rec"], want_wavefield=True,
+                             prec="f64")
+        ref = O.resample(wfr + wfe, w.npml, P["dat"], w.ofs, prec="f64")
+        del wfr, wfe
+        ref = O.filmartin2d(ref.astype(np.float32), float(P["dt"]), *w.band)
+        got = datum[s].cpu().numpy()
+        e_dat.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+        e_hist.append(float(np.max(np.abs(hist[s][:len(h)] / h - 1))))
+    return {"f64_shots": nshots, "datum_rel_l2_vs_f64": e_dat, "minres_hist_max_rel_diff_vs_f64": e_hist}
+
+
 def calibrate_copy(nbytes=1 << 30):
     import torch
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -155,6 +187,11 @@ def main():
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0, help="tile rows, 0 for the autotuner")
     ap.add_argument("--tile-cols", type=int, default=64)
+    ap.add_argument("--snapshots", choices=["fp32", "bf16", "fp16"], default="fp32",
+                    help="storage of the Hessian operator's wavefield snapshots")
+    ap.add_argument("--graph", action="store_true", help="replay the sweeps as captured hipGraphs")
+    ap.add_argument("--f64-shots", type=int, default=0,
+                    help="also run the FP64 oracle of the plain method on this many shots (slow, CPU)")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -167,6 +204,8 @@ def main():
         return
     daw, syn = _pkg()
     daw.set_tile(args.tile_rows, args.tile_cols)
+    daw.set_snapshot_format(args.snapshots)
+    daw.set_graph(args.graph)
     _log("setup: modelling observed data")
     P = setup(daw, syn, args.config, shots=args.shots, nt=args.nt, maxiter=args.maxiter)
     w = P["w"]
@@ -188,6 +227,7 @@ def main():
     elapsed = time.perf_counter() - t0
     daw.set_timing(False)
     st = daw.stats()
+    tile = list(daw.get_tile())
     nsh = P["nsh"]
     shots_per_hour = nsh * args.steps / elapsed * 3600.0
     gpts = st["point_updates"] / elapsed / 1e9
@@ -198,7 +238,10 @@ def main():
     modes = {1: "k_step<1> (back-propagation)", 2: "k_step<2> (re-injection)"}
     dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
     avg_ms = st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1)
-    bytes_per_shot_step = 36.0 * n1e * n2e + 12.0 * w.nz * w.nx
+    # state r+w 32 B + kappa 4 B per extended point; snapshots of the interior
+    # (3 fields, 4 B each, or 2 B each + one 4 B scale per 64 columns)
+    snap_b = 12.0 if args.snapshots == "fp32" else 6.0 + 12.0 / 64.0
+    bytes_per_shot_step = 36.0 * n1e * n2e + snap_b * w.nz * w.nx
     shots_per_launch = nsh / int(np.ceil(nsh / mb))
     alg_bytes = bytes_per_shot_step * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
@@ -216,6 +259,20 @@ def main():
         tp = time.perf_counter() - t1
         plain = args.plain_shots / tp * 3600.0
         diff = float(torch.max(torch.abs(dp - datum[:args.plain_shots])) / torch.max(torch.abs(datum)))
+    # mixed precision: the same shots with FP32 snapshots (untimed) and, on
+    # request, the FP64 oracle of the plain method
+    mixed = None
+    if args.snapshots != "fp32":
+        daw.set_snapshot_format("fp32")
+        _log("fp32-snapshot reference run")
+        d32, h32 = run_redatum(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+        daw.set_snapshot_format(args.snapshots)
+        per = [float(torch.linalg.norm(datum[s] - d32[s]) / torch.linalg.norm(d32[s])) for s in range(nsh)]
+        mixed = {"snapshots": args.snapshots,
+                 "datum_rel_l2_vs_fp32_max": max(per), "datum_rel_l2_vs_fp32_mean": float(np.mean(per)),
+                 "minres_hist_max_rel_diff_vs_fp32": float(np.max(np.abs(hist / h32 - 1)))}
+    if args.f64_shots > 0:
+        mixed = dict(mixed or {}, **f64_check(P, datum, hist, args.f64_shots))
     cpu = None
     if not args.no_cpu:
         _log(f"cpu baseline sample nt={args.cpu_nt}")
@@ -235,9 +292,11 @@ def main():
         "data": "synthetic: seeded layered model (sigma 3 smoothing), background sigma 10, Ricker, 1% seeded noise",
         "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m", "shots": nsh, "nt": w.nt,
                    "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
-                   "shots_per_launch": shots_per_launch, "snapshots": "fp32",
-                   "tile": list(daw.get_tile())},
+                   "shots_per_launch": shots_per_launch, "snapshots": args.snapshots, "graph": args.graph,
+                   "tile": tile},
         "gpts_per_s": gpts,
+        "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
+                              for i, k in enumerate(("forward", "back-propagation", "re-injection"))},
         "plain_shots_per_hour": plain,
         "batched_over_plain": (shots_per_hour / plain) if plain else None,
         "batched_vs_plain_max_rel_diff": diff if plain else None,
@@ -248,6 +307,8 @@ def main():
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
+    if mixed:
+        out["precision_check"] = mixed
     print(json.dumps(out), flush=True)
 
 

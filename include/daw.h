@@ -89,6 +89,15 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
  * the re-injection pass, which makes the MINRES operator symmetric. */
 int daw_set_hessian(int exact);
 
+/* Storage of the wavefield snapshots sp, svx, svz that the reference
+ * Hessian-type operator keeps between its two passes (submodeling2.f90:1060-1087
+ * write, :1164-1185 read): 0 = FP32 as the reference (default), 1 = BF16,
+ * 2 = FP16, each 16-bit value scaled by a power of two per 64-column segment
+ * of a snapshot row (the segment maximum maps below 1).  16-bit snapshots
+ * halve the snapshot store and its traffic; the exact Hessian
+ * (daw_set_hessian(1)) always keeps FP32.  No reference counterpart. */
+int daw_set_snapshot_format(int fmt);
+
 /* subwavefieldresample(n1,n2,npml,nt,dt,ir,nr,sis,wavefield),
  * subwavefieldresample.f90:1-41. */
 int daw_subwavefieldresample(int n1, int n2, int npml, int nt, float dt, const int *ir, int nr, float *sis,
@@ -185,6 +194,11 @@ typedef struct daw_stats {
 } daw_stats;
 
 int daw_set_timing(int on);
+/* 1: capture each full sweep of a batch (state reset + nt step launches) as a
+ * hipGraph once and replay it for the later sweeps of the same kind, e.g. the
+ * 2 x itn Hessian sweeps of the MINRES solve; 0 (default): launch step by
+ * step.  Results are identical.  No reference counterpart. */
+int daw_set_graph(int on);
 /* Step-kernel tile (rows x columns): 8x128, 16x64, 16x128, 32x64, 32x128 or
  * 64x64; rows = 0 lets daw_redatum_batch time every shape on a short probe
  * sweep per batch shape and keep the fastest (the default).  No reference

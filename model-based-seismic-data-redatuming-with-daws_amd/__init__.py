@@ -34,7 +34,8 @@ EXPORTS = [
     "daw_submodeling_adjoint_esource_data", "daw_subwavefieldresample", "daw_subresiduals",
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
     "daw_get_stats", "daw_modeling_batch", "daw_set_tile", "daw_get_tile", "daw_filmartin2d_dev",
-    "daw_submodeling_transpose", "daw_set_hessian",
+    "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
+    "daw_set_graph",
 ]
 
 
@@ -213,6 +214,22 @@ def submodeling_transpose(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab):
 def set_hessian(exact):
     """0: the reference's Hessian-type operator; 1: exact (symmetric) variant."""
     _check(lib().daw_set_hessian(1 if exact else 0))
+
+
+def set_graph(on):
+    """Replay full sweeps as captured hipGraphs (True) or launch step by step."""
+    _check(lib().daw_set_graph(1 if on else 0))
+
+
+SNAPSHOT_FORMATS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+def set_snapshot_format(fmt):
+    """Snapshot store of the Hessian-type operator between its passes:
+    "fp32" (the reference's, default), "bf16" or "fp16" (16-bit values with a
+    power-of-two scale per 64-column segment of each snapshot row)."""
+    code = SNAPSHOT_FORMATS[fmt] if isinstance(fmt, str) else int(fmt)
+    _check(lib().daw_set_snapshot_format(code))
 
 
 def submodeling_adjoint_esource(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, want_wavefield=False):

@@ -17,6 +17,7 @@
 namespace daw {
 
 enum SweepMode { kFwd = 0, kBack = 1, kReinject = 2 };
+enum SnapFormat { kSnapF32 = 0, kSnapBF16 = 1, kSnapF16 = 2 };
 
 // Owned tile per workgroup and the halo widths the 4th-order staggered
 // stencil needs (two cells each side for the derivative, and one more
@@ -51,8 +52,13 @@ struct ShotDev {
     // record band: band_row[z] = slot >= 0 for rows some record window touches
     const int *band_row;
     float *pband[2];                    // [nslot][ldx], two time slots
-    // snapshots (kBack writes, kReinject reads)
-    float *sp, *svx, *svz;              // [nt][n1][n2]
+    // snapshots (kBack writes, kReinject reads), [nt][n1][n2]: FP32, or
+    // 16-bit (BF16 / FP16) with one power-of-two scale per 64-column row
+    // segment of each snapshot plane: ssc[((it*3 + field)*n1 + z)*nseg + x/64]
+    float *sp, *svx, *svz;
+    unsigned short *sp16, *svx16, *svz16;
+    float *ssc;
+    int nseg;
     int xlo1, xhi1, xlo2, xhi2;         // pressure extended-source range, extended coords inclusive
     // optional full interior wavefield, Fortran (n1,n2,nt)
     float *wf;

@@ -62,6 +62,8 @@ struct Global {
     int txw = 1;               // tile width in wavefronts (64 columns each)
     bool tz_auto = true;       // pick tz per batch shape with a timed probe
     bool hess_exact = false;   // Hessian pass 1 = exact transpose of pass 2
+    int snap_fmt = kSnapF32;   // snapshot store of the reference Hessian (SnapFormat)
+    bool graph = false;        // replay full sweeps as captured hipGraphs
     std::map<std::vector<int>, std::pair<int, int>> tz_cache;
 };
 static Global g;
@@ -313,10 +315,46 @@ struct SweepSet {
     int ntiles = 0;
     int nt_max = 0;
     long long pts_per_step = 0;
+    int snap = kSnapF32;       // SnapFormat of the shots' snapshot store
+    // record items of the next sweep: one device buffer for the batch's
+    // lifetime (so a captured sweep stays valid), host copy kept for the
+    // asynchronous upload
+    DevBuf d_items;
+    std::vector<RecItem> items;
+    // captured sweeps, keyed by everything their launches bake in
+    std::map<std::vector<long long>, hipGraphExec_t> graphs;
+    SweepSet() = default;
+    SweepSet(const SweepSet &) = delete;
+    SweepSet &operator=(const SweepSet &) = delete;
+    ~SweepSet() { clear_graphs(); }
+    void clear_graphs()
+    {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
 };
+
+// Copies the record items of the next sweep to the batch's item buffer
+// (stream-ordered: the previous sweep has read the old items by then).
+static int set_items(SweepSet &W, const std::vector<RecItem> &items)
+{
+    const size_t bytes = std::max<size_t>(items.size(), 1) * sizeof(RecItem);
+    if (bytes > W.d_items.bytes) {
+        HIPCHK(hipStreamSynchronize(g.stream));
+        W.clear_graphs();
+        if (int rc = W.d_items.alloc(std::max(bytes, 2 * W.d_items.bytes))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));   // the host copy of the last upload is free
+    W.items = items;
+    if (!items.empty())
+        HIPCHK(hipMemcpyAsync(W.d_items.p, W.items.data(), items.size() * sizeof(RecItem), hipMemcpyHostToDevice,
+                              g.stream));
+    return 0;
+}
 
 static int finalize_sweepset(SweepSet &W)
 {
+    W.clear_graphs();
     std::vector<int> tps(W.nshots);
     W.nt_max = 0;
     W.pts_per_step = 0;
@@ -338,10 +376,10 @@ static int refresh_shots(SweepSet &W)
     return 0;
 }
 
-// Runs one sweep of nt_max steps over the batch and the trailing record.
-static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int nsteps_limit = 0)
+// The launches of one sweep: zero the read half of the state, then nsteps
+// step launches and the trailing record.
+static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int nsteps)
 {
-    // zero the read half of the double-buffered state
     for (int s = 0; s < W.nshots; ++s) {
         const ShotDev &S = W.host[s];
         const size_t pe = (size_t)S.n1e * S.ldx;
@@ -355,14 +393,7 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
     P.tiles = W.d_tiles.as<int2>();
     P.rec = d_rec;
     const int rec_blocks = (nrec + kBlock - 1) / kBlock;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (g.timing) {
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, g.stream));
-    }
     int cur = 0;
-    const int nsteps = nsteps_limit > 0 ? std::min(nsteps_limit, W.nt_max) : W.nt_max;
     for (int it = 0; it <= nsteps; ++it) {
         P.ntiles = (it < nsteps) ? W.ntiles : 0;
         P.it = it;
@@ -372,8 +403,49 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
         P.nrec_blocks = rec_prev ? rec_blocks : 0;
         P.rec_it = it - 1;
         P.rec_slot = (it - 1) & 1;
-        launch_step(mode, P, g.tz, g.txw, g.stream);
+        launch_step(mode, W.snap, P, g.tz, g.txw, g.stream);
         cur ^= 1;
+    }
+}
+
+// Runs one sweep of nt_max (or nsteps_limit) steps over the batch.  With
+// graphs on, a full sweep is captured once per batch and launch shape and
+// replayed: the shot descriptors and record items it reads are refreshed in
+// place before each replay.
+static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int nsteps_limit = 0)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g.timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g.stream));
+    }
+    const int nsteps = nsteps_limit > 0 ? std::min(nsteps_limit, W.nt_max) : W.nt_max;
+    if (g.graph && nsteps_limit == 0) {
+        const std::vector<long long> key{mode, nsteps, nrec, g.tz, g.txw, W.snap, W.ntiles, (long long)(intptr_t)d_rec,
+                                         (long long)(intptr_t)W.d_tiles.p, (long long)(intptr_t)W.d_shots.p};
+        auto hit = W.graphs.find(key);
+        hipGraphExec_t ex = nullptr;
+        if (hit == W.graphs.end()) {
+            HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            launch_sweep(mode, W, d_rec, nrec, nsteps);
+            const hipError_t el = hipGetLastError();
+            hipGraph_t gr = nullptr;
+            const hipError_t ec = hipStreamEndCapture(g.stream, &gr);
+            if (el != hipSuccess || ec != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return fail(DAW_ERR_HIP, "sweep capture: %s", hipGetErrorString(el != hipSuccess ? el : ec));
+            }
+            const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ei != hipSuccess) return fail(DAW_ERR_HIP, "sweep graph instantiate: %s", hipGetErrorString(ei));
+            W.graphs[key] = ex;
+        } else {
+            ex = hit->second;
+        }
+        HIPCHK(hipGraphLaunch(ex, g.stream));
+    } else {
+        launch_sweep(mode, W, d_rec, nrec, nsteps);
     }
     HIPCHK(hipGetLastError());
     if (g.timing) {
@@ -429,19 +501,40 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
 }
 
 // Per-shot device storage of one batch.
+// Snapshot store: sp, svx, svz(n1,n2,nt) per shot, FP32 or 16-bit with one
+// power-of-two FP32 scale per 64-column segment of each snapshot row.
 struct ShotStore {
     DevBuf fields;   // 8 state arrays per shot
     DevBuf band;     // 2 slots per shot
-    DevBuf snap;     // sp, svx, svz per shot
+    DevBuf snap;     // FP32 snapshots
+    DevBuf snap16;   // 16-bit snapshots
+    DevBuf scale;    // segment scales of the 16-bit snapshots
+    int fmt = -1;    // SnapFormat, -1: no snapshots
+    bool has_snap() const { return fmt >= 0; }
 };
 
-static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int nt, bool want_snap)
+static int snap_segments(const Grid &G) { return (G.n2e + TX - 1) / TX; }
+
+// Device bytes of one shot's snapshot store in format fmt.
+static size_t snap_bytes(const Grid &G, int nt, int fmt)
+{
+    if (fmt < 0) return 0;
+    if (fmt == kSnapF32) return 3 * G.plane_i() * nt * sizeof(float);
+    return 3 * G.plane_i() * nt * 2 + 3 * (size_t)nt * G.n1 * snap_segments(G) * sizeof(float);
+}
+
+static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int nt, int fmt)
 {
     const size_t pe = G.plane_e();
     if (int rc = st.fields.alloc((size_t)nshots * 8 * pe * sizeof(float))) return rc;
     if (int rc = st.band.alloc((size_t)nshots * 2 * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
-    if (want_snap)
+    st.fmt = fmt;
+    if (fmt == kSnapF32) {
         if (int rc = st.snap.alloc((size_t)nshots * 3 * G.plane_i() * nt * sizeof(float))) return rc;
+    } else if (fmt > 0) {
+        if (int rc = st.snap16.alloc((size_t)nshots * 3 * G.plane_i() * nt * 2)) return rc;
+        if (int rc = st.scale.alloc((size_t)nshots * 3 * nt * G.n1 * snap_segments(G) * sizeof(float))) return rc;
+    }
     return 0;
 }
 
@@ -465,9 +558,15 @@ static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &s
         S.pband[0] = b;
         S.pband[1] = b + (size_t)std::max(nslot, 1) * G.ldx;
     }
-    if (st.snap.p) {
-        float *sn = st.snap.as<float>() + (size_t)s * 3 * G.plane_i() * nt;
-        S.sp = sn; S.svx = sn + G.plane_i() * nt; S.svz = sn + 2 * G.plane_i() * nt;
+    const size_t pn = G.plane_i() * nt;
+    if (st.fmt == kSnapF32) {
+        float *sn = st.snap.as<float>() + (size_t)s * 3 * pn;
+        S.sp = sn; S.svx = sn + pn; S.svz = sn + 2 * pn;
+    } else if (st.fmt > 0) {
+        unsigned short *sn = static_cast<unsigned short *>(st.snap16.p) + (size_t)s * 3 * pn;
+        S.sp16 = sn; S.svx16 = sn + pn; S.svz16 = sn + 2 * pn;
+        S.nseg = snap_segments(G);
+        S.ssc = st.scale.as<float>() + (size_t)s * 3 * nt * G.n1 * S.nseg;
     }
     S.xlo1 = G.npml; S.xhi1 = G.npml + G.n1 - 1;
     S.xlo2 = G.npml; S.xhi2 = G.npml + G.n2 - 1;
@@ -575,7 +674,7 @@ static int autotune_tiles(Batch &B)
         }
         return 0;
     }
-    if (!B.store.snap.p) return 0;
+    if (!B.store.has_snap()) return 0;
     DevBuf zeros;
     if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
     launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
@@ -655,11 +754,10 @@ static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o
         if (o.datum) add_records(items, B.dat, s, 1, 0, B.d_dat_coef, o.datum + (size_t)s * o.datum_shot, B.nt);
     }
     if (!o.record) for (auto &S : W.host) S.wf = nullptr;
-    DevBuf d_items;
-    if (int rc = upload(d_items, items)) return rc;
+    if (int rc = set_items(W, items)) return rc;
     if (int rc = refresh_shots(W)) return rc;
-    int rc = run_sweep(kReinject, W, d_items.as<RecItem>(), (int)items.size());
-    HIPCHK(hipStreamSynchronize(g.stream));   // d_items lifetime
+    int rc = run_sweep(kReinject, W, W.d_items.as<RecItem>(), (int)items.size());
+    HIPCHK(hipStreamSynchronize(g.stream));
     return rc;
 }
 
@@ -677,10 +775,9 @@ static int forward(Batch &B, const float *s_wavelet, int type_src, int record, f
         if (record && sis) add_records(items, B.rec, s, 0, 0, B.d_rec_coef, sis + (size_t)s * sis_shot, B.nt);
         if (datum) add_records(items, B.dat, s, 1, 0, B.d_dat_coef, datum + (size_t)s * datum_shot, B.nt);
     }
-    DevBuf d_items;
-    if (int rc = upload(d_items, items)) return rc;
+    if (int rc = set_items(W, items)) return rc;
     if (int rc = refresh_shots(W)) return rc;
-    int rc = run_sweep(kFwd, W, d_items.as<RecItem>(), (int)items.size());
+    int rc = run_sweep(kFwd, W, W.d_items.as<RecItem>(), (int)items.size());
     HIPCHK(hipStreamSynchronize(g.stream));
     return rc;
 }
@@ -721,8 +818,11 @@ static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int n
     if (int rc = upload(B.d_rec_coef, B.rec.coef)) return rc;
     if (B.dat.n)
         if (int rc = upload(B.d_dat_coef, B.dat.coef)) return rc;
-    if (int rc = alloc_store(B.store, nshots, G, B.band.nslot, nt, want_snap)) return rc;
+    // the exact Hessian's transpose pass writes FP32 snapshots
+    const int fmt = want_snap ? (g.hess_exact ? (int)kSnapF32 : g.snap_fmt) : -1;
+    if (int rc = alloc_store(B.store, nshots, G, B.band.nslot, nt, fmt)) return rc;
     B.W.nshots = nshots;
+    B.W.snap = fmt < 0 ? (int)kSnapF32 : fmt;
     B.W.host.resize(nshots);
     for (int s = 0; s < nshots; ++s) fill_shot(B.W.host[s], G, *M, B.store, s, nt, B.band.nslot, &B.band);
     return finalize_sweepset(B.W);
@@ -1097,7 +1197,8 @@ int daw_redatum_batch(const daw_redatum_args *a)
     const int nt = a->nt, nrec = a->nrec, ndat = a->ndat;
     const size_t nvec = (size_t)nrec * nt, ndv = (size_t)ndat * nt;
     // HBM admission: snapshots dominate (3*n1*n2*nt floats per shot)
-    size_t per_shot = 3 * G.plane_i() * nt * 4 + 8 * G.plane_e() * 4 + 14 * nvec * 4 + 3 * ndv * 4 + (1u << 20);
+    const int fmt = g.hess_exact ? (int)kSnapF32 : g.snap_fmt;
+    size_t per_shot = snap_bytes(G, nt, fmt) + 8 * G.plane_e() * 4 + 14 * nvec * 4 + 3 * ndv * 4 + (1u << 20);
     int maxb = a->max_batch;
     if (maxb <= 0) {
         size_t freeb = 0, total = 0;
@@ -1250,6 +1351,22 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
     if (int rc = run_adjoint_sweep(false, B.W, d_items.as<RecItem>(), (int)items.size())) return rc;
     HIPCHK(hipStreamSynchronize(g.stream));
     HIPCHK(hipMemcpy(gsrc, d_g.p, (size_t)nt * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int daw_set_graph(int on)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.graph = on != 0;
+    return 0;
+}
+
+int daw_set_snapshot_format(int fmt)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (fmt != kSnapF32 && fmt != kSnapBF16 && fmt != kSnapF16)
+        return fail(DAW_ERR_ARG, "snapshot format %d: 0 (fp32), 1 (bf16) or 2 (fp16)", fmt);
+    g.snap_fmt = fmt;
     return 0;
 }
 

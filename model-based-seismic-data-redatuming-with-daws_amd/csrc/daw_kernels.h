@@ -6,7 +6,8 @@
 
 namespace daw {
 
-void launch_step(int mode, const StepParams &P, int tz, int txw, hipStream_t st);
+// snap: SnapFormat of the snapshot store written by kBack and read by kReinject.
+void launch_step(int mode, int snap, const StepParams &P, int tz, int txw, hipStream_t st);
 // Tile shape (rows tz, 64*txw columns) has compiled kernels.
 bool tile_shape_ok(int tz, int txw);
 // Exact adjoint of one step (reverse time); hess selects the transpose of

@@ -84,6 +84,9 @@ struct View {
     const int *band_row;
     float *pband;
     float *sp, *svx, *svz, *wf;
+    unsigned short *sp16, *svx16, *svz16;
+    float *ssc;
+    int nseg;
 };
 
 __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur, int it)
@@ -104,7 +107,82 @@ __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur
     v.band_row = S->band_row;
     v.pband = (it & 1) ? S->pband[1] : S->pband[0];
     v.sp = S->sp; v.svx = S->svx; v.svz = S->svz; v.wf = S->wf;
+    v.sp16 = S->sp16; v.svx16 = S->svx16; v.svz16 = S->svz16; v.ssc = S->ssc; v.nseg = S->nseg;
     return v;
+}
+
+
+// ---- 16-bit snapshot codec.  Each 64-column row segment of a snapshot plane
+// is written by one wavefront, which finds its max |value| with a shuffle
+// reduction and stores the values divided by the power of two at or above
+// it (an exact rescaling) as BF16 or FP16, round to nearest even.
+__device__ __forceinline__ float wave_max_abs(float v)
+{
+    v = fabsf(v);
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float pow2_scale(float amax)
+{
+    if (!(amax > 0.0f)) return 1.0f;
+    int e;
+    (void)frexpf(amax, &e);          // amax = m * 2^e, m in [0.5, 1)
+    return ldexpf(1.0f, e < -100 ? -100 : e);   // > amax; 1/scale stays finite
+}
+
+template <int F>
+__device__ __forceinline__ unsigned short snap_encode(float v)
+{
+    if (F == kSnapBF16) {
+        unsigned int u = __float_as_uint(v);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    } else {
+        return __builtin_bit_cast(unsigned short, (_Float16)v);   // round to nearest even
+    }
+}
+
+template <int F>
+__device__ __forceinline__ float snap_decode(unsigned short q)
+{
+    if (F == kSnapBF16) return __uint_as_float((unsigned int)q << 16);
+    return (float)__builtin_bit_cast(_Float16, q);
+}
+
+// interior snapshot element and its segment scale
+__device__ __forceinline__ size_t snap_idx(const View &S, int t, int gz, int gx)
+{
+    return (size_t)t * ((size_t)S.n1 * S.n2) + (size_t)(gz - S.npml) * S.n2 + (gx - S.npml);
+}
+
+__device__ __forceinline__ size_t scale_idx(const View &S, int t, int field, int gz, int gx)
+{
+    return (((size_t)t * 3 + field) * S.n1 + (gz - S.npml)) * S.nseg + (gx >> 6);
+}
+
+template <int F>
+__device__ __forceinline__ float snap_load(const View &S, const float *f32, const unsigned short *f16, int field, int t,
+                                           int gz, int gx)
+{
+    if (F == kSnapF32) return f32[snap_idx(S, t, gz, gx)];
+    return snap_decode<F>(f16[snap_idx(S, t, gz, gx)]) * S.ssc[scale_idx(S, t, field, gz, gx)];
+}
+
+// Whole wavefront (all 64 lanes, converged, one 64-aligned column segment):
+// store v where `interior`.  A wavefront of a tile that reaches past the grid
+// (segment >= nseg) has no interior lane and writes no scale.
+template <int F>
+__device__ __forceinline__ void snap_store_wave(const View &S, float *f32, unsigned short *f16, int field, int t, int gz,
+                                                int gx, float v, bool interior, bool row_interior)
+{
+    if (F == kSnapF32) {
+        if (interior) f32[snap_idx(S, t, gz, gx)] = v;
+        return;
+    }
+    const float sc = pow2_scale(wave_max_abs(interior ? v : 0.0f));
+    if (interior) f16[snap_idx(S, t, gz, gx)] = snap_encode<F>(v * (1.0f / sc));
+    if (row_interior && (threadIdx.x & 63) == 0 && (gx >> 6) < S.nseg) S.ssc[scale_idx(S, t, field, gz, gx)] = sc;
 }
 
 template <int MODE, int VZW_>
@@ -150,8 +228,6 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
         pzn[g] = pzv;
         const int npml = S.npml;
         const bool interior = gz >= npml && gz < npml + S.n1 && gx >= npml && gx < npml + S.n2;
-        if (MODE == kBack && interior)
-            S.sp[(size_t)it * ((size_t)S.n1 * S.n2) + (size_t)(gz - npml) * S.n2 + (gx - npml)] = p;
         if (MODE != kBack) {
             if (S.band_row != nullptr) {
                 const int slot = S.band_row[gz];
@@ -182,7 +258,7 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
 //   3. velocity half-step of the owned points, vz source or snapshot
 //      re-injection, the vz(npml) = vz(npml+1) mirror, writes.
 // ---------------------------------------------------------------------------
-template <int MODE, int TZ, int TXW>
+template <int MODE, int TZ, int TXW, int SNAP>
 __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
 {
     constexpr int NW = kBlock / 64;
@@ -236,7 +312,6 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     // and (re-injection) the snapshot values
     constexpr int XR = (VXH_ + NWZ - 1) / NWZ, ZR = (VZH_ + NWZ - 1) / NWZ;
     const int npml = S.npml, n1 = S.n1, n2 = S.n2;
-    const size_t plane = (size_t)n1 * n2;
     float vxl[XR], vzl[ZR];
 #pragma unroll
     for (int j = 0; j < XR; ++j) {
@@ -289,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             pzr[j] = pzo[g];
             kar[j] = S.kappa[g];
             if (MODE == kReinject && gz >= S.xlo1 && gz <= S.xhi1 && gxl >= S.xlo2 && gxl <= S.xhi2)
-                spr[j] = S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gxl - npml)];
+                spr[j] = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, gz, gxl);
         }
     }
     // side points of the ring: columns x0-1, x0+TX, x0+TX+1 on the tile rows,
@@ -310,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             s_pz = pzo[g];
             s_ka = S.kappa[g];
             if (MODE == kReinject && s_gz >= S.xlo1 && s_gz <= S.xhi1 && s_gx >= S.xlo2 && s_gx <= S.xhi2)
-                s_sp = S.sp[(size_t)ic * plane + (size_t)(s_gz - npml) * n2 + (s_gx - npml)];
+                s_sp = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, s_gz, s_gx);
         }
     }
     float svxr[VROWS], svzr[VROWS];
@@ -319,9 +394,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         const int gz = z0 + rz + NWZ * j;
         svxr[j] = 0.0f; svzr[j] = 0.0f;
         if (MODE == kReinject && colok && gz >= npml && gz < npml + n1 && gxl >= npml && gxl < npml + n2) {
-            const size_t sidx = (size_t)ic * plane + (size_t)(gz - npml) * n2 + (gxl - npml);
-            svxr[j] = S.svx[sidx];
-            svzr[j] = S.svz[sidx];
+            svxr[j] = snap_load<SNAP>(S, S.svx, S.svx16, 1, ic, gz, gxl);
+            svzr[j] = snap_load<SNAP>(S, S.svz, S.svz16, 2, ic, gz, gxl);
         }
     }
 #pragma unroll
@@ -357,6 +431,12 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
                                         injq, r >= 1 && r <= TZ, lo, hiz, hix, spr[j], pxn, pzn);
             }
             s_p[r * PW_ + cl + 1] = p;
+            if (MODE == kBack && r >= 1 && r <= TZ) {      // snapshot sp(:,:,it) (:1060-1064)
+                const bool rowi = gz >= npml && gz < npml + n1;
+                if (rowi)
+                    snap_store_wave<SNAP>(S, S.sp, S.sp16, 0, it, gz, gxl, p,
+                                          colok && gxl >= npml && gxl < npml + n2, rowi);
+            }
         }
     }
     if (side) {
@@ -371,55 +451,58 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     }
     __syncthreads();
 
-    // ---- 3. velocity half-step of the owned points
-    if (!colok) return;
+    // ---- 3. velocity half-step of the owned points (lanes past the grid
+    // stay in the loop: the snapshot store reduces over whole wavefronts)
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
-    const float c_d2b = S.d2b[gxl], c_bw2 = S.bw2[gxl];
+    const float c_d2b = colok ? S.d2b[gxl] : 1.0f, c_bw2 = colok ? S.bw2[gxl] : 0.0f;
 #pragma unroll
     for (int j = 0; j < VROWS; ++j) {
         const int r = rz + NWZ * j;
         const int gz = z0 + r, gx = gxl;
         if (gz >= n1e) break;
         const size_t g = (size_t)gz * ldx + gx;
-        float vxv = s_vx[(r + 1) * VXW_ + cl + 3];
-        float vzv = s_vz[(r + 3) * VZW_ + cl + 1];
-        if (in_box(gz, gx, lo, hiz, hix)) {
-            const float *pp = &s_p[(r + 1) * PW_ + cl + 1];         // p(gz, gx)
-            const float der2 = a0 * (pp[1] - pp[0]) + a1 * (pp[2] - pp[-1]);
-            const float der1 = a0 * (pp[PW_] - pp[0]) + a1 * (pp[2 * PW_] - pp[-PW_]);
-            if (MODE == kBack) {
-                vxv = c_d2b * vxv - c_bw2 * der2;
-                vzv = S.d1b[gz] * vzv - S.bw1[gz] * der1;
+        const bool rowi = gz >= npml && gz < npml + n1;
+        const bool interior = rowi && gx >= npml && gx < npml + n2;
+        float vxv = 0.0f, vzv = 0.0f;
+        if (colok) {
+            vxv = s_vx[(r + 1) * VXW_ + cl + 3];
+            vzv = s_vz[(r + 3) * VZW_ + cl + 1];
+            if (in_box(gz, gx, lo, hiz, hix)) {
+                const float *pp = &s_p[(r + 1) * PW_ + cl + 1];         // p(gz, gx)
+                const float der2 = a0 * (pp[1] - pp[0]) + a1 * (pp[2] - pp[-1]);
+                const float der1 = a0 * (pp[PW_] - pp[0]) + a1 * (pp[2 * PW_] - pp[-PW_]);
+                if (MODE == kBack) {
+                    vxv = c_d2b * vxv - c_bw2 * der2;
+                    vzv = S.d1b[gz] * vzv - S.bw1[gz] * der1;
+                } else {
+                    vxv = c_d2b * vxv + c_bw2 * der2;
+                    vzv = S.d1b[gz] * vzv + S.bw1[gz] * der1;
+                }
+            }
+            if (MODE == kFwd && S.vinj && S.inj_row != nullptr) {
+                const int q = S.inj_row[gz];
+                if (q >= 0) {
+                    const int *ptr = S.inj_ptr + (size_t)q * (n2e + 1);
+                    for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
+                        vzv = vzv + S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + it];   // * bue (= 1), :252
+                }
+            }
+            if (MODE == kReinject && interior) {
+                vxv = vxv + (-S.dt) * svxr[j];                         // * bue (= 1), :1177
+                vzv = vzv + (-S.dt) * svzr[j];                         // :1184
+            }
+            vxn[g] = vxv;
+            if (S.ofs == 1 && gz == npml - 1) {
+                // vz(npml,:) is overwritten by the mirror below; its owner writes nothing
             } else {
-                vxv = c_d2b * vxv + c_bw2 * der2;
-                vzv = S.d1b[gz] * vzv + S.bw1[gz] * der1;
+                vzn[g] = vzv;
+                if (S.ofs == 1 && gz == npml) vzn[g - ldx] = vzv;     // vz(npml,:) = vz(npml+1,:)
             }
         }
-        if (MODE == kFwd && S.vinj && S.inj_row != nullptr) {
-            const int q = S.inj_row[gz];
-            if (q >= 0) {
-                const int *ptr = S.inj_ptr + (size_t)q * (n2e + 1);
-                for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
-                    vzv = vzv + S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + it];   // * bue (= 1), :252
-            }
-        }
-        const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
-        if (MODE == kReinject && interior) {
-            vxv = vxv + (-S.dt) * svxr[j];                         // * bue (= 1), :1177
-            vzv = vzv + (-S.dt) * svzr[j];                         // :1184
-        }
-        vxn[g] = vxv;
-        if (S.ofs == 1 && gz == npml - 1) {
-            // vz(npml,:) is overwritten by the mirror below; its owner writes nothing
-        } else {
-            vzn[g] = vzv;
-            if (S.ofs == 1 && gz == npml) vzn[g - ldx] = vzv;     // vz(npml,:) = vz(npml+1,:)
-        }
-        if (MODE == kBack && interior) {
-            const size_t s = (size_t)it * plane + (size_t)(gz - npml) * n2 + (gx - npml);
-            S.svx[s] = vxv;
-            S.svz[s] = vzv;
+        if (MODE == kBack && rowi) {                          // svx, svz(:,:,it) (:1082-1087), whole wavefront
+            snap_store_wave<SNAP>(S, S.svx, S.svx16, 1, it, gz, gx, vxv, interior, true);
+            snap_store_wave<SNAP>(S, S.svz, S.svz16, 2, it, gz, gx, vzv, interior, true);
         }
     }
 }
@@ -610,13 +693,13 @@ void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t
 #undef DAW_ADJ
 }
 
-template <int MODE>
+template <int MODE, int SNAP>
 static void launch_mode(const StepParams &P, int tz, int txw, hipStream_t st)
 {
     const int grid = P.ntiles + P.nrec_blocks;
 #define DAW_STEP(Z, W)                                                                                         \
     if (tz == Z && txw == W) {                                                                                 \
-        hipLaunchKernelGGL((k_step<MODE, Z, W>), dim3(grid), dim3(kBlock), 0, st, P);                          \
+        hipLaunchKernelGGL((k_step<MODE, Z, W, SNAP>), dim3(grid), dim3(kBlock), 0, st, P);                    \
         return;                                                                                                \
     }
     DAW_TILE_SHAPES(DAW_STEP)
@@ -631,13 +714,21 @@ bool tile_shape_ok(int tz, int txw)
     return false;
 }
 
-void launch_step(int mode, const StepParams &P, int tz, int txw, hipStream_t st)
+template <int MODE>
+static void launch_snap(int snap, const StepParams &P, int tz, int txw, hipStream_t st)
+{
+    if (snap == kSnapBF16) launch_mode<MODE, kSnapBF16>(P, tz, txw, st);
+    else if (snap == kSnapF16) launch_mode<MODE, kSnapF16>(P, tz, txw, st);
+    else launch_mode<MODE, kSnapF32>(P, tz, txw, st);
+}
+
+void launch_step(int mode, int snap, const StepParams &P, int tz, int txw, hipStream_t st)
 {
     const int grid = P.ntiles + P.nrec_blocks;
     if (grid == 0) return;
-    if (mode == kFwd) launch_mode<kFwd>(P, tz, txw, st);
-    else if (mode == kBack) launch_mode<kBack>(P, tz, txw, st);
-    else launch_mode<kReinject>(P, tz, txw, st);
+    if (mode == kFwd) launch_mode<kFwd, kSnapF32>(P, tz, txw, st);   // no snapshots
+    else if (mode == kBack) launch_snap<kBack>(snap, P, tz, txw, st);
+    else launch_snap<kReinject>(snap, P, tz, txw, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1085,7 +1176,7 @@ int launch_filter(const float *din, float *dout, int ntr, int nt, int nto, const
     int bits = 0;
     while ((1 << bits) < nto) ++bits;
     const size_t lds = (size_t)nto * sizeof(double2);
-    hipFuncSetAttribute((const void *)k_filter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void *)k_filter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_filter, dim3((ntr + 1) / 2), dim3(256), lds, st, din, dout, ntr, nt, nto, bits, wgt, tw);
     return 0;
 }

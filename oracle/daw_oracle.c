@@ -25,6 +25,7 @@
  *   src/libfourier.f90    REALFT2S :199-276, four2 :282-331
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -355,6 +356,53 @@ int oracle_filmartin2d(const float *din, int nr, int nt, float dt, float flow, f
     }
     free(pre); free(pim); free(dm);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 16-bit snapshot codec (no reference counterpart: the reference keeps its  */
+/* snapshots sp, svx, svz in REAL*4, submodeling2.f90:1060-1087).  A value   */
+/* v of a 64-column segment with maximum |v| = amax is stored as            */
+/* round16(v / 2^e), 2^e the smallest power of two above amax (e >= -100),  */
+/* and read back as round16(...) * 2^e.                                     */
+/* ------------------------------------------------------------------------ */
+/* 1: the long-running solvers print one line per iteration to stderr */
+static int oracle_progress = 0;
+void oracle_set_progress(int on) { oracle_progress = on; }
+
+float oracle_snap_scale(float amax)
+{
+    if (!(amax > 0.0f)) return 1.0f;
+    int e;
+    (void)frexpf(amax, &e);
+    return ldexpf(1.0f, e < -100 ? -100 : e);
+}
+
+/* Round x (|x| <= 1) to the nearest BF16 (fmt 1) or FP16 (fmt 2) value,
+ * ties to even, returned as a float. */
+float oracle_snap_round(float x, int fmt)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if (fmt == 1) {                          /* keep 8 significant bits */
+        u += 0x7fffu + ((u >> 16) & 1u);
+        u &= 0xffff0000u;
+    } else if (fabsf(x) >= 0x1p-14f) {       /* FP16 normal: keep 11 significant bits */
+        u += 0x0fffu + ((u >> 13) & 1u);
+        u &= 0xffffe000u;
+    } else {                                 /* FP16 subnormal: multiples of 2^-24 */
+        volatile float t = x + 0.75f;        /* ulp(0.75) = 2^-24 */
+        float y = t - 0.75f;
+        memcpy(&u, &y, 4);
+        if (y == 0.0f) u = signbit(x) ? 0x80000000u : 0u;   /* keep the sign of zero */
+    }
+    float y;
+    memcpy(&y, &u, 4);
+    return y;
+}
+
+void oracle_snap_round_array(float *x, int n, int fmt)
+{
+    for (int i = 0; i < n; i++) x[i] = oracle_snap_round(x[i], fmt);
 }
 
 /* ------------------------------------------------------------------------ */

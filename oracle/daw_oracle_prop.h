@@ -294,9 +294,28 @@ int SFX(oracle_forward)(const float *vp, int n1, int n2, int npml, float h, int 
 static void SFX(exact_pass1)(SFX(setup_t) *S, int ofs, int nt, const R *res, int nr, const int *ir,
                              const float *rt, int variant, R *sp, R *svx, R *svz);
 
+/* Replaces every snapshot value by its 16-bit round trip (fmt 1 BF16, 2 FP16):
+ * per time step and row, one scale per 64 columns of the extended grid
+ * (extended 0-based column npml + i2 - 1 in segment (npml + i2 - 1) / 64). */
+void SFX(snap_quantize)(R *a, int n1, int n2, int npml, int nt, int fmt)
+{
+    for (int it = 1; it <= nt; it++)
+        for (int i1 = 1; i1 <= n1; i1++)
+            for (int k = npml / 64; k <= (npml + n2 - 1) / 64; k++) {
+                int lo = 64 * k - npml + 1, hi = 64 * k + 63 - npml + 1;
+                if (lo < 1) lo = 1;
+                if (hi > n2) hi = n2;
+                float amax = 0.0f;
+                for (int i2 = lo; i2 <= hi; i2++) amax = fmaxf(amax, fabsf((float)I3(a, i1, i2, it)));
+                const float sc = oracle_snap_scale(amax), inv = 1.0f / sc;
+                for (int i2 = lo; i2 <= hi; i2++)
+                    I3(a, i1, i2, it) = (R)(oracle_snap_round((float)I3(a, i1, i2, it) * inv, fmt) * sc);
+            }
+}
+
 static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt,
                                     float dt, const R *res, int type_rec, int nr, const int *ir, R *sis,
-                                    R *wavefieldp, int variant, R *snap_cost, int exact)
+                                    R *wavefieldp, int variant, R *snap_cost, int exact, int snap_fmt)
 {
     SFX(setup_t) S;
     SFX(setup_make)(&S, vp, n1, n2, npml, h, dt);
@@ -324,6 +343,11 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
                 I3(svx, i1, i2, it) = E(F.vx, i1 + npml, i2 + npml);
                 I3(svz, i1, i2, it) = E(F.vz, i1 + npml, i2 + npml);
             }
+    }
+    if (snap_fmt == 1 || snap_fmt == 2) {
+        SFX(snap_quantize)(sp, n1, n2, npml, nt, snap_fmt);
+        SFX(snap_quantize)(svx, n1, n2, npml, nt, snap_fmt);
+        SFX(snap_quantize)(svz, n1, n2, npml, nt, snap_fmt);
     }
     if (snap_cost) {
         R c = 0;
@@ -370,7 +394,15 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
 int SFX(oracle_esource_data)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
                              const R *res, int type_rec, int nr, const int *ir, R *sis)
 {
-    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, NULL, 0, NULL, 0);
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, NULL, 0, NULL, 0, 0);
+}
+
+/* esource_data with the snapshots kept in 16-bit (snap_fmt 1 BF16, 2 FP16). */
+int SFX(oracle_esource_data_snap)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                                  const R *res, int type_rec, int nr, const int *ir, R *sis, int snap_fmt)
+{
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, NULL, 0, NULL, 0,
+                                    snap_fmt);
 }
 
 /* Exact-Hessian variant (no reference counterpart): pass 1 is the transpose
@@ -379,13 +411,13 @@ int SFX(oracle_esource_data)(const float *vp, int n1, int n2, int npml, float h,
 int SFX(oracle_esource_data_exact)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
                                    const R *res, int nr, const int *ir, R *sis)
 {
-    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, 0, nr, ir, sis, NULL, 0, NULL, 1);
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, 0, nr, ir, sis, NULL, 0, NULL, 1, 0);
 }
 
 int SFX(oracle_esource)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
                         const R *res, int type_rec, int nr, const int *ir, R *sis, R *wavefieldp, R *cost)
 {
-    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, wavefieldp, 1, cost, 0);
+    return SFX(oracle_esource_impl)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr, ir, sis, wavefieldp, 1, cost, 0, 0);
 }
 
 /* subwavefieldresample (subwavefieldresample.f90:1-41): window clipped to the
@@ -530,6 +562,7 @@ int SFX(oracle_linearminres)(const float *vp, int n1, int n2, int npml, float h,
             Arnorml = rnorml * rootl;
             Acond = gmax / gmin;
             if (hist) hist[itn - 1] = qrnorm / beta1;
+            if (oracle_progress) fprintf(stderr, "[oracle] MINRES itn %d qrnorm/beta1 %.6e\n", itn, (double)(qrnorm / beta1));
             if (istop == 0) {
                 if (itn >= itnlim) istop = 5;
                 if (qrnorm / beta1 <= (R)rtol) istop = 1;

@@ -47,6 +47,11 @@ def lib():
         L.oracle_filmartin_design.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                               C.c_float, _f64p, _i32p]
         L.oracle_filmartin_design.restype = C.c_int
+        L.oracle_set_progress.argtypes = [C.c_int]
+        L.oracle_snap_scale.argtypes = [C.c_float]
+        L.oracle_snap_scale.restype = C.c_float
+        L.oracle_snap_round_array.argtypes = [_f32p, C.c_int, C.c_int]
+        L.snap_quantize_f32.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         for sfx, rp in (("f32", _f32p), ("f64", _f64p)):
             f = getattr(L, "oracle_forward_" + sfx)
             f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
@@ -55,6 +60,10 @@ def lib():
             f = getattr(L, "oracle_esource_data_" + sfx)
             f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
                           rp, C.c_int, C.c_int, _i32p, rp]
+            f.restype = C.c_int
+            f = getattr(L, "oracle_esource_data_snap_" + sfx)
+            f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
+                          rp, C.c_int, C.c_int, _i32p, rp, C.c_int]
             f.restype = C.c_int
             f = getattr(L, "oracle_esource_" + sfx)
             f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
@@ -165,15 +174,25 @@ def forward_adjoint(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab, prec="f64"):
     return g
 
 
-def esource_data(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, prec="f32"):
+SNAP_FORMATS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+def esource_data(vp, npml, h, ofs, nt, dt, res, rec_tab, type_rec=0, prec="f32", snap="fp32"):
+    """submodeling_adjoint_esource_data; snap "bf16"/"fp16" keeps the
+    snapshots between the passes in 16-bit with 64-column segment scales."""
     vp = _c(vp, np.float32)
     n2, n1 = vp.shape
     rec_tab = _c(rec_tab, np.int32)
     nr = rec_tab.shape[1]
     res = _c(res, _rdt(prec))
     sis = np.zeros((nr, nt), _rdt(prec))
-    rc = getattr(lib(), "oracle_esource_data_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr,
-                                                        rec_tab, sis)
+    fmt = SNAP_FORMATS[snap]
+    if fmt:
+        rc = getattr(lib(), "oracle_esource_data_snap_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr,
+                                                                 rec_tab, sis, fmt)
+    else:
+        rc = getattr(lib(), "oracle_esource_data_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, res, type_rec, nr,
+                                                            rec_tab, sis)
     if rc != 0:
         raise RuntimeError("oracle esource_data failed (out of memory?)")
     return sis
@@ -237,6 +256,31 @@ def linearminres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, itnlim, rtol=0.
     info = dict(fac=out[0], Anorm=out[1], Acond=out[2], rnorm=out[3], Arnorm=out[4], ynorm=out[5],
                 beta1=out[6], istop=int(iout[0]), itn=int(iout[1]))
     return x, hist[:info["itn"]], info
+
+
+def set_progress(on):
+    """Per-iteration progress lines on stderr from the long-running solvers."""
+    lib().oracle_set_progress(1 if on else 0)
+
+
+def snap_round(x, snap):
+    """16-bit round trip (ties to even) of float32 values |x| <= 1."""
+    x = np.array(x, np.float32).ravel()
+    lib().oracle_snap_round_array(x, x.size, SNAP_FORMATS[snap])
+    return x
+
+
+def snap_quantize(a, npml, snap):
+    """16-bit round trip of snapshots a[nt][n2][n1] (Fortran a(n1,n2,nt)) with
+    one scale per time step, row and 64-column segment of the extended grid."""
+    a = np.array(a, np.float32)
+    nt, n2, n1 = a.shape
+    lib().snap_quantize_f32(a, n1, n2, npml, nt, SNAP_FORMATS[snap])
+    return a
+
+
+def snap_scale(amax):
+    return float(lib().oracle_snap_scale(C.c_float(amax)))
 
 
 # ---------------------------------------------------------------- filter
