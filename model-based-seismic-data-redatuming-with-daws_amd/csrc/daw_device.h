@@ -26,6 +26,9 @@ enum SnapFormat { kSnapF32 = 0, kSnapBF16 = 1, kSnapF16 = 2 };
 // TX columns (one wavefront width) by TZ rows; TZ is 16 or 32, a launch
 // parameter the tile autotuner picks.
 constexpr int TX = 64;
+// Record blocks: items per workgroup and the LDS stride of one item's products.
+constexpr int kRecPerBlock = 8;
+constexpr int kRecStride = 444;   // 441 window products padded to whole float4s
 constexpr int kBlock = 256;
 
 struct ShotDev {
@@ -70,7 +73,7 @@ struct RecItem {
     int clip;           // 0: window clipped to the extended grid, 1: to the interior
     int reverse;        // write time index nt-1-it instead of it
     int pad;
-    const float *coef;  // 441 coefficients, (i1+10) + 21*(i2+10)
+    const float *coef;  // 441 coefficients, (i1+10)*21 + (i2+10) (record order)
     float *out;         // trace base (nt floats)
 };
 

@@ -192,6 +192,17 @@ struct PointSet {
     std::vector<float> coef;        // n * 441
 };
 
+// Coefficient tiles for the record blocks: each 21x21 tile in the order the
+// record sums it, (i1+10)*21 + (i2+10) (PointSet keeps (i1+10) + 21*(i2+10)).
+static int upload_record_coef(DevBuf &b, const std::vector<float> &coef)
+{
+    std::vector<float> t(coef.size());
+    for (size_t r = 0; r < coef.size() / 441; ++r)
+        for (int a = 0; a < 21; ++a)
+            for (int c = 0; c < 21; ++c) t[r * 441 + a * 21 + c] = coef[r * 441 + a + 21 * c];
+    return upload(b, t);
+}
+
 static int make_points(PointSet &P, const int *tab, int n, const Grid &G, bool interior_clip)
 {
     P.n = n;
@@ -345,7 +356,11 @@ static int set_items(SweepSet &W, const std::vector<RecItem> &items)
         if (int rc = W.d_items.alloc(std::max(bytes, 2 * W.d_items.bytes))) return rc;
     }
     HIPCHK(hipStreamSynchronize(g.stream));   // the host copy of the last upload is free
+    // receiver-major: the items of one receiver (one coefficient tile) for
+    // all shots run in adjacent wavefronts and share the tile in cache
     W.items = items;
+    std::stable_sort(W.items.begin(), W.items.end(),
+                     [](const RecItem &a, const RecItem &b) { return a.coef < b.coef; });
     if (!items.empty())
         HIPCHK(hipMemcpyAsync(W.d_items.p, W.items.data(), items.size() * sizeof(RecItem), hipMemcpyHostToDevice,
                               g.stream));
@@ -392,7 +407,7 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
     P.shots = W.d_shots.as<ShotDev>();
     P.tiles = W.d_tiles.as<int2>();
     P.rec = d_rec;
-    const int rec_blocks = (nrec + kBlock - 1) / kBlock;
+    const int rec_blocks = (nrec + kRecPerBlock - 1) / kRecPerBlock;
     int cur = 0;
     for (int it = 0; it <= nsteps; ++it) {
         P.ntiles = (it < nsteps) ? W.ntiles : 0;
@@ -479,7 +494,7 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
     P.shots = W.d_shots.as<ShotDev>();
     P.tiles = W.d_tiles.as<int2>();
     P.rec = d_rec;
-    const int rec_blocks = (nrec + kBlock - 1) / kBlock;
+    const int rec_blocks = (nrec + kRecPerBlock - 1) / kRecPerBlock;
     int cur = 0;
     for (int k = 0; k <= W.nt_max; ++k) {
         const int it = W.nt_max - 1 - k;
@@ -815,9 +830,9 @@ static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int n
     std::vector<const PointSet *> sets{&B.rec};
     if (B.dat.n) sets.push_back(&B.dat);
     if (int rc = make_band(B.band, sets, G)) return rc;
-    if (int rc = upload(B.d_rec_coef, B.rec.coef)) return rc;
+    if (int rc = upload_record_coef(B.d_rec_coef, B.rec.coef)) return rc;
     if (B.dat.n)
-        if (int rc = upload(B.d_dat_coef, B.dat.coef)) return rc;
+        if (int rc = upload_record_coef(B.d_dat_coef, B.dat.coef)) return rc;
     // the exact Hessian's transpose pass writes FP32 snapshots
     const int fmt = want_snap ? (g.hess_exact ? (int)kSnapF32 : g.snap_fmt) : -1;
     if (int rc = alloc_store(B.store, nshots, G, B.band.nslot, nt, fmt)) return rc;
@@ -1333,7 +1348,7 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
     Band sband;
     if (int rc = make_band(sband, {&src}, G)) return rc;
     DevBuf d_scoef, d_d, d_g, d_band, d_items;
-    if (int rc = upload(d_scoef, src.coef)) return rc;
+    if (int rc = upload_record_coef(d_scoef, src.coef)) return rc;
     if (int rc = d_d.alloc((size_t)nr * nt * 4)) return rc;
     HIPCHK(hipMemcpy(d_d.p, d, (size_t)nr * nt * 4, hipMemcpyHostToDevice));
     if (int rc = d_g.alloc((size_t)nt * 4)) return rc;

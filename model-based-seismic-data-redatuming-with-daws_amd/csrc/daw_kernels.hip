@@ -13,57 +13,124 @@ namespace daw {
 
 namespace {
 
+// Pointers the step kernels dereference are global memory: typed so, the
+// compiler emits global (not flat) loads and stores with 32-bit offsets, and
+// LDS waits do not wait on them.
+#define GLB __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GLB T *glb(T *p)
+{
+    return (GLB T *)p;
+}
+
+// Element idx of a global array addressed as a uniform base plus a 32-bit
+// byte offset (the scalar-base form of the global load and store).
+template <class T>
+__device__ __forceinline__ T ld(const GLB T *base, unsigned idx)
+{
+    return *(const GLB T *)((const GLB char *)base + idx * (unsigned)sizeof(T));
+}
+
+template <class T>
+__device__ __forceinline__ void st(GLB T *base, unsigned idx, T v)
+{
+    *(GLB T *)((GLB char *)base + idx * (unsigned)sizeof(T)) = v;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
 __device__ __forceinline__ bool in_box(int z, int x, int lo, int hiz, int hix)
 {
     return z >= lo && z <= hiz && x >= lo && x <= hix;
 }
 
 // ---------------------------------------------------------------------------
-// Record blocks: sis(it, r) = sum over the Hicks window of c * p, one
-// sequential chain i1 outer / i2 inner exactly as submodeling2.f90:214-219
-// (extended-grid clip) and subwavefieldresample.f90:32-37 (interior clip)
-// accumulate it.  Each 21-point window row is loaded first and summed after,
-// so a row costs one memory latency instead of 21.
+// Record blocks: the Hicks record of (shot, receiver) items at step rec_it,
+// kRecPerBlock items per workgroup.  sis(it, r) = sum over the 21x21 window
+// of c(i1,i2) * p(i1,i2), one sequential chain i1 outer / i2 inner exactly as
+// submodeling2.f90:214-219 (window clipped to the extended grid) and
+// subwavefieldresample.f90:32-37 (clipped to the interior) accumulate it.
+// The workgroup first forms every product of its items (all loads in flight
+// at once, product k of an item at (i1+10)*21 + (i2+10), coefficients stored
+// in that order) in LDS; then one thread per item runs the chain over its
+// products in k order, so the additions happen in the reference's order.  A
+// clipped-away entry contributes +0, which leaves the sum unchanged: a sum
+// that starts at +0 is never -0 under round-to-nearest.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void record_item(const StepParams &P, int k)
+struct RecSlot {
+    const GLB float *band;
+    const GLB int *brow;
+    const GLB float *coef;
+    GLB float *out;
+    int ix1, ix2, lo1, hi1, lo2, hi2, ldx, live;
+};
+
+__device__ __forceinline__ void record_block(const StepParams &P, int blk, float *s_lds)
 {
-    const RecItem R = P.rec[k];
-    const ShotDev *__restrict__ S = P.shots + R.shot;
-    const int nt = S->nt;
-    if (P.rec_it >= nt) return;
-    const int npml = S->npml, n1e = S->n1e, n2e = S->n2e, ldx = S->ldx;
-    int lo1, hi1, lo2, hi2;
-    if (R.clip == 0) {
-        lo1 = -R.ix1; hi1 = n1e - 1 - R.ix1;
-        lo2 = -R.ix2; hi2 = n2e - 1 - R.ix2;
-    } else {
-        lo1 = npml - R.ix1; hi1 = npml + S->n1 - 1 - R.ix1;
-        lo2 = npml - R.ix2; hi2 = npml + S->n2 - 1 - R.ix2;
-    }
-    lo1 = lo1 > -10 ? lo1 : -10; hi1 = hi1 < 10 ? hi1 : 10;
-    lo2 = lo2 > -10 ? lo2 : -10; hi2 = hi2 < 10 ? hi2 : 10;
-    const float *__restrict__ band = S->pband[P.rec_slot];
-    const int *__restrict__ brow = S->band_row;
-    const float *__restrict__ coef = R.coef;
-    float acc = 0.0f;
-    if (lo2 == -10 && hi2 == 10) {
-        for (int i1 = lo1; i1 <= hi1; ++i1) {
-            const float *row = band + (size_t)brow[R.ix1 + i1] * ldx + (R.ix2 - 10);
-            const float *c = coef + (i1 + 10);
-            float v[21], w[21];
-#pragma unroll
-            for (int j = 0; j < 21; ++j) { v[j] = row[j]; w[j] = c[21 * j]; }
-#pragma unroll
-            for (int j = 0; j < 21; ++j) acc = acc + w[j] * v[j];
+    __shared__ RecSlot s_slot[kRecPerBlock];
+    const int tid = threadIdx.x;
+    const int first = blk * kRecPerBlock;
+    if (tid < kRecPerBlock) {
+        RecSlot q;
+        q.live = 0;
+        const int item = first + tid;
+        if (item < P.nrec) {
+            const RecItem R = P.rec[item];
+            const ShotDev *__restrict__ S = P.shots + R.shot;
+            const int nt = S->nt;
+            if (P.rec_it < nt) {
+                q.live = 1;
+                if (R.clip == 0) {
+                    q.lo1 = -R.ix1; q.hi1 = S->n1e - 1 - R.ix1;
+                    q.lo2 = -R.ix2; q.hi2 = S->n2e - 1 - R.ix2;
+                } else {
+                    q.lo1 = S->npml - R.ix1; q.hi1 = S->npml + S->n1 - 1 - R.ix1;
+                    q.lo2 = S->npml - R.ix2; q.hi2 = S->npml + S->n2 - 1 - R.ix2;
+                }
+                q.band = glb(S->pband[P.rec_slot]);
+                q.brow = glb(S->band_row);
+                q.coef = glb(R.coef);
+                q.out = glb(R.out) + (R.reverse ? (nt - 1 - P.rec_it) : P.rec_it);
+                q.ix1 = R.ix1;
+                q.ix2 = R.ix2;
+                q.ldx = S->ldx;
+            }
         }
-    } else {
-        for (int i1 = lo1; i1 <= hi1; ++i1) {
-            const float *row = band + (size_t)brow[R.ix1 + i1] * ldx + R.ix2;
-            const float *c = coef + (i1 + 10);
-            for (int i2 = lo2; i2 <= hi2; ++i2) acc = acc + c[21 * (i2 + 10)] * row[i2];
+        s_slot[tid] = q;
+    }
+    __syncthreads();
+    constexpr int NPROD = kRecPerBlock * kRecStride;
+    constexpr int NJ = (NPROD + kBlock - 1) / kBlock;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int f = tid + kBlock * j;
+        if (f < NPROD) {
+            const int it = f / kRecStride, k = f - kRecStride * it;
+            const int i1 = k / 21 - 10, i2 = k - 21 * (k / 21) - 10;
+            const RecSlot &q = s_slot[it];
+            float v = 0.0f;
+            if (q.live && k < 441 && i1 >= q.lo1 && i1 <= q.hi1 && i2 >= q.lo2 && i2 <= q.hi2)
+                v = q.coef[k] * q.band[q.brow[q.ix1 + i1] * q.ldx + (q.ix2 + i2)];
+            s_lds[f] = v;
         }
     }
-    R.out[R.reverse ? (nt - 1 - P.rec_it) : P.rec_it] = acc;
+    __syncthreads();
+    if (tid < kRecPerBlock && s_slot[tid].live) {
+        const float4 *pr = reinterpret_cast<const float4 *>(s_lds + tid * kRecStride);
+        float acc = 0.0f;
+#pragma unroll 8
+        for (int m = 0; m < kRecStride / 4; ++m) {
+            const float4 w = pr[m];
+            acc = acc + w.x;
+            acc = acc + w.y;
+            acc = acc + w.z;
+            acc = acc + w.w;
+        }
+        *s_slot[tid].out = acc;
+    }
 }
 
 // Pressure half-step at one point of the pressure rectangle (the reference's
@@ -73,41 +140,33 @@ __device__ __forceinline__ void record_item(const StepParams &P, int k)
 // Flat per-launch view of one shot: every field a scalar or a pointer, the
 // double-buffer halves already selected, so it lives in registers.
 struct View {
-    int n1, n2, npml, n1e, n2e, ldx, nt, ofs, amp_stride, amp_rev, vinj;
-    int xlo1, xhi1, xlo2, xhi2;
+    int n1, n2, npml, n1e, n2e, ldx, nt, ofs, par;
     float dt, dtdx;
-    const float *kappa, *d1a, *d1a1, *d1b, *bw1, *d2a, *d2a1, *d2b, *bw2;
-    const float *pxo, *pzo, *vxo, *vzo;
-    float *pxn, *pzn, *vxn, *vzn;
-    const int *inj_row, *inj_ptr, *inj_r;
-    const float *inj_cdt, *inj_amp;
-    const int *band_row;
-    float *pband;
-    float *sp, *svx, *svz, *wf;
-    unsigned short *sp16, *svx16, *svz16;
-    float *ssc;
+    const GLB float *kappa, *d1a, *d1a1, *d1b, *bw1, *d2a, *d2a1, *d2b, *bw2;
+    const GLB float *pxo, *pzo, *vxo, *vzo;
+    GLB float *pxn, *pzn, *vxn, *vzn;
+    GLB float *sp, *svx, *svz;
+    GLB unsigned short *sp16, *svx16, *svz16;
+    GLB float *ssc;
     int nseg;
+    const ShotDev *__restrict__ sd;   // the rest (injection, record band, ranges), read where used
 };
 
 __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur, int it)
 {
     View v;
     v.n1 = S->n1; v.n2 = S->n2; v.npml = S->npml; v.n1e = S->n1e; v.n2e = S->n2e; v.ldx = S->ldx;
-    v.nt = S->nt; v.ofs = S->ofs; v.amp_stride = S->amp_stride; v.amp_rev = S->amp_rev; v.vinj = S->vinj;
-    v.xlo1 = S->xlo1; v.xhi1 = S->xhi1; v.xlo2 = S->xlo2; v.xhi2 = S->xhi2;
+    v.nt = S->nt; v.ofs = S->ofs; v.par = it & 1; v.sd = S;
     v.dt = S->dt; v.dtdx = S->dtdx;
-    v.kappa = S->kappa; v.d1a = S->d1a; v.d1a1 = S->d1a1; v.d1b = S->d1b; v.bw1 = S->bw1;
-    v.d2a = S->d2a; v.d2a1 = S->d2a1; v.d2b = S->d2b; v.bw2 = S->bw2;
+    v.kappa = glb(S->kappa); v.d1a = glb(S->d1a); v.d1a1 = glb(S->d1a1); v.d1b = glb(S->d1b); v.bw1 = glb(S->bw1);
+    v.d2a = glb(S->d2a); v.d2a1 = glb(S->d2a1); v.d2b = glb(S->d2b); v.bw2 = glb(S->bw2);
     const bool c1 = cur != 0;
-    v.pxo = c1 ? S->px[1] : S->px[0]; v.pzo = c1 ? S->pz[1] : S->pz[0];
-    v.vxo = c1 ? S->vx[1] : S->vx[0]; v.vzo = c1 ? S->vz[1] : S->vz[0];
-    v.pxn = c1 ? S->px[0] : S->px[1]; v.pzn = c1 ? S->pz[0] : S->pz[1];
-    v.vxn = c1 ? S->vx[0] : S->vx[1]; v.vzn = c1 ? S->vz[0] : S->vz[1];
-    v.inj_row = S->inj_row; v.inj_ptr = S->inj_ptr; v.inj_r = S->inj_r; v.inj_cdt = S->inj_cdt; v.inj_amp = S->inj_amp;
-    v.band_row = S->band_row;
-    v.pband = (it & 1) ? S->pband[1] : S->pband[0];
-    v.sp = S->sp; v.svx = S->svx; v.svz = S->svz; v.wf = S->wf;
-    v.sp16 = S->sp16; v.svx16 = S->svx16; v.svz16 = S->svz16; v.ssc = S->ssc; v.nseg = S->nseg;
+    v.pxo = glb(c1 ? S->px[1] : S->px[0]); v.pzo = glb(c1 ? S->pz[1] : S->pz[0]);
+    v.vxo = glb(c1 ? S->vx[1] : S->vx[0]); v.vzo = glb(c1 ? S->vz[1] : S->vz[0]);
+    v.pxn = glb(c1 ? S->px[0] : S->px[1]); v.pzn = glb(c1 ? S->pz[0] : S->pz[1]);
+    v.vxn = glb(c1 ? S->vx[0] : S->vx[1]); v.vzn = glb(c1 ? S->vz[0] : S->vz[1]);
+    v.sp = glb(S->sp); v.svx = glb(S->svx); v.svz = glb(S->svz);
+    v.sp16 = glb(S->sp16); v.svx16 = glb(S->svx16); v.svz16 = glb(S->svz16); v.ssc = glb(S->ssc); v.nseg = S->nseg;
     return v;
 }
 
@@ -150,46 +209,58 @@ __device__ __forceinline__ float snap_decode(unsigned short q)
     return (float)__builtin_bit_cast(_Float16, q);
 }
 
-// interior snapshot element and its segment scale
-__device__ __forceinline__ size_t snap_idx(const View &S, int t, int gz, int gx)
+// Interior snapshot element: plane t of an (n1*n2)-point snapshot array
+// (a uniform 64-bit base) plus a 32-bit offset; segment scales likewise.
+template <class T>
+__device__ __forceinline__ T *plane_at(T *base, const View &S, int t)
 {
-    return (size_t)t * ((size_t)S.n1 * S.n2) + (size_t)(gz - S.npml) * S.n2 + (gx - S.npml);
+    return base + (size_t)t * ((size_t)S.n1 * S.n2);
 }
 
-__device__ __forceinline__ size_t scale_idx(const View &S, int t, int field, int gz, int gx)
+__device__ __forceinline__ unsigned snap_off(const View &S, int gz, int gx)
 {
-    return (((size_t)t * 3 + field) * S.n1 + (gz - S.npml)) * S.nseg + (gx >> 6);
+    return (unsigned)((gz - S.npml) * S.n2 + (gx - S.npml));
+}
+
+__device__ __forceinline__ GLB float *scale_at(const View &S, int t, int field)
+{
+    return S.ssc + ((size_t)t * 3 + field) * ((size_t)S.n1 * S.nseg);
+}
+
+__device__ __forceinline__ unsigned scale_off(const View &S, int gz, int gx)
+{
+    return (unsigned)((gz - S.npml) * S.nseg + (gx >> 6));
 }
 
 template <int F>
-__device__ __forceinline__ float snap_load(const View &S, const float *f32, const unsigned short *f16, int field, int t,
-                                           int gz, int gx)
+__device__ __forceinline__ float snap_load(const View &S, const GLB float *f32, const GLB unsigned short *f16, int field,
+                                           int t, int gz, int gx)
 {
-    if (F == kSnapF32) return f32[snap_idx(S, t, gz, gx)];
-    return snap_decode<F>(f16[snap_idx(S, t, gz, gx)]) * S.ssc[scale_idx(S, t, field, gz, gx)];
+    if (F == kSnapF32) return ld(plane_at(f32, S, t), snap_off(S, gz, gx));
+    return snap_decode<F>(ld(plane_at(f16, S, t), snap_off(S, gz, gx))) * ld(scale_at(S, t, field), scale_off(S, gz, gx));
 }
 
 // Whole wavefront (all 64 lanes, converged, one 64-aligned column segment):
 // store v where `interior`.  A wavefront of a tile that reaches past the grid
 // (segment >= nseg) has no interior lane and writes no scale.
 template <int F>
-__device__ __forceinline__ void snap_store_wave(const View &S, float *f32, unsigned short *f16, int field, int t, int gz,
-                                                int gx, float v, bool interior, bool row_interior)
+__device__ __forceinline__ void snap_store_wave(const View &S, GLB float *f32, GLB unsigned short *f16, int field, int t,
+                                                int gz, int gx, float v, bool interior, bool row_interior)
 {
     if (F == kSnapF32) {
-        if (interior) f32[snap_idx(S, t, gz, gx)] = v;
+        if (interior) st(plane_at(f32, S, t), snap_off(S, gz, gx), v);
         return;
     }
     const float sc = pow2_scale(wave_max_abs(interior ? v : 0.0f));
-    if (interior) f16[snap_idx(S, t, gz, gx)] = snap_encode<F>(v * (1.0f / sc));
-    if (row_interior && (threadIdx.x & 63) == 0 && (gx >> 6) < S.nseg) S.ssc[scale_idx(S, t, field, gz, gx)] = sc;
+    if (interior) st(plane_at(f16, S, t), snap_off(S, gz, gx), snap_encode<F>(v * (1.0f / sc)));
+    if (row_interior && (threadIdx.x & 63) == 0 && (gx >> 6) < S.nseg) st(scale_at(S, t, field), scale_off(S, gz, gx), sc);
 }
 
 template <int MODE, int VZW_>
 __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, int gx, float pxv, float pzv,
                                          float kap, const float *vr, const float *vc, float d1a, float d1a1,
                                          float d2a, float d2a1, int injq, bool owned, int lo, int hiz, int hix,
-                                         float spv, float *__restrict__ pxn, float *__restrict__ pzn)
+                                         float spv, GLB float *__restrict__ pxn, GLB float *__restrict__ pzn)
 {
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
@@ -208,33 +279,33 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
         }
     }
     if (MODE != kReinject && injq >= 0) {
-        const int *ptr = S.inj_ptr + (size_t)injq * (S.n2e + 1);
+        const GLB int *ptr = glb(S.sd->inj_ptr) + injq * (S.n2e + 1);
         const int kb = ptr[gx], ke = ptr[gx + 1];
         for (int k = kb; k < ke; ++k) {
-            const float term = (S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + it]) * kap;
+            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it]) * kap;
             if (MODE == kBack) pxv = pxv - term;   // :1042
             else pxv = pxv + term;                 // :175
         }
     }
     if (MODE == kReinject) {
-        if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2)
+        if (gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gx >= S.sd->xlo2 && gx <= S.sd->xhi2)
             pxv = pxv + ((-S.dt) * spv) * kap;         // ss*dt*sp*kappa, ss = -1 (:1123)
     }
     float p = pxv + pzv;
     if (S.ofs == 1 && gz <= S.npml) p = 0.0f;           // p(1:npml+1,:) = 0
     if (owned) {
-        const size_t g = (size_t)gz * S.ldx + gx;
-        pxn[g] = pxv;
-        pzn[g] = pzv;
+        const unsigned g = (unsigned)(gz * S.ldx + gx);
+        st(pxn, g, pxv);
+        st(pzn, g, pzv);
         const int npml = S.npml;
         const bool interior = gz >= npml && gz < npml + S.n1 && gx >= npml && gx < npml + S.n2;
         if (MODE != kBack) {
-            if (S.band_row != nullptr) {
-                const int slot = S.band_row[gz];
-                if (slot >= 0) S.pband[(size_t)slot * S.ldx + gx] = p;
+            if (S.sd->band_row != nullptr) {
+                const int slot = glb(S.sd->band_row)[gz];
+                if (slot >= 0) glb(S.sd->pband[S.par])[slot * S.ldx + gx] = p;
             }
-            if (S.wf != nullptr && interior)
-                S.wf[(size_t)it * ((size_t)S.n1 * S.n2) + (size_t)(gx - npml) * S.n1 + (gz - npml)] = p;
+            if (S.sd->wf != nullptr && interior)
+                plane_at(glb(S.sd->wf), S, it)[(gx - npml) * S.n1 + (gz - npml)] = p;
         }
     }
     return p;
@@ -271,14 +342,16 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     constexpr int VROWS = TZ / NWZ;
     static_assert(TZ % NWZ == 0 && 3 * TZ <= kBlock, "tile rows must split evenly over the wavefront rows");
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int LDS_N = VXH_ * VXW_ + VZH_ * VZW_ + PH_ * PW_;
+    static_assert(LDS_N >= kRecPerBlock * kRecStride, "record blocks reuse the tile's LDS");
+    __shared__ __align__(16) float s_lds[LDS_N];
     if ((int)blockIdx.x < P.nrec_blocks) {
-        const int k = blockIdx.x * kBlock + tid;
-        if (k < P.nrec) record_item(P, k);
+        record_block(P, blockIdx.x, s_lds);
         return;
     }
-    __shared__ float s_vx[VXH_ * VXW_];
-    __shared__ float s_vz[VZH_ * VZW_];
-    __shared__ float s_p[PH_ * PW_];
+    float *const s_vx = s_lds;
+    float *const s_vz = s_lds + VXH_ * VXW_;
+    float *const s_p = s_vz + VZH_ * VZW_;
 
     const int2 t = P.tiles[blockIdx.x - P.nrec_blocks];
     const int it = P.it;
@@ -293,14 +366,14 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const int lo = 2;
     const int hiz = (MODE == kFwd) ? n1e - 3 : n1e - 4;
     const int hix = (MODE == kFwd) ? n2e - 3 : n2e - 4;
-    const float *__restrict__ vxo = S.vxo;
-    const float *__restrict__ vzo = S.vzo;
-    const float *__restrict__ pxo = S.pxo;
-    const float *__restrict__ pzo = S.pzo;
-    float *__restrict__ pxn = S.pxn;
-    float *__restrict__ pzn = S.pzn;
-    float *__restrict__ vxn = S.vxn;
-    float *__restrict__ vzn = S.vzn;
+    const GLB float *__restrict__ vxo = S.vxo;
+    const GLB float *__restrict__ vzo = S.vzo;
+    const GLB float *__restrict__ pxo = S.pxo;
+    const GLB float *__restrict__ pzo = S.pzo;
+    GLB float *__restrict__ pxn = S.pxn;
+    GLB float *__restrict__ pzn = S.pzn;
+    GLB float *__restrict__ vxn = S.vxn;
+    GLB float *__restrict__ vzn = S.vzn;
     const int ic = S.nt - 1 - it;   // time-reversed snapshot index (icount = nt-it+1)
     const int cl = (wv % TXW) * 64 + lane;   // this lane's column in the tile
     const int rz = wv / TXW;                  // this wavefront's first row
@@ -312,16 +385,21 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     // and (re-injection) the snapshot values
     constexpr int XR = (VXH_ + NWZ - 1) / NWZ, ZR = (VZH_ + NWZ - 1) / NWZ;
     const int npml = S.npml, n1 = S.n1, n2 = S.n2;
+    // loads outside the grid read a clamped in-grid address and are zeroed
+    // after: no branch around any load
+    const int gxc = colok ? gxl : n2e - 1;
     float vxl[XR], vzl[ZR];
 #pragma unroll
     for (int j = 0; j < XR; ++j) {
         const int r = rz + NWZ * j, gz = z0 - 1 + r;
-        vxl[j] = (r < VXH_ && gz >= 0 && gz < n1e && colok) ? vxo[(size_t)gz * ldx + gxl] : 0.0f;
+        const float v = ld(vxo, (unsigned)(clampi(gz, 0, n1e - 1) * ldx + gxc));
+        vxl[j] = (r < VXH_ && gz >= 0 && gz < n1e && colok) ? v : 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < ZR; ++j) {
         const int r = rz + NWZ * j, gz = z0 - 3 + r;
-        vzl[j] = (r < VZH_ && gz >= 0 && gz < n1e && colok) ? vzo[(size_t)gz * ldx + gxl] : 0.0f;
+        const float v = ld(vzo, (unsigned)(clampi(gz, 0, n1e - 1) * ldx + gxc));
+        vzl[j] = (r < VZH_ && gz >= 0 && gz < n1e && colok) ? v : 0.0f;
     }
     constexpr int HXN = (VXH_ * 6 + kBlock - 1) / kBlock, HZN = (VZH_ * 3 + kBlock - 1) / kBlock;
     float hx[HXN], hz[HZN];
@@ -336,7 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             const int c = k < 3 ? k : TXT + k;
             const int gz = z0 - 1 + r, gx = x0 - 3 + c;
             hx_i[q] = r * VXW_ + c;
-            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hx[q] = vxo[(size_t)gz * ldx + gx];
+            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hx[q] = ld(vxo, (unsigned)(gz * ldx + gx));
         }
     }
 #pragma unroll
@@ -349,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             const int c = k == 0 ? 0 : TXT + k;
             const int gz = z0 - 3 + r, gx = x0 - 1 + c;
             hz_i[q] = r * VZW_ + c;
-            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hz[q] = vzo[(size_t)gz * ldx + gx];
+            if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) hz[q] = ld(vzo, (unsigned)(gz * ldx + gx));
         }
     }
     float pxr[PROWS], pzr[PROWS], kar[PROWS], spr[PROWS];
@@ -357,14 +435,18 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     for (int j = 0; j < PROWS; ++j) {
         const int r = rz + NWZ * j;
         const int gz = z0 - 1 + r;
-        pxr[j] = 0.0f; pzr[j] = 0.0f; kar[j] = 0.0f; spr[j] = 0.0f;
-        if (r < PH_ && gz >= 0 && gz < n1e && colok) {
-            const size_t g = (size_t)gz * ldx + gxl;
-            pxr[j] = pxo[g];
-            pzr[j] = pzo[g];
-            kar[j] = S.kappa[g];
-            if (MODE == kReinject && gz >= S.xlo1 && gz <= S.xhi1 && gxl >= S.xlo2 && gxl <= S.xhi2)
-                spr[j] = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, gz, gxl);
+        const bool ok = r < PH_ && gz >= 0 && gz < n1e && colok;
+        const unsigned g = (unsigned)(clampi(gz, 0, n1e - 1) * ldx + gxc);
+        const float a = ld(pxo, g), b = ld(pzo, g), c = ld(S.kappa, g);
+        pxr[j] = ok ? a : 0.0f;
+        pzr[j] = ok ? b : 0.0f;
+        kar[j] = ok ? c : 0.0f;
+        spr[j] = 0.0f;
+        if (MODE == kReinject) {
+            const bool in = ok && gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gxl >= S.sd->xlo2 && gxl <= S.sd->xhi2;
+            const float v = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, clampi(gz, S.sd->xlo1, S.sd->xhi1),
+                                            clampi(gxl, S.sd->xlo2, S.sd->xhi2));
+            spr[j] = in ? v : 0.0f;
         }
     }
     // side points of the ring: columns x0-1, x0+TX, x0+TX+1 on the tile rows,
@@ -380,11 +462,11 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         s_gz = z0 - 1 + s_r;
         s_gx = x0 - 1 + s_c;
         if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
-            const size_t g = (size_t)s_gz * ldx + s_gx;
-            s_px = pxo[g];
-            s_pz = pzo[g];
-            s_ka = S.kappa[g];
-            if (MODE == kReinject && s_gz >= S.xlo1 && s_gz <= S.xhi1 && s_gx >= S.xlo2 && s_gx <= S.xhi2)
+            const int g = s_gz * ldx + s_gx;
+            s_px = ld(pxo, (unsigned)g);
+            s_pz = ld(pzo, (unsigned)g);
+            s_ka = ld(S.kappa, (unsigned)g);
+            if (MODE == kReinject && s_gz >= S.sd->xlo1 && s_gz <= S.sd->xhi1 && s_gx >= S.sd->xlo2 && s_gx <= S.sd->xhi2)
                 s_sp = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, s_gz, s_gx);
         }
     }
@@ -393,9 +475,13 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     for (int j = 0; j < VROWS; ++j) {
         const int gz = z0 + rz + NWZ * j;
         svxr[j] = 0.0f; svzr[j] = 0.0f;
-        if (MODE == kReinject && colok && gz >= npml && gz < npml + n1 && gxl >= npml && gxl < npml + n2) {
-            svxr[j] = snap_load<SNAP>(S, S.svx, S.svx16, 1, ic, gz, gxl);
-            svzr[j] = snap_load<SNAP>(S, S.svz, S.svz16, 2, ic, gz, gxl);
+        if (MODE == kReinject) {
+            const bool in = colok && gz >= npml && gz < npml + n1 && gxl >= npml && gxl < npml + n2;
+            const int cz = clampi(gz, npml, npml + n1 - 1), cx = clampi(gxl, npml, npml + n2 - 1);
+            const float a = snap_load<SNAP>(S, S.svx, S.svx16, 1, ic, cz, cx);
+            const float b = snap_load<SNAP>(S, S.svz, S.svz16, 2, ic, cz, cx);
+            svxr[j] = in ? a : 0.0f;
+            svzr[j] = in ? b : 0.0f;
         }
     }
 #pragma unroll
@@ -425,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             const int gz = z0 - 1 + r;
             float p = 0.0f;
             if (gz >= 0 && gz < n1e && colok) {
-                const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[gz] : -1;
+                const int injq = (MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj) ? glb(S.sd->inj_row)[gz] : -1;
                 p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + cl + 3],
                                         &s_vz[(r + 2) * VZW_ + cl + 1], S.d1a[gz], S.d1a1[gz], c_d2a, c_d2a1,
                                         injq, r >= 1 && r <= TZ, lo, hiz, hix, spr[j], pxn, pzn);
@@ -442,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     if (side) {
         float p = 0.0f;
         if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
-            const int injq = (MODE != kReinject && S.inj_row != nullptr && !S.vinj) ? S.inj_row[s_gz] : -1;
+            const int injq = (MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj) ? glb(S.sd->inj_row)[s_gz] : -1;
             p = p_point<MODE, VZW_>(S, it, ic, s_gz, s_gx, s_px, s_pz, s_ka, &s_vx[s_r * VXW_ + s_c + 2],
                                     &s_vz[(s_r + 2) * VZW_ + s_c], S.d1a[s_gz], S.d1a1[s_gz], S.d2a[s_gx],
                                     S.d2a1[s_gx], injq, false, lo, hiz, hix, s_sp, pxn, pzn);
@@ -461,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         const int r = rz + NWZ * j;
         const int gz = z0 + r, gx = gxl;
         if (gz >= n1e) break;
-        const size_t g = (size_t)gz * ldx + gx;
+        const unsigned g = (unsigned)(gz * ldx + gx);
         const bool rowi = gz >= npml && gz < npml + n1;
         const bool interior = rowi && gx >= npml && gx < npml + n2;
         float vxv = 0.0f, vzv = 0.0f;
@@ -480,24 +566,24 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
                     vzv = S.d1b[gz] * vzv + S.bw1[gz] * der1;
                 }
             }
-            if (MODE == kFwd && S.vinj && S.inj_row != nullptr) {
-                const int q = S.inj_row[gz];
+            if (MODE == kFwd && S.sd->vinj && S.sd->inj_row != nullptr) {
+                const int q = glb(S.sd->inj_row)[gz];
                 if (q >= 0) {
-                    const int *ptr = S.inj_ptr + (size_t)q * (n2e + 1);
+                    const GLB int *ptr = glb(S.sd->inj_ptr) + q * (n2e + 1);
                     for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
-                        vzv = vzv + S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + it];   // * bue (= 1), :252
+                        vzv = vzv + glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it];   // * bue (= 1), :252
                 }
             }
             if (MODE == kReinject && interior) {
                 vxv = vxv + (-S.dt) * svxr[j];                         // * bue (= 1), :1177
                 vzv = vzv + (-S.dt) * svzr[j];                         // :1184
             }
-            vxn[g] = vxv;
+            st(vxn, g, vxv);
             if (S.ofs == 1 && gz == npml - 1) {
                 // vz(npml,:) is overwritten by the mirror below; its owner writes nothing
             } else {
-                vzn[g] = vzv;
-                if (S.ofs == 1 && gz == npml) vzn[g - ldx] = vzv;     // vz(npml,:) = vz(npml+1,:)
+                st(vzn, g, vzv);
+                if (S.ofs == 1 && gz == npml) st(vzn, g - ldx, vzv);  // vz(npml,:) = vz(npml+1,:)
             }
         }
         if (MODE == kBack && rowi) {                          // svx, svz(:,:,it) (:1082-1087), whole wavefront
@@ -538,15 +624,17 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     constexpr int VZW_ = TXT + 3, VZH_ = TZ + 6;
     constexpr int VROWS = TZ / NWZ;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int LDS_N = VXH_ * VXW_ + VZH_ * VZW_ + 2 * PH_ * PW_;
+    static_assert(LDS_N >= kRecPerBlock * kRecStride, "record blocks reuse the tile's LDS");
+    __shared__ __align__(16) float s_lds[LDS_N];
     if ((int)blockIdx.x < P.nrec_blocks) {
-        const int k = blockIdx.x * kBlock + tid;
-        if (k < P.nrec) record_item(P, k);
+        record_block(P, blockIdx.x, s_lds);
         return;
     }
-    __shared__ float s_qvx[VXH_ * VXW_];
-    __shared__ float s_qvz[VZH_ * VZW_];
-    __shared__ float s_wx[PH_ * PW_];
-    __shared__ float s_wz[PH_ * PW_];
+    float *const s_qvx = s_lds;
+    float *const s_qvz = s_lds + VXH_ * VXW_;
+    float *const s_wx = s_qvz + VZH_ * VZW_;
+    float *const s_wz = s_wx + PH_ * PW_;
 
     const int2 t = P.tiles[blockIdx.x - P.nrec_blocks];
     const int it = P.it;
@@ -562,7 +650,6 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     const int gxl = x0 + cl;
     const bool colok = gxl < n2e;
     const int ic = S.nt - 1 - it;
-    const size_t plane = (size_t)n1 * n2;
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
     const bool fs = S.ofs == 1;
@@ -570,19 +657,19 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     auto qvz_at = [&](int gz, int gx) -> float {
         if (gz < 0 || gz >= n1e || gx < 0 || gx >= n2e) return 0.0f;
         if (fs && gz == npml - 1) return 0.0f;
-        const float v = S.vzo[(size_t)gz * ldx + gx];
-        return (fs && gz == npml) ? v + S.vzo[(size_t)(gz - 1) * ldx + gx] : v;
+        const float v = S.vzo[gz * ldx + gx];
+        return (fs && gz == npml) ? v + S.vzo[(gz - 1) * ldx + gx] : v;
     };
     for (int r = rz; r < VXH_; r += NWZ) {
         const int gz = z0 - 1 + r;
-        s_qvx[r * VXW_ + cl + 3] = (gz >= 0 && gz < n1e && colok) ? S.vxo[(size_t)gz * ldx + gxl] : 0.0f;
+        s_qvx[r * VXW_ + cl + 3] = (gz >= 0 && gz < n1e && colok) ? S.vxo[gz * ldx + gxl] : 0.0f;
     }
     for (int r = rz; r < VZH_; r += NWZ) s_qvz[r * VZW_ + cl + 1] = qvz_at(z0 - 3 + r, gxl);
     for (int i = tid; i < VXH_ * 6; i += kBlock) {
         const int r = i / 6, k = i - 6 * r;
         const int c = k < 3 ? k : TXT + k;
         const int gz = z0 - 1 + r, gx = x0 - 3 + c;
-        s_qvx[r * VXW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? S.vxo[(size_t)gz * ldx + gx] : 0.0f;
+        s_qvx[r * VXW_ + c] = (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) ? S.vxo[gz * ldx + gx] : 0.0f;
     }
     for (int i = tid; i < VZH_ * 3; i += kBlock) {
         const int r = i / 3, k = i - 3 * r;
@@ -596,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         const int gz = z0 - 1 + r, gx = x0 - 1 + c;
         float wx = 0.0f, wz = 0.0f;
         if (gz >= 0 && gz < n1e && gx >= 0 && gx < n2e) {
-            const size_t g = (size_t)gz * ldx + gx;
+            const unsigned g = (unsigned)(gz * ldx + gx);
             const float kap = S.kappa[g];
             // U = buw * qv on the box, zero outside
             auto ux = [&](int xx) -> float {
@@ -607,13 +694,13 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
             };
             float qp = a0 * ux(gx - 1) - a0 * ux(gx) + a1 * ux(gx - 2) - a1 * ux(gx + 1);
             qp += a0 * uz(gz - 1) - a0 * uz(gz) + a1 * uz(gz - 2) - a1 * uz(gz + 1);
-            if (S.inj_row != nullptr) {              // R^T d: adjoint data spread with the Hicks tiles
-                const int q = S.inj_row[gz];
+            if (S.sd->inj_row != nullptr) {              // R^T d: adjoint data spread with the Hicks tiles
+                const int q = glb(S.sd->inj_row)[gz];
                 if (q >= 0) {
-                    const int *ptr = S.inj_ptr + (size_t)q * (n2e + 1);
-                    const int ia = S.amp_rev ? S.nt - 1 - it : it;
+                    const GLB int *ptr = glb(S.sd->inj_ptr) + q * (n2e + 1);
+                    const int ia = S.sd->amp_rev ? S.nt - 1 - it : it;
                     for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
-                        qp += S.inj_cdt[k] * S.inj_amp[(size_t)S.inj_r[k] * S.amp_stride + ia];
+                        qp += glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + ia];
                 }
             }
             if (fs && gz <= npml) qp = 0.0f;          // C^T: M
@@ -627,13 +714,13 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
                 S.pzn[g] = box ? S.d1a[gz] * qpz : qpz;
                 const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
                 if (HESS) {
-                    if (gz >= S.xlo1 && gz <= S.xhi1 && gx >= S.xlo2 && gx <= S.xhi2)
-                        S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml)] = -S.dt * kap * qpx;
+                    if (gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gx >= S.sd->xlo2 && gx <= S.sd->xhi2)
+                        plane_at(S.sp, S, ic)[(gz - npml) * n2 + (gx - npml)] = -S.dt * kap * qpx;
                     else if (interior)
-                        S.sp[(size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml)] = 0.0f;
-                } else if (S.band_row != nullptr) {
-                    const int slot = S.band_row[gz];
-                    if (slot >= 0) S.pband[(size_t)slot * ldx + gx] = S.dt * kap * qpx;
+                        plane_at(S.sp, S, ic)[(gz - npml) * n2 + (gx - npml)] = 0.0f;
+                } else if (S.sd->band_row != nullptr) {
+                    const int slot = glb(S.sd->band_row)[gz];
+                    if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = S.dt * kap * qpx;
                 }
             }
         }
@@ -658,7 +745,7 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         const int r = rz + NWZ * j;
         const int gz = z0 + r, gx = gxl;
         if (gz >= n1e) break;
-        const size_t g = (size_t)gz * ldx + gx;
+        const unsigned g = (unsigned)(gz * ldx + gx);
         const float qvx = s_qvx[(r + 1) * VXW_ + cl + 3];
         const float qvz = s_qvz[(r + 3) * VZW_ + cl + 1];
         const bool box = in_box(gz, gx, lo, hiz, hix);
@@ -669,9 +756,9 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         S.vxn[g] = (box ? S.d2b[gx] * qvx : qvx) + ax;
         S.vzn[g] = (box ? S.d1b[gz] * qvz : qvz) + az;
         if (HESS && gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2) {
-            const size_t sidx = (size_t)ic * plane + (size_t)(gz - npml) * n2 + (gx - npml);
-            S.svx[sidx] = -S.dt * qvx;
-            S.svz[sidx] = -S.dt * qvz;
+            const int sidx = (gz - npml) * n2 + (gx - npml);
+            plane_at(S.svx, S, ic)[sidx] = -S.dt * qvx;
+            plane_at(S.svz, S, ic)[sidx] = -S.dt * qvz;
         }
     }
 }
