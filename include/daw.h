@@ -151,7 +151,8 @@ typedef struct daw_redatum_args {
     float *datae_dev;         /* device out or NULL: datar + dataesc [shot][nrec][nt] */
     float *deltade_dev;       /* device out or NULL: MINRES solutions [shot][nrec][nt] */
     double *hist_host;        /* host out or NULL: qrnorm/beta1 [shot][maxiter] */
-    int max_batch;            /* shots advanced together; <= 0: admit by the HBM budget */
+    int max_batch;            /* shots advanced together; <= 0: admit by the HBM budget, at most
+                                 as many as keep a launch's wave state within ~128 MB */
     double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
     int filter;               /* 1: band-pass the gathers as mode 2 does (filmartin2d) */
     float flow, fhig, llow, lhig;

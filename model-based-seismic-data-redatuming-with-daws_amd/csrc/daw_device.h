@@ -55,13 +55,16 @@ struct ShotDev {
     // record band: band_row[z] = slot >= 0 for rows some record window touches
     const int *band_row;
     float *pband[2];                    // [nslot][ldx], two time slots
-    // snapshots (kBack writes, kReinject reads), [nt][n1][n2]: FP32, or
-    // 16-bit (BF16 / FP16) with one power-of-two scale per 64-column row
-    // segment of each snapshot plane: ssc[((it*3 + field)*n1 + z)*nseg + x/64]
+    // snapshots (kBack writes, kReinject reads) of the interior rows, [nt][n1][spitch]:
+    // element (z, extended column gx) at z*spitch + gx - sxb, sxb = npml
+    // rounded down to 64, so a wavefront's 64 columns are one aligned run.
+    // FP32, or 16-bit (BF16 / FP16) with one power-of-two scale per 64-column
+    // row segment of each snapshot plane: ssc[((it*3 + field)*n1 + z)*nseg + gx/64]
     float *sp, *svx, *svz;
     unsigned short *sp16, *svx16, *svz16;
     float *ssc;
     int nseg;
+    int spitch, sxb;
     int xlo1, xhi1, xlo2, xhi2;         // pressure extended-source range, extended coords inclusive
     // optional full interior wavefield, Fortran (n1,n2,nt)
     float *wf;

@@ -118,6 +118,10 @@ struct Grid {
     float h, dt;
     size_t plane_e() const { return (size_t)n1e * ldx; }
     size_t plane_i() const { return (size_t)n1 * n2; }
+    // snapshot rows: extended columns sxb .. npml+n2-1, padded to whole 64-column segments
+    int sxb() const { return npml & ~63; }
+    int spitch() const { return (npml + n2 - sxb() + 63) & ~63; }
+    size_t plane_s() const { return (size_t)n1 * spitch(); }
 };
 
 static Grid make_grid(int n1, int n2, int npml, float h, float dt, int ofs)
@@ -534,8 +538,8 @@ static int snap_segments(const Grid &G) { return (G.n2e + TX - 1) / TX; }
 static size_t snap_bytes(const Grid &G, int nt, int fmt)
 {
     if (fmt < 0) return 0;
-    if (fmt == kSnapF32) return 3 * G.plane_i() * nt * sizeof(float);
-    return 3 * G.plane_i() * nt * 2 + 3 * (size_t)nt * G.n1 * snap_segments(G) * sizeof(float);
+    if (fmt == kSnapF32) return 3 * G.plane_s() * nt * sizeof(float);
+    return 3 * G.plane_s() * nt * 2 + 3 * (size_t)nt * G.n1 * snap_segments(G) * sizeof(float);
 }
 
 static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int nt, int fmt)
@@ -545,9 +549,9 @@ static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int 
     if (int rc = st.band.alloc((size_t)nshots * 2 * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
     st.fmt = fmt;
     if (fmt == kSnapF32) {
-        if (int rc = st.snap.alloc((size_t)nshots * 3 * G.plane_i() * nt * sizeof(float))) return rc;
+        if (int rc = st.snap.alloc((size_t)nshots * 3 * G.plane_s() * nt * sizeof(float))) return rc;
     } else if (fmt > 0) {
-        if (int rc = st.snap16.alloc((size_t)nshots * 3 * G.plane_i() * nt * 2)) return rc;
+        if (int rc = st.snap16.alloc((size_t)nshots * 3 * G.plane_s() * nt * 2)) return rc;
         if (int rc = st.scale.alloc((size_t)nshots * 3 * nt * G.n1 * snap_segments(G) * sizeof(float))) return rc;
     }
     return 0;
@@ -573,7 +577,9 @@ static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &s
         S.pband[0] = b;
         S.pband[1] = b + (size_t)std::max(nslot, 1) * G.ldx;
     }
-    const size_t pn = G.plane_i() * nt;
+    const size_t pn = G.plane_s() * nt;
+    S.spitch = G.spitch();
+    S.sxb = G.sxb();
     if (st.fmt == kSnapF32) {
         float *sn = st.snap.as<float>() + (size_t)s * 3 * pn;
         S.sp = sn; S.svx = sn + pn; S.svz = sn + 2 * pn;
@@ -1220,6 +1226,13 @@ int daw_redatum_batch(const daw_redatum_args *a)
         HIPCHK(hipMemGetInfo(&freeb, &total));
         const double budget = a->hbm_budget_bytes > 0 ? (double)a->hbm_budget_bytes : 0.85 * (double)freeb;
         maxb = (int)std::max(1.0, std::floor(budget / (double)per_shot));
+        // shots per launch: no more than keep a launch's double-buffered wave
+        // state within ~128 MB, so that the state written by one step is
+        // still in the 256 MB last-level cache when the next step reads it
+        // (measured at 201x801: 16 shots per launch beat 32 on the Hessian
+        // sweeps); at least 8, so a launch still fills the GPU
+        const double state = 8.0 * (double)G.plane_e() * 4.0;
+        maxb = std::min(maxb, std::max(8, (int)(128e6 / state)));
     }
     maxb = std::min(maxb, a->nshots);
     {   // equal-sized sub-batches

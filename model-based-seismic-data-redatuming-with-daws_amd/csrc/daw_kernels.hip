@@ -148,7 +148,7 @@ struct View {
     GLB float *sp, *svx, *svz;
     GLB unsigned short *sp16, *svx16, *svz16;
     GLB float *ssc;
-    int nseg;
+    int nseg, spitch, sxb;
     const ShotDev *__restrict__ sd;   // the rest (injection, record band, ranges), read where used
 };
 
@@ -167,19 +167,39 @@ __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur
     v.vxn = glb(c1 ? S->vx[0] : S->vx[1]); v.vzn = glb(c1 ? S->vz[0] : S->vz[1]);
     v.sp = glb(S->sp); v.svx = glb(S->svx); v.svz = glb(S->svz);
     v.sp16 = glb(S->sp16); v.svx16 = glb(S->svx16); v.svz16 = glb(S->svz16); v.ssc = glb(S->ssc); v.nseg = S->nseg;
+    v.spitch = S->spitch; v.sxb = S->sxb;
     return v;
 }
 
 
 // ---- 16-bit snapshot codec.  Each 64-column row segment of a snapshot plane
-// is written by one wavefront, which finds its max |value| with a shuffle
+// is written by one wavefront, which finds its max |value| with a DPP
 // reduction and stores the values divided by the power of two at or above
 // it (an exact rescaling) as BF16 or FP16, round to nearest even.
+//
+// Max of |v| over the wavefront: non-negative floats order like their bit
+// patterns, so the integer maximum is taken (NaN counts as 0, as fmaxf
+// ignores it).  Prefix maxima within each 16-lane row (row_shr 1, 2, 4, 8),
+// then row 0 into row 1 and row 2 into row 3 (row_bcast:15), then row 1 into
+// rows 2-3 (row_bcast:31): lane 63 holds the maximum.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_max(int x)
+{
+    const int y = __builtin_amdgcn_update_dpp(0, x, CTRL, ROW_MASK, 0xf, false);
+    return x > y ? x : y;
+}
+
 __device__ __forceinline__ float wave_max_abs(float v)
 {
-    v = fabsf(v);
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    int x = __float_as_int(fabsf(v));
+    x = x > 0x7f800000 ? 0 : x;
+    x = dpp_max<0x111, 0xf>(x);   // row_shr:1
+    x = dpp_max<0x112, 0xf>(x);   // row_shr:2
+    x = dpp_max<0x114, 0xf>(x);   // row_shr:4
+    x = dpp_max<0x118, 0xf>(x);   // row_shr:8
+    x = dpp_max<0x142, 0xa>(x);   // row_bcast:15
+    x = dpp_max<0x143, 0xc>(x);   // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
 __device__ __forceinline__ float pow2_scale(float amax)
@@ -217,9 +237,16 @@ __device__ __forceinline__ T *plane_at(T *base, const View &S, int t)
     return base + (size_t)t * ((size_t)S.n1 * S.n2);
 }
 
+// snapshot plane t (rows of spitch elements) and the element (gz, gx) in it
+template <class T>
+__device__ __forceinline__ T *splane_at(T *base, const View &S, int t)
+{
+    return base + (size_t)t * ((size_t)S.n1 * S.spitch);
+}
+
 __device__ __forceinline__ unsigned snap_off(const View &S, int gz, int gx)
 {
-    return (unsigned)((gz - S.npml) * S.n2 + (gx - S.npml));
+    return (unsigned)((gz - S.npml) * S.spitch + (gx - S.sxb));
 }
 
 __device__ __forceinline__ GLB float *scale_at(const View &S, int t, int field)
@@ -236,8 +263,19 @@ template <int F>
 __device__ __forceinline__ float snap_load(const View &S, const GLB float *f32, const GLB unsigned short *f16, int field,
                                            int t, int gz, int gx)
 {
-    if (F == kSnapF32) return ld(plane_at(f32, S, t), snap_off(S, gz, gx));
-    return snap_decode<F>(ld(plane_at(f16, S, t), snap_off(S, gz, gx))) * ld(scale_at(S, t, field), scale_off(S, gz, gx));
+    if (F == kSnapF32) return ld(splane_at(f32, S, t), snap_off(S, gz, gx));
+    return snap_decode<F>(ld(splane_at(f16, S, t), snap_off(S, gz, gx))) * ld(scale_at(S, t, field), scale_off(S, gz, gx));
+}
+
+// The same for a whole wavefront on one snapshot row gz (uniform) of its
+// 64-column segment wseg (uniform): the scale is one scalar load.
+template <int F>
+__device__ __forceinline__ float snap_load_w(const View &S, const GLB float *f32, const GLB unsigned short *f16, int field,
+                                             int t, int gz, int gx, int wseg)
+{
+    if (F == kSnapF32) return ld(splane_at(f32, S, t), snap_off(S, gz, gx));
+    const unsigned so = (unsigned)__builtin_amdgcn_readfirstlane((gz - S.npml) * S.nseg + wseg);
+    return snap_decode<F>(ld(splane_at(f16, S, t), snap_off(S, gz, gx))) * ld(scale_at(S, t, field), so);
 }
 
 // Whole wavefront (all 64 lanes, converged, one 64-aligned column segment):
@@ -248,11 +286,11 @@ __device__ __forceinline__ void snap_store_wave(const View &S, GLB float *f32, G
                                                 int gz, int gx, float v, bool interior, bool row_interior)
 {
     if (F == kSnapF32) {
-        if (interior) st(plane_at(f32, S, t), snap_off(S, gz, gx), v);
+        if (interior) st(splane_at(f32, S, t), snap_off(S, gz, gx), v);
         return;
     }
     const float sc = pow2_scale(wave_max_abs(interior ? v : 0.0f));
-    if (interior) st(plane_at(f16, S, t), snap_off(S, gz, gx), snap_encode<F>(v * (1.0f / sc)));
+    if (interior) st(splane_at(f16, S, t), snap_off(S, gz, gx), snap_encode<F>(v * (1.0f / sc)));
     if (row_interior && (threadIdx.x & 63) == 0 && (gx >> 6) < S.nseg) st(scale_at(S, t, field), scale_off(S, gz, gx), sc);
 }
 
@@ -379,6 +417,9 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const int rz = wv / TXW;                  // this wavefront's first row
     const int gxl = x0 + cl;
     const bool colok = gxl < n2e;
+    // this wavefront's 64-column snapshot segment (lanes past the grid read
+    // the last one and discard the value)
+    const int wseg = __builtin_amdgcn_readfirstlane(min((x0 >> 6) + (wv % TXW), S.nseg - 1));
 
     // ---- 1. every global load of the step is issued here, into registers,
     // before anything waits on one: velocity halos, the pressure rectangle,
@@ -444,8 +485,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         spr[j] = 0.0f;
         if (MODE == kReinject) {
             const bool in = ok && gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gxl >= S.sd->xlo2 && gxl <= S.sd->xhi2;
-            const float v = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, clampi(gz, S.sd->xlo1, S.sd->xhi1),
-                                            clampi(gxl, S.sd->xlo2, S.sd->xhi2));
+            const float v = snap_load_w<SNAP>(S, S.sp, S.sp16, 0, ic, clampi(gz, S.sd->xlo1, S.sd->xhi1),
+                                              clampi(gxl, S.sd->xlo2, S.sd->xhi2), wseg);
             spr[j] = in ? v : 0.0f;
         }
     }
@@ -478,8 +519,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         if (MODE == kReinject) {
             const bool in = colok && gz >= npml && gz < npml + n1 && gxl >= npml && gxl < npml + n2;
             const int cz = clampi(gz, npml, npml + n1 - 1), cx = clampi(gxl, npml, npml + n2 - 1);
-            const float a = snap_load<SNAP>(S, S.svx, S.svx16, 1, ic, cz, cx);
-            const float b = snap_load<SNAP>(S, S.svz, S.svz16, 2, ic, cz, cx);
+            const float a = snap_load_w<SNAP>(S, S.svx, S.svx16, 1, ic, cz, cx, wseg);
+            const float b = snap_load_w<SNAP>(S, S.svz, S.svz16, 2, ic, cz, cx, wseg);
             svxr[j] = in ? a : 0.0f;
             svzr[j] = in ? b : 0.0f;
         }
@@ -715,9 +756,9 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
                 const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
                 if (HESS) {
                     if (gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gx >= S.sd->xlo2 && gx <= S.sd->xhi2)
-                        plane_at(S.sp, S, ic)[(gz - npml) * n2 + (gx - npml)] = -S.dt * kap * qpx;
+                        st(splane_at(S.sp, S, ic), snap_off(S, gz, gx), -S.dt * kap * qpx);
                     else if (interior)
-                        plane_at(S.sp, S, ic)[(gz - npml) * n2 + (gx - npml)] = 0.0f;
+                        st(splane_at(S.sp, S, ic), snap_off(S, gz, gx), 0.0f);
                 } else if (S.sd->band_row != nullptr) {
                     const int slot = glb(S.sd->band_row)[gz];
                     if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = S.dt * kap * qpx;
@@ -756,9 +797,9 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         S.vxn[g] = (box ? S.d2b[gx] * qvx : qvx) + ax;
         S.vzn[g] = (box ? S.d1b[gz] * qvz : qvz) + az;
         if (HESS && gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2) {
-            const int sidx = (gz - npml) * n2 + (gx - npml);
-            plane_at(S.svx, S, ic)[sidx] = -S.dt * qvx;
-            plane_at(S.svz, S, ic)[sidx] = -S.dt * qvz;
+            const unsigned sidx = snap_off(S, gz, gx);
+            st(splane_at(S.svx, S, ic), sidx, -S.dt * qvx);
+            st(splane_at(S.svz, S, ic), sidx, -S.dt * qvz);
         }
     }
 }
