@@ -187,6 +187,7 @@ def main():
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0, help="tile rows, 0 for the autotuner")
     ap.add_argument("--tile-cols", type=int, default=64)
+    ap.add_argument("--launch-steps", type=int, default=0, help="time steps per sweep launch: 1, 2, 0 for the autotuner")
     ap.add_argument("--snapshots", choices=["fp32", "bf16", "fp16"], default="fp32",
                     help="storage of the Hessian operator's wavefield snapshots")
     ap.add_argument("--graph", action="store_true", help="replay the sweeps as captured hipGraphs")
@@ -203,6 +204,7 @@ def main():
         calibrate_copy()
         return
     daw, syn = _pkg()
+    daw.set_steps(args.launch_steps)
     daw.set_tile(args.tile_rows, args.tile_cols)
     daw.set_snapshot_format(args.snapshots)
     daw.set_graph(args.graph)
@@ -228,22 +230,26 @@ def main():
     daw.set_timing(False)
     st = daw.stats()
     tile = list(daw.get_tile())
+    launch_steps = daw.get_steps()
     nsh = P["nsh"]
     shots_per_hour = nsh * args.steps / elapsed * 3600.0
     gpts = st["point_updates"] / elapsed / 1e9
     n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml
     mb = st["max_batch_used"]
-    # dominant kernel: back-propagation or re-injection step (one launch = one
-    # time step for all shots of a sub-batch)
-    modes = {1: "k_step<1> (back-propagation)", 2: "k_step<2> (re-injection)"}
+    # dominant kernel: back-propagation or re-injection sweep; one launch
+    # advances all shots of a sub-batch by launch_steps time steps
+    kname = "k_step2" if launch_steps == 2 else "k_step"
+    modes = {1: f"{kname}<1> (back-propagation)", 2: f"{kname}<2> (re-injection)"}
     dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
-    avg_ms = st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1)
-    # state r+w 32 B + kappa 4 B per extended point; snapshots of the interior
-    # (3 fields, 4 B each, or 2 B each + one 4 B scale per 64 columns)
+    avg_ms = st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1) * launch_steps   # per launch
+    # algorithmic bytes of one launch per shot: the wave state read and written
+    # once (px, pz, vx, vz: 32 B) and kappa read (4 B) per extended point, plus
+    # each step's snapshots of the interior (3 fields, 4 B each, or 2 B each +
+    # one 4 B scale per 64 columns)
     snap_b = 12.0 if args.snapshots == "fp32" else 6.0 + 12.0 / 64.0
-    bytes_per_shot_step = 36.0 * n1e * n2e + snap_b * w.nz * w.nx
+    bytes_per_shot_launch = 36.0 * n1e * n2e + launch_steps * snap_b * w.nz * w.nx
     shots_per_launch = nsh / int(np.ceil(nsh / mb))
-    alg_bytes = bytes_per_shot_step * shots_per_launch
+    alg_bytes = bytes_per_shot_launch * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # quality: redatumed (unfiltered) gathers vs data modelled at the datum in the true model
     err = float(torch.linalg.norm(datum - P["dnew"]) / torch.linalg.norm(P["dnew"]))
@@ -293,6 +299,7 @@ def main():
         "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m", "shots": nsh, "nt": w.nt,
                    "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
                    "shots_per_launch": shots_per_launch, "snapshots": args.snapshots, "graph": args.graph,
+                   "steps_per_launch": launch_steps,
                    "tile": tile},
         "gpts_per_s": gpts,
         "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
@@ -304,7 +311,8 @@ def main():
         "minres_final_rel_residual_mean": float(np.mean(hist[:, -1])) if hist is not None else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None, "kernel": modes[dom],
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes},
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes,
+                     "steps_per_launch": launch_steps},
         "cpu_baseline": cpu,
     }
     if mixed:

@@ -206,6 +206,13 @@ int daw_set_graph(int on);
  * counterpart: the launch shape of the autotuner. */
 int daw_set_tile(int rows, int cols);
 int daw_get_tile(int *rows, int *cols);
+/* Time steps advanced per launch of the sweep kernels: 1 (one launch per
+ * step), 2 (the two-step kernel: the wave state crosses HBM once per two
+ * steps; tiles 16x64, 32x64, 16x128) or 0 (default) to let the
+ * probe of daw_redatum_batch pick it with the tile.  Results are identical.
+ * No reference counterpart. */
+int daw_set_steps(int steps);
+int daw_get_steps(void);
 int daw_reset_stats(void);
 int daw_get_stats(daw_stats *st);
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
     "daw_get_stats", "daw_modeling_batch", "daw_set_tile", "daw_get_tile", "daw_filmartin2d_dev",
     "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
-    "daw_set_graph",
+    "daw_set_graph", "daw_set_steps", "daw_get_steps",
 ]
 
 
@@ -214,6 +214,15 @@ def submodeling_transpose(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab):
 def set_hessian(exact):
     """0: the reference's Hessian-type operator; 1: exact (symmetric) variant."""
     _check(lib().daw_set_hessian(1 if exact else 0))
+
+
+def set_steps(steps):
+    """Time steps per launch of the sweep kernels: 1, 2 or 0 (autotuned)."""
+    _check(lib().daw_set_steps(int(steps)))
+
+
+def get_steps():
+    return int(lib().daw_get_steps())
 
 
 def set_graph(on):

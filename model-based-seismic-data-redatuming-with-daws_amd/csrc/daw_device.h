@@ -54,7 +54,7 @@ struct ShotDev {
     int vinj;                           // 1: the source is a vz source (type_src = 1), applied in the velocity phase
     // record band: band_row[z] = slot >= 0 for rows some record window touches
     const int *band_row;
-    float *pband[2];                    // [nslot][ldx], two time slots
+    float *pband[4];                    // [nslot][ldx], time slot it & 3
     // snapshots (kBack writes, kReinject reads) of the interior rows, [nt][n1][spitch]:
     // element (z, extended column gx) at z*spitch + gx - sxb, sxb = npml
     // rounded down to 64, so a wavefront's 64 columns are one aligned run.
@@ -84,12 +84,13 @@ struct StepParams {
     const ShotDev *shots;
     const int2 *tiles;  // (shot, tile) in XCD-aware launch order
     int ntiles;
-    int it;             // step advanced by the tile blocks
+    int it;             // first step advanced by the tile blocks
+    int nsub;           // steps advanced by this launch (1, or 2 for the two-step kernel)
     int cur;            // state buffers read
-    const RecItem *rec; // record of step rec_it (pband slot rec_slot) by the trailing blocks
-    int nrec;
+    const RecItem *rec; // record of steps rec_it .. rec_it+rec_nsub-1 (pband slot step & 3)
+    int nrec;           // items per recorded step
     int rec_it;
-    int rec_slot;
+    int rec_nsub;
     int nrec_blocks;
 };
 

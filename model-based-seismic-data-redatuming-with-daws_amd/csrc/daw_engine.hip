@@ -61,10 +61,12 @@ struct Global {
     int tz = 16;               // tile rows of k_step
     int txw = 1;               // tile width in wavefronts (64 columns each)
     bool tz_auto = true;       // pick tz per batch shape with a timed probe
+    int steps = 1;             // time steps per launch of the sweep kernels (1: k_step, 2: k_step2)
+    bool steps_auto = true;    // let the probe pick the steps per launch too
     bool hess_exact = false;   // Hessian pass 1 = exact transpose of pass 2
     int snap_fmt = kSnapF32;   // snapshot store of the reference Hessian (SnapFormat)
     bool graph = false;        // replay full sweeps as captured hipGraphs
-    std::map<std::vector<int>, std::pair<int, int>> tz_cache;
+    std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw}
 };
 static Global g;
 
@@ -349,6 +351,9 @@ struct SweepSet {
     }
 };
 
+// Steps per launch of the sweeps: two only where the tile has a two-step kernel.
+static int steps_per_launch() { return (g.steps == 2 && tile2_shape_ok(g.tz, g.txw)) ? 2 : 1; }
+
 // Copies the record items of the next sweep to the batch's item buffer
 // (stream-ordered: the previous sweep has read the old items by then).
 static int set_items(SweepSet &W, const std::vector<RecItem> &items)
@@ -411,19 +416,26 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
     P.shots = W.d_shots.as<ShotDev>();
     P.tiles = W.d_tiles.as<int2>();
     P.rec = d_rec;
-    const int rec_blocks = (nrec + kRecPerBlock - 1) / kRecPerBlock;
-    int cur = 0;
-    for (int it = 0; it <= nsteps; ++it) {
-        P.ntiles = (it < nsteps) ? W.ntiles : 0;
+    P.nrec = nrec;
+    // launch k advances steps it .. it+nsub-1 and records the steps of launch k-1
+    const int per = steps_per_launch();
+    int cur = 0, prev_it = 0, prev_n = 0;
+    for (int it = 0;; it += per) {
+        const bool more = it < nsteps;
+        const int n = more ? std::min(per, nsteps - it) : 0;
+        P.ntiles = more ? W.ntiles : 0;
         P.it = it;
+        P.nsub = n;
         P.cur = cur;
-        const bool rec_prev = it > 0 && nrec > 0;
-        P.nrec = rec_prev ? nrec : 0;
-        P.nrec_blocks = rec_prev ? rec_blocks : 0;
-        P.rec_it = it - 1;
-        P.rec_slot = (it - 1) & 1;
-        launch_step(mode, W.snap, P, g.tz, g.txw, g.stream);
+        P.rec_it = prev_it;
+        P.rec_nsub = nrec > 0 ? prev_n : 0;
+        P.nrec_blocks = (nrec * P.rec_nsub + kRecPerBlock - 1) / kRecPerBlock;
+        if (per == 2) launch_step2(mode, W.snap, P, g.tz, g.txw, g.stream);
+        else launch_step(mode, W.snap, P, g.tz, g.txw, g.stream);
         cur ^= 1;
+        prev_it = it;
+        prev_n = n;
+        if (!more) break;
     }
 }
 
@@ -441,7 +453,7 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
     }
     const int nsteps = nsteps_limit > 0 ? std::min(nsteps_limit, W.nt_max) : W.nt_max;
     if (g.graph && nsteps_limit == 0) {
-        const std::vector<long long> key{mode, nsteps, nrec, g.tz, g.txw, W.snap, W.ntiles, (long long)(intptr_t)d_rec,
+        const std::vector<long long> key{mode, nsteps, nrec, steps_per_launch(), g.tz, g.txw, W.snap, W.ntiles, (long long)(intptr_t)d_rec,
                                          (long long)(intptr_t)W.d_tiles.p, (long long)(intptr_t)W.d_shots.p};
         auto hit = W.graphs.find(key);
         hipGraphExec_t ex = nullptr;
@@ -478,7 +490,7 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
         (void)hipEventDestroy(e1);
     }
     g.pts += W.pts_per_step * nsteps;
-    g.launches += nsteps + 1;
+    g.launches += (nsteps + steps_per_launch() - 1) / steps_per_launch() + 1;
     return 0;
 }
 
@@ -500,16 +512,17 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
     P.rec = d_rec;
     const int rec_blocks = (nrec + kRecPerBlock - 1) / kRecPerBlock;
     int cur = 0;
+    P.nrec = nrec;
+    P.nsub = 1;
     for (int k = 0; k <= W.nt_max; ++k) {
         const int it = W.nt_max - 1 - k;
         P.ntiles = (k < W.nt_max) ? W.ntiles : 0;
         P.it = it < 0 ? 0 : it;
         P.cur = cur;
         const bool rec_prev = k > 0 && nrec > 0;
-        P.nrec = rec_prev ? nrec : 0;
+        P.rec_nsub = rec_prev ? 1 : 0;
         P.nrec_blocks = rec_prev ? rec_blocks : 0;
         P.rec_it = it + 1;
-        P.rec_slot = (it + 1) & 1;
         launch_adjoint(hess, P, g.tz, g.txw, g.stream);
         cur ^= 1;
     }
@@ -546,7 +559,7 @@ static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int 
 {
     const size_t pe = G.plane_e();
     if (int rc = st.fields.alloc((size_t)nshots * 8 * pe * sizeof(float))) return rc;
-    if (int rc = st.band.alloc((size_t)nshots * 2 * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
+    if (int rc = st.band.alloc((size_t)nshots * 4 * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
     st.fmt = fmt;
     if (fmt == kSnapF32) {
         if (int rc = st.snap.alloc((size_t)nshots * 3 * G.plane_s() * nt * sizeof(float))) return rc;
@@ -573,9 +586,8 @@ static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &s
     S.px[1] = f + 4 * pe; S.pz[1] = f + 5 * pe; S.vx[1] = f + 6 * pe; S.vz[1] = f + 7 * pe;
     if (band) {
         S.band_row = band->d_row.as<int>();
-        float *b = st.band.as<float>() + (size_t)s * 2 * std::max(nslot, 1) * G.ldx;
-        S.pband[0] = b;
-        S.pband[1] = b + (size_t)std::max(nslot, 1) * G.ldx;
+        float *b = st.band.as<float>() + (size_t)s * 4 * std::max(nslot, 1) * G.ldx;
+        for (int k = 0; k < 4; ++k) S.pband[k] = b + (size_t)k * std::max(nslot, 1) * G.ldx;
     }
     const size_t pn = G.plane_s() * nt;
     S.spitch = G.spitch();
@@ -684,13 +696,26 @@ struct Batch {
 // DAW_TUNE_LOG=1 prints the probe times to stderr.
 static int autotune_tiles(Batch &B)
 {
-    if (!g.tz_auto) return 0;
-    const std::vector<int> key{B.G.n1e, B.G.n2e, B.nshots};
+    if (!g.tz_auto && !g.steps_auto) return 0;
+    // candidates: {steps, rows, wavefronts across}; fixed settings are kept
+    static const int shapes[][3] = {{1, 8, 2}, {1, 16, 1}, {1, 16, 2}, {1, 32, 1}, {1, 32, 2}, {1, 64, 1},
+                                    {2, 16, 1}, {2, 32, 1}, {2, 16, 2}};
+    std::vector<std::vector<int>> cand;
+    for (const auto &sh : shapes) {
+        if (!g.steps_auto && sh[0] != g.steps) continue;
+        if (!g.tz_auto && (sh[1] != g.tz || sh[2] != g.txw)) continue;
+        cand.push_back({sh[0], sh[1], sh[2]});
+    }
+    if (cand.empty()) return 0;
+    const std::vector<int> key{B.G.n1e, B.G.n2e, B.nshots, g.steps_auto ? 0 : g.steps, g.tz_auto ? 0 : g.tz,
+                               g.tz_auto ? 0 : g.txw};
     auto hit = g.tz_cache.find(key);
     if (hit != g.tz_cache.end()) {
-        if (hit->second.first != g.tz || hit->second.second != g.txw) {
-            g.tz = hit->second.first;
-            g.txw = hit->second.second;
+        const std::vector<int> &b = hit->second;
+        if (b[0] != g.steps || b[1] != g.tz || b[2] != g.txw) {
+            g.steps = b[0];
+            g.tz = b[1];
+            g.txw = b[2];
             return finalize_sweepset(B.W);
         }
         return 0;
@@ -705,16 +730,16 @@ static int autotune_tiles(Batch &B)
     const long long pts = g.pts, launches = g.launches;
     g.timing = false;
     const int probe = std::min(24, B.nt);
-    std::pair<int, int> best{16, 1};
+    std::vector<int> best = cand[0];
     float best_ms = 1e30f;
     const bool log = getenv("DAW_TUNE_LOG") != nullptr;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    static const int shapes[][2] = {{8, 2}, {16, 1}, {16, 2}, {32, 1}, {32, 2}, {64, 1}};
-    for (const auto &sh : shapes) {
-        g.tz = sh[0];
-        g.txw = sh[1];
+    for (const auto &sh : cand) {
+        g.steps = sh[0];
+        g.tz = sh[1];
+        g.txw = sh[2];
         if (int rc = finalize_sweepset(B.W)) return rc;
         if (int rc = run_sweep(kBack, B.W, nullptr, 0, 4)) return rc;       // warm
         HIPCHK(hipEventRecord(e0, g.stream));
@@ -723,9 +748,9 @@ static int autotune_tiles(Batch &B)
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (log) fprintf(stderr, "[daw tune] %d shots %dx%d grid: tile %dx%d  %.1f us/step\n", B.nshots, B.G.n1e,
-                         B.G.n2e, sh[0], 64 * sh[1], 1000.0f * ms / (probe + 1));
-        if (ms < best_ms) { best_ms = ms; best = {sh[0], sh[1]}; }
+        if (log) fprintf(stderr, "[daw tune] %d shots %dx%d grid: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
+                         B.nshots, B.G.n1e, B.G.n2e, sh[0], sh[1], 64 * sh[2], 1000.0f * ms / probe);
+        if (ms < best_ms) { best_ms = ms; best = sh; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -733,8 +758,9 @@ static int autotune_tiles(Batch &B)
     g.pts = pts;
     g.launches = launches;
     g.tz_cache[key] = best;
-    g.tz = best.first;
-    g.txw = best.second;
+    g.steps = best[0];
+    g.tz = best[1];
+    g.txw = best[2];
     return finalize_sweepset(B.W);
 }
 
@@ -1365,13 +1391,12 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
     if (int rc = d_d.alloc((size_t)nr * nt * 4)) return rc;
     HIPCHK(hipMemcpy(d_d.p, d, (size_t)nr * nt * 4, hipMemcpyHostToDevice));
     if (int rc = d_g.alloc((size_t)nt * 4)) return rc;
-    if (int rc = d_band.alloc((size_t)2 * std::max(sband.nslot, 1) * G.ldx * 4)) return rc;
+    if (int rc = d_band.alloc((size_t)4 * std::max(sband.nslot, 1) * G.ldx * 4)) return rc;
     ShotDev &S = B.W.host[0];
     set_inject(S, B.inj_rec_c, d_d.as<float>(), nt, 0);
     S.amp_rev = 0;
     S.band_row = sband.d_row.as<int>();
-    S.pband[0] = d_band.as<float>();
-    S.pband[1] = d_band.as<float>() + (size_t)std::max(sband.nslot, 1) * G.ldx;
+    for (int k = 0; k < 4; ++k) S.pband[k] = d_band.as<float>() + (size_t)k * std::max(sband.nslot, 1) * G.ldx;
     std::vector<RecItem> items;
     add_records(items, src, 0, 0, 0, d_scoef, d_g.as<float>(), nt);
     if (int rc = upload(d_items, items)) return rc;
@@ -1411,10 +1436,30 @@ int daw_set_tile(int rows, int cols)
     if (rows == 0) { g.tz_auto = true; return 0; }
     if (cols % 64 != 0 || !tile_shape_ok(rows, cols / 64))
         return fail(DAW_ERR_ARG, "tile %dx%d has no compiled kernel (8x128, 16x64, 16x128, 32x64, 32x128, 64x64)", rows, cols);
+    if (!g.steps_auto && g.steps == 2 && !tile2_shape_ok(rows, cols / 64))
+        return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel (16x64, 32x64, 16x128)", rows, cols);
     g.tz_auto = false;
     g.tz = rows;
     g.txw = cols / 64;
     return 0;
+}
+
+int daw_set_steps(int steps)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (steps == 0) { g.steps_auto = true; return 0; }
+    if (steps != 1 && steps != 2) return fail(DAW_ERR_ARG, "steps per launch %d: 0 (autotuned), 1 or 2", steps);
+    if (steps == 2 && !g.tz_auto && !tile2_shape_ok(g.tz, g.txw))
+        return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel", g.tz, 64 * g.txw);
+    g.steps_auto = false;
+    g.steps = steps;
+    return 0;
+}
+
+int daw_get_steps(void)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    return g.steps;
 }
 
 int daw_get_tile(int *rows, int *cols)

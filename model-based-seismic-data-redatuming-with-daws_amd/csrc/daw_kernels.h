@@ -10,6 +10,9 @@ namespace daw {
 void launch_step(int mode, int snap, const StepParams &P, int tz, int txw, hipStream_t st);
 // Tile shape (rows tz, 64*txw columns) has compiled kernels.
 bool tile_shape_ok(int tz, int txw);
+// Two steps (it, it+1) per launch: tile of tz rows and 64*nc columns.
+void launch_step2(int mode, int snap, const StepParams &P, int tz, int nc, hipStream_t st);
+bool tile2_shape_ok(int tz, int nc);
 // Exact adjoint of one step (reverse time); hess selects the transpose of
 // the re-injection pass (snapshots out) over that of the forward sweep.
 void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st);

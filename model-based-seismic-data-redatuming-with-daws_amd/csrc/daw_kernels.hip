@@ -76,12 +76,14 @@ __device__ __forceinline__ void record_block(const StepParams &P, int blk, float
     if (tid < kRecPerBlock) {
         RecSlot q;
         q.live = 0;
-        const int item = first + tid;
-        if (item < P.nrec) {
+        const int q_all = first + tid;            // (step, item) of the launch's record work
+        const int sub = q_all / max(P.nrec, 1), item = q_all - sub * P.nrec;
+        const int rit = P.rec_it + sub;
+        if (sub < P.rec_nsub) {
             const RecItem R = P.rec[item];
             const ShotDev *__restrict__ S = P.shots + R.shot;
             const int nt = S->nt;
-            if (P.rec_it < nt) {
+            if (rit < nt) {
                 q.live = 1;
                 if (R.clip == 0) {
                     q.lo1 = -R.ix1; q.hi1 = S->n1e - 1 - R.ix1;
@@ -90,10 +92,10 @@ __device__ __forceinline__ void record_block(const StepParams &P, int blk, float
                     q.lo1 = S->npml - R.ix1; q.hi1 = S->npml + S->n1 - 1 - R.ix1;
                     q.lo2 = S->npml - R.ix2; q.hi2 = S->npml + S->n2 - 1 - R.ix2;
                 }
-                q.band = glb(S->pband[P.rec_slot]);
+                q.band = glb(S->pband[rit & 3]);
                 q.brow = glb(S->band_row);
                 q.coef = glb(R.coef);
-                q.out = glb(R.out) + (R.reverse ? (nt - 1 - P.rec_it) : P.rec_it);
+                q.out = glb(R.out) + (R.reverse ? (nt - 1 - rit) : rit);
                 q.ix1 = R.ix1;
                 q.ix2 = R.ix2;
                 q.ldx = S->ldx;
@@ -156,7 +158,7 @@ __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur
 {
     View v;
     v.n1 = S->n1; v.n2 = S->n2; v.npml = S->npml; v.n1e = S->n1e; v.n2e = S->n2e; v.ldx = S->ldx;
-    v.nt = S->nt; v.ofs = S->ofs; v.par = it & 1; v.sd = S;
+    v.nt = S->nt; v.ofs = S->ofs; v.par = it & 3; v.sd = S;
     v.dt = S->dt; v.dtdx = S->dtdx;
     v.kappa = glb(S->kappa); v.d1a = glb(S->d1a); v.d1a1 = glb(S->d1a1); v.d1b = glb(S->d1b); v.bw1 = glb(S->bw1);
     v.d2a = glb(S->d2a); v.d2a1 = glb(S->d2a1); v.d2b = glb(S->d2b); v.bw2 = glb(S->bw2);
@@ -804,6 +806,353 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two time steps per launch (it and it+1) for every active shot of a batch:
+// the wave state is read from HBM once and written once per two steps.
+//
+// A workgroup owns a TZ x 64*NC core tile.  It stages the state of the core
+// plus a 6-cell ring in LDS (vx, vz, and p once formed; px, pz and kappa stay
+// in registers of the thread that owns the point) and advances the shrinking
+// regions each half-step needs (region rows / columns relative to the ring
+// origin z0-6, x0-6, inclusive):
+//   P_A [2, TZ+11] x [2, TX+10]    V_A [3, TZ+9] x [3, TX+8]
+//   P_B [5, TZ+7]  x [5, TX+7]     V_B [6, TZ+5] x [6, TX+5] (the core)
+// V_A keeps one row more than P_B reads so that the free-surface mirror
+// vz(npml) = vz(npml+1) always has its source row.  Every point is updated
+// with the same operations in the same order as in k_step, so the results are
+// bit-identical; only the core's final state, record band, snapshots and
+// wavefield are written.
+//
+// Thread layout: "main" items are (region row, 64-column core segment) pairs,
+// one per wavefront with lane = column (aligned 256-byte rows, and the 16-bit
+// snapshot segment is exactly one wavefront); the 12 ring columns of every
+// row are "halo" items spread over all threads.
+// ---------------------------------------------------------------------------
+template <int MODE, int RW>
+__device__ __forceinline__ float p_half(const View &S, int t, int gz, int gx, float &pxv, float &pzv, float kap,
+                                        const float *vr, const float *vc, float d1a, float d1a1, float d2a, float d2a1,
+                                        int injq, int lo, int hiz, int hix, float spv)
+{
+    const float a0 = 1.125f;
+    const float a1 = -(1.0f / 24.0f);
+    if (in_box(gz, gx, lo, hiz, hix)) {
+        const float der2 = a0 * (vr[0] - vr[-1]) + a1 * (vr[1] - vr[-2]);
+        const float der1 = a0 * (vc[0] - vc[-RW]) + a1 * (vc[RW] - vc[-2 * RW]);
+        const float tk = S.dtdx * kap;                      // kappaw = dtdx*kappa*damp1 (:105-106)
+        const float kw2 = tk * d2a1;
+        const float kw1 = tk * d1a1;
+        if (MODE == kBack) {
+            pxv = d2a * pxv - kw2 * der2;
+            pzv = d1a * pzv - kw1 * der1;
+        } else {
+            pxv = d2a * pxv + kw2 * der2;
+            pzv = d1a * pzv + kw1 * der1;
+        }
+    }
+    if (MODE != kReinject && injq >= 0) {
+        const GLB int *ptr = glb(S.sd->inj_ptr) + injq * (S.n2e + 1);
+        const int kb = ptr[gx], ke = ptr[gx + 1];
+        for (int k = kb; k < ke; ++k) {
+            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + t]) * kap;
+            if (MODE == kBack) pxv = pxv - term;   // :1042
+            else pxv = pxv + term;                 // :175
+        }
+    }
+    if (MODE == kReinject) pxv = pxv + ((-S.dt) * spv) * kap;   // ss*dt*sp*kappa, ss = -1 (:1123); spv = 0 off range
+    float p = pxv + pzv;
+    if (S.ofs == 1 && gz <= S.npml) p = 0.0f;           // p(1:npml+1,:) = 0
+    return p;
+}
+
+template <int MODE, int RW>
+__device__ __forceinline__ void v_half(const View &S, int t, int gz, int gx, float &vxv, float &vzv, const float *pp,
+                                       float d1b, float bw1, float d2b, float bw2, int lo, int hiz, int hix, bool interior,
+                                       float svx, float svz)
+{
+    const float a0 = 1.125f;
+    const float a1 = -(1.0f / 24.0f);
+    if (in_box(gz, gx, lo, hiz, hix)) {
+        const float der2 = a0 * (pp[1] - pp[0]) + a1 * (pp[2] - pp[-1]);
+        const float der1 = a0 * (pp[RW] - pp[0]) + a1 * (pp[2 * RW] - pp[-RW]);
+        if (MODE == kBack) {
+            vxv = d2b * vxv - bw2 * der2;
+            vzv = d1b * vzv - bw1 * der1;
+        } else {
+            vxv = d2b * vxv + bw2 * der2;
+            vzv = d1b * vzv + bw1 * der1;
+        }
+    }
+    if (MODE == kFwd && S.sd->vinj && S.sd->inj_row != nullptr) {
+        const int q = glb(S.sd->inj_row)[gz];
+        if (q >= 0) {
+            const GLB int *ptr = glb(S.sd->inj_ptr) + q * (S.n2e + 1);
+            for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
+                vzv = vzv + glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + t];   // :252
+        }
+    }
+    if (MODE == kReinject && interior) {
+        vxv = vxv + (-S.dt) * svx;                         // * bue (= 1), :1177
+        vzv = vzv + (-S.dt) * svz;                         // :1184
+    }
+}
+
+// p of an owned point: record band and (single-shot calls) the wavefield
+template <int MODE>
+__device__ __forceinline__ void p_out(const View &S, int t, int gz, int gx, float p)
+{
+    if (MODE == kBack) return;
+    const int npml = S.npml;
+    if (S.sd->band_row != nullptr) {
+        const int slot = glb(S.sd->band_row)[gz];
+        if (slot >= 0) glb(S.sd->pband[t & 3])[slot * S.ldx + gx] = p;
+    }
+    if (S.sd->wf != nullptr && gz >= npml && gz < npml + S.n1 && gx >= npml && gx < npml + S.n2)
+        plane_at(glb(S.sd->wf), S, t)[(gx - npml) * S.n1 + (gz - npml)] = p;
+}
+
+template <int MODE, int SNAP, int TZ, int NC>
+__global__ __launch_bounds__(kBlock) void k_step2(StepParams P)
+{
+    constexpr int TXT = 64 * NC;
+    constexpr int RH = TZ + 13, RW = TXT + 12;
+    constexpr int NTASK = RH * NC, TPW = (NTASK + 3) / 4;
+    constexpr int NHALO = 12 * RH, HPT = (NHALO + kBlock - 1) / kBlock;
+    constexpr int LDS_N = 3 * RH * RW;
+    static_assert(LDS_N >= kRecPerBlock * kRecStride, "record blocks reuse the tile's LDS");
+    __shared__ __align__(16) float s_lds[LDS_N];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if ((int)blockIdx.x < P.nrec_blocks) {
+        record_block(P, blockIdx.x, s_lds);
+        return;
+    }
+    float *const s_vx = s_lds;
+    float *const s_vz = s_lds + RH * RW;
+    float *const s_p = s_vz + RH * RW;
+
+    const int2 tl = P.tiles[blockIdx.x - P.nrec_blocks];
+    const int itA = P.it, itB = P.it + 1;
+    const View S = make_view(P.shots + tl.x, P.cur, itA);
+    if (itA >= S.nt) return;
+    const bool two = P.nsub == 2 && itB < S.nt;
+    const int n1e = S.n1e, n2e = S.n2e, ldx = S.ldx, npml = S.npml, n1 = S.n1, n2 = S.n2;
+    const int ntx = (n2e + TXT - 1) / TXT;
+    const int z0 = (tl.y / ntx) * TZ, x0 = (tl.y % ntx) * TXT;
+    const int zr = z0 - 6, xr = x0 - 6;                    // region origin
+    // stencil bounds: 3..n-2 forward (submodeling2.f90:148-149), 3..n-3 for
+    // both passes of the Hessian-type sweeps (:1014-1015, :1108-1109)
+    const int lo = 2;
+    const int hiz = (MODE == kFwd) ? n1e - 3 : n1e - 4;
+    const int hix = (MODE == kFwd) ? n2e - 3 : n2e - 4;
+    const int icA = S.nt - 1 - itA, icB = S.nt - 2 - itA;   // time-reversed snapshot index
+    const bool inj = MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj;
+    const int xlo1 = S.sd->xlo1, xhi1 = S.sd->xhi1, xlo2 = S.sd->xlo2, xhi2 = S.sd->xhi2;
+
+    // per-lane column profiles of the core segments
+    float c_d2a[NC], c_d2a1[NC], c_d2b[NC], c_bw2[NC];
+#pragma unroll
+    for (int sg = 0; sg < NC; ++sg) {
+        const unsigned gx = (unsigned)min(x0 + 64 * sg + lane, n2e - 1);
+        c_d2a[sg] = ld(S.d2a, gx);
+        c_d2a1[sg] = ld(S.d2a1, gx);
+        c_d2b[sg] = ld(S.d2b, gx);
+        c_bw2[sg] = ld(S.bw2, gx);
+    }
+
+    // ---- 0. the state of the region: vx, vz to LDS, px, pz, kappa to registers
+    float m_px[TPW], m_pz[TPW], m_ka[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+        m_px[i] = 0.0f; m_pz[i] = 0.0f; m_ka[i] = 0.0f;
+        const int q = wv + 4 * i;
+        if (q < NTASK) {
+            const int r = q / NC, sg = q - NC * r;
+            const int gz = zr + r, gx = x0 + 64 * sg + lane, c = 6 + 64 * sg + lane;
+            const bool ok = gz >= 0 && gz < n1e && gx < n2e;
+            const unsigned g = (unsigned)(clampi(gz, 0, n1e - 1) * ldx + min(gx, n2e - 1));
+            const float a = ld(S.vxo, g), b = ld(S.vzo, g);
+            s_vx[r * RW + c] = ok ? a : 0.0f;
+            s_vz[r * RW + c] = ok ? b : 0.0f;
+            if (r >= 2 && r <= TZ + 11) {
+                const float u = ld(S.pxo, g), v = ld(S.pzo, g), k = ld(S.kappa, g);
+                m_px[i] = ok ? u : 0.0f;
+                m_pz[i] = ok ? v : 0.0f;
+                m_ka[i] = ok ? k : 0.0f;
+            }
+        }
+    }
+    float h_px[HPT], h_pz[HPT], h_ka[HPT];
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+        h_px[j] = 0.0f; h_pz[j] = 0.0f; h_ka[j] = 0.0f;
+        const int k = tid + kBlock * j;
+        if (k < NHALO) {
+            const int r = k / 12, kk = k - 12 * r, c = kk < 6 ? kk : TXT + kk;
+            const int gz = zr + r, gx = xr + c;
+            const bool ok = gz >= 0 && gz < n1e && gx >= 0 && gx < n2e;
+            const unsigned g = (unsigned)(clampi(gz, 0, n1e - 1) * ldx + clampi(gx, 0, n2e - 1));
+            const float a = ld(S.vxo, g), b = ld(S.vzo, g), u = ld(S.pxo, g), v = ld(S.pzo, g), w = ld(S.kappa, g);
+            s_vx[r * RW + c] = ok ? a : 0.0f;
+            s_vz[r * RW + c] = ok ? b : 0.0f;
+            h_px[j] = ok ? u : 0.0f;
+            h_pz[j] = ok ? v : 0.0f;
+            h_ka[j] = ok ? w : 0.0f;
+        }
+    }
+    __syncthreads();
+
+    // ---- 1. and 3. pressure half-steps of steps A (region P_A) and B (P_B)
+#pragma unroll
+    for (int stp = 0; stp < 2; ++stp) {
+        if (stp == 1 && !two) break;
+        const int t = stp == 0 ? itA : itB, ic = stp == 0 ? icA : icB;
+        const int rlo = stp == 0 ? 2 : 5, rhi = stp == 0 ? TZ + 11 : TZ + 7;
+        const int clo = rlo, chi = stp == 0 ? TXT + 10 : TXT + 7;
+        const bool last = stp == 1 || !two;
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+            const int q = wv + 4 * i;
+            if (q >= NTASK) break;
+            const int r = q / NC, sg = q - NC * r;
+            if (r < rlo || r > rhi) continue;
+            const int gz = zr + r, gx = x0 + 64 * sg + lane, c = 6 + 64 * sg + lane;
+            const bool ok = gz >= 0 && gz < n1e && gx < n2e;
+            const int wseg = min((x0 >> 6) + sg, S.nseg - 1);
+            float spv = 0.0f;
+            if (MODE == kReinject) {
+                const bool in = ok && gz >= xlo1 && gz <= xhi1 && gx >= xlo2 && gx <= xhi2;
+                const float v = snap_load_w<SNAP>(S, S.sp, S.sp16, 0, ic, clampi(gz, xlo1, xhi1), clampi(gx, xlo2, xhi2),
+                                                  wseg);
+                spv = in ? v : 0.0f;
+            }
+            const int gzc = clampi(gz, 0, n1e - 1);
+            const float d1a = ld(S.d1a, (unsigned)gzc), d1a1 = ld(S.d1a1, (unsigned)gzc);
+            const int injq = inj ? glb(S.sd->inj_row)[gzc] : -1;
+            float p = 0.0f;
+            if (ok)
+                p = p_half<MODE, RW>(S, t, gz, gx, m_px[i], m_pz[i], m_ka[i], &s_vx[r * RW + c], &s_vz[r * RW + c], d1a,
+                                     d1a1, c_d2a[sg], c_d2a1[sg], injq, lo, hiz, hix, spv);
+            s_p[r * RW + c] = p;
+            if (r >= 6 && r <= TZ + 5) {                          // the core
+                if (ok) {
+                    p_out<MODE>(S, t, gz, gx, p);
+                    if (last) {
+                        const unsigned g = (unsigned)(gz * ldx + gx);
+                        st(S.pxn, g, m_px[i]);
+                        st(S.pzn, g, m_pz[i]);
+                    }
+                }
+                if (MODE == kBack && gz >= npml && gz < npml + n1)   // snapshot sp(:,:,t) (:1060-1064)
+                    snap_store_wave<SNAP>(S, S.sp, S.sp16, 0, t, gz, gx, p, ok && gx >= npml && gx < npml + n2, true);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < HPT; ++j) {
+            const int k = tid + kBlock * j;
+            if (k < NHALO) {
+                const int r = k / 12, kk = k - 12 * r, c = kk < 6 ? kk : TXT + kk;
+                const int gz = zr + r, gx = xr + c;
+                if (r >= rlo && r <= rhi && c >= clo && c <= chi) {
+                    const bool ok = gz >= 0 && gz < n1e && gx >= 0 && gx < n2e;
+                    float p = 0.0f;
+                    if (ok) {
+                        float spv = 0.0f;
+                        if (MODE == kReinject && gz >= xlo1 && gz <= xhi1 && gx >= xlo2 && gx <= xhi2)
+                            spv = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, gz, gx);
+                        const int injq = inj ? glb(S.sd->inj_row)[gz] : -1;
+                        p = p_half<MODE, RW>(S, t, gz, gx, h_px[j], h_pz[j], h_ka[j], &s_vx[r * RW + c],
+                                             &s_vz[r * RW + c], ld(S.d1a, (unsigned)gz), ld(S.d1a1, (unsigned)gz),
+                                             ld(S.d2a, (unsigned)gx), ld(S.d2a1, (unsigned)gx), injq, lo, hiz, hix, spv);
+                    }
+                    s_p[r * RW + c] = p;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 2. and 4. velocity half-steps of steps A (region V_A) and B (the core)
+        const int vlo = stp == 0 ? 3 : 6, vhi = stp == 0 ? TZ + 9 : TZ + 5;
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+            const int q = wv + 4 * i;
+            if (q >= NTASK) break;
+            const int r = q / NC, sg = q - NC * r;
+            if (r < vlo || r > vhi) continue;
+            const int gz = zr + r, gx = x0 + 64 * sg + lane, c = 6 + 64 * sg + lane;
+            const bool ok = gz >= 0 && gz < n1e && gx < n2e;
+            const bool rowi = gz >= npml && gz < npml + n1;
+            const bool interior = rowi && gx >= npml && gx < npml + n2;
+            float svx = 0.0f, svz = 0.0f;
+            if (MODE == kReinject) {
+                const int wseg = min((x0 >> 6) + sg, S.nseg - 1);
+                const int cz = clampi(gz, npml, npml + n1 - 1), cx = clampi(gx, npml, npml + n2 - 1);
+                const float a = snap_load_w<SNAP>(S, S.svx, S.svx16, 1, ic, cz, cx, wseg);
+                const float b = snap_load_w<SNAP>(S, S.svz, S.svz16, 2, ic, cz, cx, wseg);
+                svx = interior && ok ? a : 0.0f;
+                svz = interior && ok ? b : 0.0f;
+            }
+            const int gzc = clampi(gz, 0, n1e - 1);
+            const float d1b = ld(S.d1b, (unsigned)gzc), bw1 = ld(S.bw1, (unsigned)gzc);
+            float vxv = 0.0f, vzv = 0.0f;
+            if (ok) {
+                vxv = s_vx[r * RW + c];
+                vzv = s_vz[r * RW + c];
+                v_half<MODE, RW>(S, t, gz, gx, vxv, vzv, &s_p[r * RW + c], d1b, bw1, c_d2b[sg], c_bw2[sg], lo, hiz, hix,
+                                 interior, svx, svz);
+                s_vx[r * RW + c] = vxv;
+                s_vz[r * RW + c] = vzv;
+                if (last && r >= 6 && r <= TZ + 5) {             // final state of the core
+                    const unsigned g = (unsigned)(gz * ldx + gx);
+                    st(S.vxn, g, vxv);
+                    if (S.ofs == 1 && gz == npml - 1) {
+                        // vz(npml,:) is overwritten by the mirror below; its owner writes nothing
+                    } else {
+                        st(S.vzn, g, vzv);
+                        if (S.ofs == 1 && gz == npml) st(S.vzn, g - ldx, vzv);   // vz(npml,:) = vz(npml+1,:)
+                    }
+                }
+            }
+            if (MODE == kBack && r >= 6 && r <= TZ + 5 && rowi) {   // svx, svz(:,:,t) (:1082-1087)
+                snap_store_wave<SNAP>(S, S.svx, S.svx16, 1, t, gz, gx, vxv, interior && ok, true);
+                snap_store_wave<SNAP>(S, S.svz, S.svz16, 2, t, gz, gx, vzv, interior && ok, true);
+            }
+        }
+        if (stp == 0) {
+#pragma unroll
+            for (int j = 0; j < HPT; ++j) {
+                const int k = tid + kBlock * j;
+                if (k < NHALO) {
+                    const int r = k / 12, kk = k - 12 * r, c = kk < 6 ? kk : TXT + kk;
+                    const int gz = zr + r, gx = xr + c;
+                    if (r >= 3 && r <= TZ + 9 && c >= 3 && c <= TXT + 8 && gz >= 0 && gz < n1e && gx >= 0 &&
+                        gx < n2e) {
+                        const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
+                        float svx = 0.0f, svz = 0.0f;
+                        if (MODE == kReinject && interior) {
+                            svx = snap_load<SNAP>(S, S.svx, S.svx16, 1, ic, gz, gx);
+                            svz = snap_load<SNAP>(S, S.svz, S.svz16, 2, ic, gz, gx);
+                        }
+                        float vxv = s_vx[r * RW + c], vzv = s_vz[r * RW + c];
+                        v_half<MODE, RW>(S, t, gz, gx, vxv, vzv, &s_p[r * RW + c], ld(S.d1b, (unsigned)gz),
+                                         ld(S.bw1, (unsigned)gz), ld(S.d2b, (unsigned)gx), ld(S.bw2, (unsigned)gx), lo,
+                                         hiz, hix, interior, svx, svz);
+                        s_vx[r * RW + c] = vxv;
+                        s_vz[r * RW + c] = vzv;
+                    }
+                }
+            }
+            if (!two) return;
+            // free surface: vz(npml,:) = vz(npml+1,:) inside the region (1-based rows)
+            const int rm = npml - 1 - zr;
+            const bool mirror = S.ofs == 1 && rm >= 3 && rm + 1 <= TZ + 9;
+            __syncthreads();
+            if (mirror) {
+                for (int c = 3 + tid; c <= TXT + 8; c += kBlock) s_vz[rm * RW + c] = s_vz[(rm + 1) * RW + c];
+                __syncthreads();
+            }
+        }
+    }
+}
+
 // Tile shapes the autotuner may pick: (rows, wavefronts across x).
 #define DAW_TILE_SHAPES(X) X(8, 2) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(64, 1)
 
@@ -848,6 +1197,47 @@ static void launch_snap(int snap, const StepParams &P, int tz, int txw, hipStrea
     if (snap == kSnapBF16) launch_mode<MODE, kSnapBF16>(P, tz, txw, st);
     else if (snap == kSnapF16) launch_mode<MODE, kSnapF16>(P, tz, txw, st);
     else launch_mode<MODE, kSnapF32>(P, tz, txw, st);
+}
+
+// Shapes of the two-step kernel: (core rows, 64-column segments).
+#define DAW_TILE2_SHAPES(X) X(16, 1) X(32, 1) X(16, 2)
+
+template <int MODE, int SNAP>
+static void launch_mode2(const StepParams &P, int tz, int nc, hipStream_t st)
+{
+    const int grid = P.ntiles + P.nrec_blocks;
+#define DAW_STEP2(Z, W)                                                                                        \
+    if (tz == Z && nc == W) {                                                                                  \
+        hipLaunchKernelGGL((k_step2<MODE, SNAP, Z, W>), dim3(grid), dim3(kBlock), 0, st, P);                   \
+        return;                                                                                                \
+    }
+    DAW_TILE2_SHAPES(DAW_STEP2)
+#undef DAW_STEP2
+}
+
+bool tile2_shape_ok(int tz, int nc)
+{
+#define DAW_OK_SHAPE2(Z, W) if (tz == Z && nc == W) return true;
+    DAW_TILE2_SHAPES(DAW_OK_SHAPE2)
+#undef DAW_OK_SHAPE2
+    return false;
+}
+
+template <int MODE>
+static void launch_snap2(int snap, const StepParams &P, int tz, int nc, hipStream_t st)
+{
+    if (snap == kSnapBF16) launch_mode2<MODE, kSnapBF16>(P, tz, nc, st);
+    else if (snap == kSnapF16) launch_mode2<MODE, kSnapF16>(P, tz, nc, st);
+    else launch_mode2<MODE, kSnapF32>(P, tz, nc, st);
+}
+
+void launch_step2(int mode, int snap, const StepParams &P, int tz, int nc, hipStream_t st)
+{
+    const int grid = P.ntiles + P.nrec_blocks;
+    if (grid == 0) return;
+    if (mode == kFwd) launch_mode2<kFwd, kSnapF32>(P, tz, nc, st);
+    else if (mode == kBack) launch_snap2<kBack>(snap, P, tz, nc, st);
+    else launch_snap2<kReinject>(snap, P, tz, nc, st);
 }
 
 void launch_step(int mode, int snap, const StepParams &P, int tz, int txw, hipStream_t st)
