@@ -691,8 +691,9 @@ struct Batch {
     SweepSet W;
 };
 
-// Tile autotuner: time a short back-propagation probe (zero residuals) at
-// every compiled tile shape and keep the fastest for this batch shape.
+// Tile autotuner: time a short back-propagation probe (zero residuals, best of
+// three) at every compiled tile shape and steps-per-launch choice and keep the
+// fastest for this batch shape.
 // DAW_TUNE_LOG=1 prints the probe times to stderr.
 static int autotune_tiles(Batch &B)
 {
@@ -742,12 +743,16 @@ static int autotune_tiles(Batch &B)
         g.txw = sh[2];
         if (int rc = finalize_sweepset(B.W)) return rc;
         if (int rc = run_sweep(kBack, B.W, nullptr, 0, 4)) return rc;       // warm
-        HIPCHK(hipEventRecord(e0, g.stream));
-        if (int rc = run_sweep(kBack, B.W, nullptr, 0, probe)) return rc;
-        HIPCHK(hipEventRecord(e1, g.stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {                                  // best of three probes
+            HIPCHK(hipEventRecord(e0, g.stream));
+            if (int rc = run_sweep(kBack, B.W, nullptr, 0, probe)) return rc;
+            HIPCHK(hipEventRecord(e1, g.stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float m = 0;
+            HIPCHK(hipEventElapsedTime(&m, e0, e1));
+            ms = std::min(ms, m);
+        }
         if (log) fprintf(stderr, "[daw tune] %d shots %dx%d grid: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
                          B.nshots, B.G.n1e, B.G.n2e, sh[0], sh[1], 64 * sh[2], 1000.0f * ms / probe);
         if (ms < best_ms) { best_ms = ms; best = sh; }
