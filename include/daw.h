@@ -160,6 +160,44 @@ typedef struct daw_redatum_args {
 
 int daw_redatum_batch(const daw_redatum_args *args);
 
+/* Uneven batch (no reference counterpart; the reference runs every shot on
+ * the whole model with one receiver table and one nt): each shot s is
+ * redatumed on its own subdomain, columns x0[s] .. x0[s]+n2s[s]-1 of c0, with
+ * its own receivers and nt[s] -- exactly what the reference computes when
+ * run on that cropped model, acquisition and nt -- and all shots of a
+ * sub-batch advance together in one launch per time step.  Tables and trace
+ * blocks of consecutive shots follow one another. */
+typedef struct daw_uneven_args {
+    const float *c0;          /* host, full model c0(n1,n2) */
+    int n1, n2, npml;
+    float h, dt;
+    int ofs, type_src;
+    const float *s_dev;       /* device, wavelet of max(nts) samples, already / h**2 */
+    int nshots;
+    const int *x0;            /* host [nshots]: first column (0-based) of each subdomain */
+    const int *n2s;           /* host [nshots]: subdomain widths */
+    const int *nts;           /* host [nshots]: time steps per shot */
+    const int *src_tab;       /* host (nshots,5), subdomain coordinates (subacqui on the subdomain) */
+    const int *nrecs;         /* host [nshots] */
+    const int *rec_tabs;      /* host: shot s's (nrecs[s],5) table, one after another */
+    const int *ndats;         /* host [nshots] or NULL: datum receivers per shot */
+    const int *dat_tabs;      /* host or NULL: shot s's (ndats[s],5) table, one after another */
+    const float *dobs_dev;    /* device: shot s's [nrecs[s]][nts[s]] traces, one after another */
+    float mu;
+    int maxiter;
+    float rtol;
+    float *datum_dev;         /* device out or NULL: [ndats[s]][nts[s]] per shot, one after another */
+    float *datae_dev;         /* device out or NULL: as dobs_dev */
+    float *deltade_dev;       /* device out or NULL: as dobs_dev */
+    double *hist_host;        /* host out or NULL: [shot][maxiter] */
+    int max_batch;            /* shots per sub-batch; <= 0: by the HBM budget and the ~128 MB state cap */
+    double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
+    int filter;
+    float flow, fhig, llow, lhig;
+} daw_uneven_args;
+
+int daw_redatum_uneven(const daw_uneven_args *args);
+
 /* Mode-0 forward modelling of sub_redatum_DAW.f90:55-65 (submodeling per
  * shot, submodeling2.f90:11-288) for a block of shots advanced together:
  * dobs_dev [shot][nrec][nt]; with datum receivers also the datum gather of

@@ -662,33 +662,47 @@ static int check_hicks()
     return 0;
 }
 
-// ------------------------------------------------------------------ Hessian-type two-pass operator
-// Pass 1 (kBack) back-propagates the per-shot amplitudes amp[s*amp_shot + r*nt + t]
-// and stores snapshots; pass 2 (kReinject) re-injects them time reversed
-// and records the acquisition receivers (and optionally datum receivers).
-struct HessOut {
-    float *sis = nullptr;      // per shot acquisition trace block, stride sis_shot
-    size_t sis_shot = 0;
-    int reverse = 0;
-    float *datum = nullptr;    // per shot datum block or null
-    size_t datum_shot = 0;
-    float *wf = nullptr;       // single shot only: full wavefield (Fortran layout)
-    int variant = 0;           // 0: esource_data ranges, 1: esource ranges (2..n-1)
-    int record = 1;            // type_rec == 0
+// ------------------------------------------------------------------ batches
+// Geometry of shots: grid, model, nt and acquisition tables.  An even batch
+// (the reference's case: one model, one receiver table, one nt) has one
+// geometry shared by all shots; an uneven batch has one per shot (its cropped
+// model, its receivers, its nt).
+struct Geo {
+    Grid G;
+    const DevModel *M = nullptr;
+    std::unique_ptr<DevModel> own;  // a cropped model owned by this geometry
+    int nt = 0;
+    PointSet rec, dat;
+    Inject inj_rec;
+    Inject inj_rec_c;               // receivers with coefficient c (adjoint data spreading, R^T)
+    Band band;
+    DevBuf d_rec_coef, d_dat_coef;
 };
 
 struct Batch {
-    Grid G;
-    const DevModel *M = nullptr;
-    int nshots = 0, nt = 0;
-    PointSet src, rec, dat;
-    Inject inj_rec;
-    Inject inj_rec_c;              // receivers with coefficient c (adjoint data spreading, R^T)
-    std::vector<Inject> inj_src;   // one per shot
-    Band band;
-    DevBuf d_rec_coef, d_dat_coef;
-    ShotStore store;
+    int nshots = 0, nt = 0;         // nt: the longest shot
+    std::vector<std::unique_ptr<Geo>> geo;
+    std::vector<int> gi;            // geometry of each shot
+    std::vector<Inject> inj_src;    // one per shot
+    std::vector<ShotStore> store;   // one per shot
+    // offsets of each shot's acquisition traces (nrec x nt) and datum traces
+    // in the batch's trace blocks; the last entry is the total
+    std::vector<size_t> roff, doff;
     SweepSet W;
+    const Geo &geo_of(int s) const { return *geo[gi[s]]; }
+};
+
+// Hessian-type two-pass operator.  Pass 1 (kBack) back-propagates the
+// per-shot amplitudes amp[roff[s] + r*nt + t] and stores snapshots; pass 2
+// (kReinject) re-injects them time reversed and records the acquisition
+// receivers (and optionally the datum receivers).
+struct HessOut {
+    float *sis = nullptr;      // acquisition traces, shot s at sis + roff[s]
+    int reverse = 0;
+    float *datum = nullptr;    // datum traces, shot s at datum + doff[s], or null
+    float *wf = nullptr;       // single shot only: full wavefield (Fortran layout)
+    int variant = 0;           // 0: esource_data ranges, 1: esource ranges (2..n-1)
+    int record = 1;            // type_rec == 0
 };
 
 // Tile autotuner: time a short back-propagation probe (zero residuals, best of
@@ -708,8 +722,8 @@ static int autotune_tiles(Batch &B)
         cand.push_back({sh[0], sh[1], sh[2]});
     }
     if (cand.empty()) return 0;
-    const std::vector<int> key{B.G.n1e, B.G.n2e, B.nshots, g.steps_auto ? 0 : g.steps, g.tz_auto ? 0 : g.tz,
-                               g.tz_auto ? 0 : g.txw};
+    const std::vector<int> key{(int)(B.W.pts_per_step >> 10), B.geo_of(0).G.n2e, B.nshots, g.steps_auto ? 0 : g.steps,
+                               g.tz_auto ? 0 : g.tz, g.tz_auto ? 0 : g.txw};
     auto hit = g.tz_cache.find(key);
     if (hit != g.tz_cache.end()) {
         const std::vector<int> &b = hit->second;
@@ -721,11 +735,14 @@ static int autotune_tiles(Batch &B)
         }
         return 0;
     }
-    if (!B.store.has_snap()) return 0;
+    if (B.store.empty() || !B.store[0].has_snap()) return 0;
     DevBuf zeros;
     if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
     launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
-    for (auto &S : B.W.host) { set_inject(S, B.inj_rec, zeros.as<float>(), 0, 0); S.amp_rev = 0; }
+    for (int s = 0; s < B.nshots; ++s) {
+        set_inject(B.W.host[s], B.geo_of(s).inj_rec, zeros.as<float>(), 0, 0);
+        B.W.host[s].amp_rev = 0;
+    }
     if (int rc = refresh_shots(B.W)) return rc;
     const bool timing = g.timing;
     const long long pts = g.pts, launches = g.launches;
@@ -754,7 +771,7 @@ static int autotune_tiles(Batch &B)
             ms = std::min(ms, m);
         }
         if (log) fprintf(stderr, "[daw tune] %d shots %dx%d grid: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
-                         B.nshots, B.G.n1e, B.G.n2e, sh[0], sh[1], 64 * sh[2], 1000.0f * ms / probe);
+                         B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, sh[0], sh[1], 64 * sh[2], 1000.0f * ms / probe);
         if (ms < best_ms) { best_ms = ms; best = sh; }
     }
     (void)hipEventDestroy(e0);
@@ -769,22 +786,31 @@ static int autotune_tiles(Batch &B)
     return finalize_sweepset(B.W);
 }
 
-static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o)
+// Pressure extended-source range of the re-injection: 1..n (esource_data,
+// submodeling2.f90:1121-1122) or 2..n-1 (esource, :777-778), extended coords.
+static void set_range(ShotDev &S, const Grid &G, int variant)
+{
+    const int e = variant == 1 ? 1 : 0;
+    S.xlo1 = G.npml + e; S.xhi1 = G.npml + G.n1 - 1 - e;
+    S.xlo2 = G.npml + e; S.xhi2 = G.npml + G.n2 - 1 - e;
+}
+
+static int hessian(Batch &B, const float *amp, const HessOut &o)
 {
     SweepSet &W = B.W;
     const bool exact = g.hess_exact;
     for (int s = 0; s < B.nshots; ++s) {
         ShotDev &S = W.host[s];
+        const Geo &Q = B.geo_of(s);
         S.wf = nullptr;
         if (exact) {
             // pass 1 = transpose of pass 2 applied to the time-reversed input
             // (the reference's res is time reversed, subresiduals.f90:14)
-            set_inject(S, B.inj_rec_c, amp + (size_t)s * amp_shot, B.nt, 0);
+            set_inject(S, Q.inj_rec_c, amp + B.roff[s], Q.nt, 0);
             S.amp_rev = 1;
-            if (o.variant == 1) { S.xlo1 = B.G.npml + 1; S.xhi1 = B.G.npml + B.G.n1 - 2; S.xlo2 = B.G.npml + 1; S.xhi2 = B.G.npml + B.G.n2 - 2; }
-            else { S.xlo1 = B.G.npml; S.xhi1 = B.G.npml + B.G.n1 - 1; S.xlo2 = B.G.npml; S.xhi2 = B.G.npml + B.G.n2 - 1; }
+            set_range(S, Q.G, o.variant);
         } else {
-            set_inject(S, B.inj_rec, amp + (size_t)s * amp_shot, B.nt, 0);
+            set_inject(S, Q.inj_rec, amp + B.roff[s], Q.nt, 0);
             S.amp_rev = 0;
         }
     }
@@ -797,13 +823,13 @@ static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o
     std::vector<RecItem> items;
     for (int s = 0; s < B.nshots; ++s) {
         ShotDev &S = W.host[s];
+        const Geo &Q = B.geo_of(s);
         S.inj_row = nullptr;
         S.amp_rev = 0;
         S.wf = o.wf;
-        if (o.variant == 1) { S.xlo1 = B.G.npml + 1; S.xhi1 = B.G.npml + B.G.n1 - 2; S.xlo2 = B.G.npml + 1; S.xhi2 = B.G.npml + B.G.n2 - 2; }
-        else { S.xlo1 = B.G.npml; S.xhi1 = B.G.npml + B.G.n1 - 1; S.xlo2 = B.G.npml; S.xhi2 = B.G.npml + B.G.n2 - 1; }
-        if (o.record && o.sis) add_records(items, B.rec, s, 0, o.reverse, B.d_rec_coef, o.sis + (size_t)s * o.sis_shot, B.nt);
-        if (o.datum) add_records(items, B.dat, s, 1, 0, B.d_dat_coef, o.datum + (size_t)s * o.datum_shot, B.nt);
+        set_range(S, Q.G, o.variant);
+        if (o.record && o.sis) add_records(items, Q.rec, s, 0, o.reverse, Q.d_rec_coef, o.sis + B.roff[s], Q.nt);
+        if (o.datum) add_records(items, Q.dat, s, 1, 0, Q.d_dat_coef, o.datum + B.doff[s], Q.nt);
     }
     if (!o.record) for (auto &S : W.host) S.wf = nullptr;
     if (int rc = set_items(W, items)) return rc;
@@ -814,18 +840,18 @@ static int hessian(Batch &B, const float *amp, size_t amp_shot, const HessOut &o
 }
 
 // Forward sweep: sources of every shot, record acquisition (+ datum).
-static int forward(Batch &B, const float *s_wavelet, int type_src, int record, float *sis, size_t sis_shot,
-                   float *datum, size_t datum_shot, float *wf)
+static int forward(Batch &B, const float *s_wavelet, int type_src, int record, float *sis, float *datum, float *wf)
 {
     SweepSet &W = B.W;
     std::vector<RecItem> items;
     for (int s = 0; s < B.nshots; ++s) {
         ShotDev &S = W.host[s];
+        const Geo &Q = B.geo_of(s);
         set_inject(S, B.inj_src[s], s_wavelet, 0, type_src == 1 ? 1 : 0);
         if (type_src != 0 && type_src != 1) S.inj_row = nullptr;
         S.wf = record ? wf : nullptr;
-        if (record && sis) add_records(items, B.rec, s, 0, 0, B.d_rec_coef, sis + (size_t)s * sis_shot, B.nt);
-        if (datum) add_records(items, B.dat, s, 1, 0, B.d_dat_coef, datum + (size_t)s * datum_shot, B.nt);
+        if (record && sis) add_records(items, Q.rec, s, 0, 0, Q.d_rec_coef, sis + B.roff[s], Q.nt);
+        if (datum) add_records(items, Q.dat, s, 1, 0, Q.d_dat_coef, datum + B.doff[s], Q.nt);
     }
     if (int rc = set_items(W, items)) return rc;
     if (int rc = refresh_shots(W)) return rc;
@@ -834,50 +860,82 @@ static int forward(Batch &B, const float *s_wavelet, int type_src, int record, f
     return rc;
 }
 
-// Builds the per-batch tables and storage.  src_tab: (nsrc_total,5) with
-// rows src_first..src_first+nshots-1 used; rec_tab (nrec,5); dat_tab (ndat,5) or null.
-static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int nshots, const int *src_rows5,
-                       int src_stride, const int *rec_tab, int nrec, const int *dat_tab, int ndat, bool want_snap)
+// The acquisition tables of a geometry: rec_tab (nrec,5), dat_tab (ndat,5) or null.
+static int make_geo(Geo &Q, const int *rec_tab, int nrec, const int *dat_tab, int ndat)
 {
-    B.G = G;
-    B.M = M;
-    B.nshots = nshots;
-    B.nt = nt;
-    if (int rc = make_points(B.rec, rec_tab, nrec, G, false)) return rc;
-    if (dat_tab && ndat > 0) {
-        if (int rc = make_points(B.dat, dat_tab, ndat, G, true)) return rc;
-    }
-    if (int rc = make_inject(B.inj_rec, B.rec, G)) return rc;
+    const Grid &G = Q.G;
+    if (int rc = make_points(Q.rec, rec_tab, nrec, G, false)) return rc;
+    if (dat_tab && ndat > 0)
+        if (int rc = make_points(Q.dat, dat_tab, ndat, G, true)) return rc;
+    if (int rc = make_inject(Q.inj_rec, Q.rec, G)) return rc;
     {
         Grid G1 = G;
         G1.dt = 1.0f;              // c*dt with dt = 1 is c
-        if (int rc = make_inject(B.inj_rec_c, B.rec, G1)) return rc;
+        if (int rc = make_inject(Q.inj_rec_c, Q.rec, G1)) return rc;
     }
-    B.inj_src.clear();
-    B.inj_src.resize(nshots);
-    if (src_rows5) {
-        for (int s = 0; s < nshots; ++s) {
-            int row[5];
-            for (int k = 0; k < 5; ++k) row[k] = src_rows5[s + k * src_stride];
-            PointSet P;
-            if (int rc = make_points(P, row, 1, G, false)) return rc;
-            if (int rc = make_inject(B.inj_src[s], P, G)) return rc;
-        }
-    }
-    std::vector<const PointSet *> sets{&B.rec};
-    if (B.dat.n) sets.push_back(&B.dat);
-    if (int rc = make_band(B.band, sets, G)) return rc;
-    if (int rc = upload_record_coef(B.d_rec_coef, B.rec.coef)) return rc;
-    if (B.dat.n)
-        if (int rc = upload_record_coef(B.d_dat_coef, B.dat.coef)) return rc;
+    std::vector<const PointSet *> sets{&Q.rec};
+    if (Q.dat.n) sets.push_back(&Q.dat);
+    if (int rc = make_band(Q.band, sets, G)) return rc;
+    if (int rc = upload_record_coef(Q.d_rec_coef, Q.rec.coef)) return rc;
+    if (Q.dat.n)
+        if (int rc = upload_record_coef(Q.d_dat_coef, Q.dat.coef)) return rc;
+    return 0;
+}
+
+// Source injection of shot s from its src_tab row (5 entries at stride).
+static int make_source(Batch &B, int s, const int *row5, int stride)
+{
+    int row[5];
+    for (int k = 0; k < 5; ++k) row[k] = row5[(size_t)k * stride];
+    PointSet P;
+    if (int rc = make_points(P, row, 1, B.geo_of(s).G, false)) return rc;
+    return make_inject(B.inj_src[s], P, B.geo_of(s).G);
+}
+
+// Storage, shot descriptors, trace offsets and tiles once geo/gi/inj_src are set.
+static int finish_batch(Batch &B, bool want_snap)
+{
     // the exact Hessian's transpose pass writes FP32 snapshots
     const int fmt = want_snap ? (g.hess_exact ? (int)kSnapF32 : g.snap_fmt) : -1;
-    if (int rc = alloc_store(B.store, nshots, G, B.band.nslot, nt, fmt)) return rc;
-    B.W.nshots = nshots;
+    B.store.clear();
+    B.store.resize(B.nshots);
+    B.W.nshots = B.nshots;
     B.W.snap = fmt < 0 ? (int)kSnapF32 : fmt;
-    B.W.host.resize(nshots);
-    for (int s = 0; s < nshots; ++s) fill_shot(B.W.host[s], G, *M, B.store, s, nt, B.band.nslot, &B.band);
+    B.W.host.resize(B.nshots);
+    B.roff.assign(B.nshots + 1, 0);
+    B.doff.assign(B.nshots + 1, 0);
+    B.nt = 0;
+    for (int s = 0; s < B.nshots; ++s) {
+        const Geo &Q = B.geo_of(s);
+        if (int rc = alloc_store(B.store[s], 1, Q.G, Q.band.nslot, Q.nt, fmt)) return rc;
+        fill_shot(B.W.host[s], Q.G, *Q.M, B.store[s], 0, Q.nt, Q.band.nslot, &Q.band);
+        B.roff[s + 1] = B.roff[s] + (size_t)Q.rec.n * Q.nt;
+        B.doff[s + 1] = B.doff[s] + (size_t)Q.dat.n * Q.nt;
+        B.nt = std::max(B.nt, Q.nt);
+    }
     return finalize_sweepset(B.W);
+}
+
+// An even batch: nshots shots on one grid, model, receiver table and nt.
+// src_rows5: (src_stride,5) with rows 0..nshots-1 used, or null.
+static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int nshots, const int *src_rows5,
+                       int src_stride, const int *rec_tab, int nrec, const int *dat_tab, int ndat, bool want_snap)
+{
+    B.nshots = nshots;
+    B.geo.clear();
+    B.geo.push_back(std::make_unique<Geo>());
+    Geo &Q = *B.geo[0];
+    Q.G = G;
+    Q.M = M;
+    Q.nt = nt;
+    if (int rc = make_geo(Q, rec_tab, nrec, dat_tab, ndat)) return rc;
+    B.gi.assign(nshots, 0);
+    B.inj_src.clear();
+    B.inj_src.resize(nshots);
+    if (src_rows5)
+        for (int s = 0; s < nshots; ++s)
+            if (int rc = make_source(B, s, src_rows5 + s, src_stride)) return rc;
+    return finish_batch(B, want_snap);
 }
 
 // ------------------------------------------------------------------ band-pass on device traces
@@ -927,13 +985,13 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
                         double *scal_host)
 {
     const int ns = B.nshots;
-    const size_t n = (size_t)B.nt * B.rec.n;
+    const size_t blk = B.roff[ns];       // all shots' vectors, shot s at roff[s]
     const int nb = minres_vec_blocks(), nsc = minres_scal_stride();
     MinresWork w;
-    if (int rc = w.vec.alloc(8 * n * ns * sizeof(float))) return rc;
-    std::vector<size_t> off(ns);
-    std::vector<int> len(ns, (int)n);
-    for (int s = 0; s < ns; ++s) off[s] = (size_t)s * n;
+    if (int rc = w.vec.alloc(8 * blk * sizeof(float))) return rc;
+    std::vector<size_t> off(B.roff.begin(), B.roff.begin() + ns);
+    std::vector<int> len(ns);
+    for (int s = 0; s < ns; ++s) len[s] = (int)(B.roff[s + 1] - B.roff[s]);
     if (int rc = upload(w.off, off)) return rc;
     if (int rc = upload(w.len, len)) return rc;
     if (int rc = w.part0.alloc((size_t)ns * nb * sizeof(double))) return rc;
@@ -941,7 +999,6 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
     if (int rc = w.scal.alloc((size_t)ns * nsc * sizeof(double))) return rc;
     if (int rc = w.hist.alloc((size_t)ns * std::max(itnlim, 1) * sizeof(double))) return rc;
     float *base = w.vec.as<float>();
-    const size_t blk = n * ns;
     float *v = base, *Y[3] = {base + blk, base + 2 * blk, base + 3 * blk};
     float *Wv[3] = {base + 4 * blk, base + 5 * blk, base + 6 * blk};
     float *hb = base + 7 * blk;
@@ -951,8 +1008,8 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
 
     // fac from ||b|| / ||H b|| (sublinearminres.f90:64-72)
     HessOut o;
-    o.sis = hb; o.sis_shot = n; o.reverse = 0; o.variant = 0;
-    if (int rc = hessian(B, b, n, o)) return rc;
+    o.sis = hb; o.reverse = 0; o.variant = 0;
+    if (int rc = hessian(B, b, o)) return rc;
     launch_dot(b, b, doff, dlen, w.part0.as<double>(), ns, g.stream);
     launch_dot(hb, hb, doff, dlen, w.part1.as<double>(), ns, g.stream);
     launch_minres_init(w.part0.as<double>(), w.part1.as<double>(), w.scal.as<double>(), mu, ns, g.stream);
@@ -974,8 +1031,8 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
         // free buffer Y[iy] receives the new operator output
         launch_minres_v(Y[ir2], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
         HessOut h;
-        h.sis = Y[iy]; h.sis_shot = n; h.reverse = 1; h.variant = 0;     // blurring: time-reversed output (:394-396)
-        if (int rc = hessian(B, v, n, h)) return rc;
+        h.sis = Y[iy]; h.reverse = 1; h.variant = 0;     // blurring: time-reversed output (:394-396)
+        if (int rc = hessian(B, v, h)) return rc;
         launch_minres_l1(Y[iy], v, Y[ir1], doff, dlen, w.scal.as<double>(), w.part0.as<double>(), ns, g.stream);
         launch_minres_l2(Y[iy], Y[ir2], doff, dlen, w.scal.as<double>(), w.part1.as<double>(), ns, g.stream);
         // r1 <- r2, r2 <- y, y <- free
@@ -993,6 +1050,74 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
     if (hist_host)
         HIPCHK(hipMemcpy(hist_host, w.hist.p, (size_t)ns * std::max(itnlim, 1) * sizeof(double), hipMemcpyDeviceToHost));
     if (scal_host) HIPCHK(hipMemcpy(scal_host, w.scal.p, (size_t)ns * nsc * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Modes 1 and 2 of sub_redatum_DAW.f90 for the shots of one batch.  Trace
+// blocks follow the batch's layout: shot s at roff[s] (acquisition) and
+// doff[s] (datum).
+struct RedatumIO {
+    const float *s_dev = nullptr;
+    int type_src = 0;
+    const float *dobs = nullptr;
+    float mu = 0.0f, rtol = 0.0f;
+    int maxiter = 0;
+    float *datum = nullptr, *datae = nullptr, *deltade = nullptr;
+    double *hist = nullptr;
+    int filter = 0;
+    float dt = 0.0f, flow = 0.0f, fhig = 0.0f, llow = 0.0f, lhig = 0.0f;
+};
+
+static int redatum_run(Batch &B, const RedatumIO &io)
+{
+    const int ns = B.nshots;
+    const size_t nv = B.roff[ns], nd = B.doff[ns];
+    DevBuf d_datr, d_datum_r, d_res, d_x, d_esc, d_datum_e;
+    if (int rc = d_datr.alloc(nv * 4)) return rc;
+    if (int rc = d_res.alloc(nv * 4)) return rc;
+    if (int rc = d_x.alloc(nv * 4)) return rc;
+    if (int rc = d_esc.alloc(nv * 4)) return rc;
+    if (nd > 0) {
+        if (int rc = d_datum_r.alloc(nd * 4)) return rc;
+        if (int rc = d_datum_e.alloc(nd * 4)) return rc;
+    }
+    // 1. background forward in c0: d^r and the datum gather of u^r (sub_redatum_DAW.f90:82-83)
+    if (int rc = forward(B, io.s_dev, io.type_src, 1, d_datr.as<float>(), nd ? d_datum_r.as<float>() : nullptr, nullptr))
+        return rc;
+    // 2. reduced residual, time reversed (:103)
+    if (B.geo.size() == 1) {
+        launch_residuals(io.dobs, d_datr.as<float>(), d_res.as<float>(), B.geo[0]->nt,
+                         (size_t)ns * B.geo[0]->rec.n, g.stream);
+    } else {
+        for (int s = 0; s < ns; ++s)
+            launch_residuals(io.dobs + B.roff[s], d_datr.as<float>() + B.roff[s], d_res.as<float>() + B.roff[s],
+                             B.geo_of(s).nt, (size_t)B.geo_of(s).rec.n, g.stream);
+    }
+    // 3. MINRES on (H + fac I) de = dr (:115)
+    if (int rc = minres_batch(B, d_res.as<float>(), d_x.as<float>(), io.mu, io.maxiter, (double)io.rtol, io.hist, nullptr))
+        return rc;
+    // 4. source extension wavefield: dataesc and its datum gather (:131-132)
+    HessOut o;
+    o.sis = d_esc.as<float>(); o.reverse = 0; o.variant = 1;
+    o.datum = nd ? d_datum_e.as<float>() : nullptr;
+    if (int rc = hessian(B, d_x.as<float>(), o)) return rc;
+    // 5. data-assimilated data and the redatumed gather of u^r + u^e (:143-145, :238)
+    if (io.datae) launch_add(d_datr.as<float>(), d_esc.as<float>(), io.datae, nv, g.stream);
+    if (io.deltade) HIPCHK(hipMemcpyAsync(io.deltade, d_x.p, nv * 4, hipMemcpyDeviceToDevice, g.stream));
+    if (nd > 0) {
+        launch_add(d_datum_r.as<float>(), d_datum_e.as<float>(), io.datum, nd, g.stream);
+        // mode 2 band-pass of the redatumed gathers (sub_redatum_DAW.f90:240)
+        if (io.filter) {
+            if (B.geo.size() == 1) {
+                if (int rc = filter_device(io.datum, io.datum, ns * B.geo[0]->dat.n, B.geo[0]->nt, io.dt, io.flow,
+                                           io.fhig, io.llow, io.lhig)) return rc;
+            } else {
+                for (int s = 0; s < ns; ++s)
+                    if (int rc = filter_device(io.datum + B.doff[s], io.datum + B.doff[s], B.geo_of(s).dat.n,
+                                               B.geo_of(s).nt, io.dt, io.flow, io.fhig, io.llow, io.lhig)) return rc;
+            }
+        }
+    }
     return 0;
 }
 
@@ -1096,8 +1221,8 @@ int daw_submodeling(const float *vp, int n1, int n2, int npml, float h, int ofs,
     if (wavefield && rec) {
         if (int rc = d_wf.alloc(G.plane_i() * nt * 4)) return rc;
     }
-    if (int rc = forward(B, d_s.as<float>(), type_src, rec ? 1 : 0, d_sis.as<float>(), (size_t)nt, nullptr, 0,
-                         d_wf.as<float>())) return rc;
+    if (int rc = forward(B, d_s.as<float>(), type_src, rec ? 1 : 0, d_sis.as<float>(), nullptr, d_wf.as<float>()))
+        return rc;
     HIPCHK(hipStreamSynchronize(g.stream));
     if (nr > 0) HIPCHK(hipMemcpy(sis, d_sis.p, (size_t)nr * nt * 4, hipMemcpyDeviceToHost));
     if (wavefield) {
@@ -1133,9 +1258,9 @@ static int esource_common(const float *vp, int n1, int n2, int npml, float h, in
     if (wavefieldp && rec)
         if (int rc = d_wf.alloc(G.plane_i() * nt * 4)) return rc;
     HessOut o;
-    o.sis = d_sis.as<float>(); o.sis_shot = n; o.reverse = 0; o.variant = variant; o.record = rec ? 1 : 0;
+    o.sis = d_sis.as<float>(); o.reverse = 0; o.variant = variant; o.record = rec ? 1 : 0;
     o.wf = d_wf.as<float>();
-    if (int rc = hessian(B, d_res.as<float>(), n, o)) return rc;
+    if (int rc = hessian(B, d_res.as<float>(), o)) return rc;
     HIPCHK(hipStreamSynchronize(g.stream));
     if (nr > 0) HIPCHK(hipMemcpy(sis, d_sis.p, (size_t)nr * nt * 4, hipMemcpyDeviceToHost));
     if (wavefieldp) {
@@ -1274,51 +1399,25 @@ int daw_redatum_batch(const daw_redatum_args *a)
     for (int s0 = 0; s0 < a->nshots; s0 += maxb) {
         const int ns = std::min(maxb, a->nshots - s0);
         Batch B;
-        int src_rows[5 * 1];
-        (void)src_rows;
         std::vector<int> srows((size_t)ns * 5);
         for (int s = 0; s < ns; ++s)
             for (int k = 0; k < 5; ++k) srows[s + (size_t)k * ns] = a->src_tab[(s0 + s) + (size_t)k * a->nsrc_total];
         if (int rc = build_batch(B, G, M, nt, ns, srows.data(), ns, a->rec_tab, nrec, a->dat_tab, ndat, true)) return rc;
         if (int rc = autotune_tiles(B)) return rc;
-        DevBuf d_datr, d_datum_r, d_res, d_x, d_esc, d_datum_e;
-        if (int rc = d_datr.alloc(ns * nvec * 4)) return rc;
-        if (int rc = d_res.alloc(ns * nvec * 4)) return rc;
-        if (int rc = d_x.alloc(ns * nvec * 4)) return rc;
-        if (int rc = d_esc.alloc(ns * nvec * 4)) return rc;
-        if (ndat > 0) {
-            if (int rc = d_datum_r.alloc(ns * ndv * 4)) return rc;
-            if (int rc = d_datum_e.alloc(ns * ndv * 4)) return rc;
-        }
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, g.stream));
-        // 1. background forward in c0: d^r and the datum gather of u^r (sub_redatum_DAW.f90:82-83)
-        if (int rc = forward(B, a->s_dev, a->type_src, 1, d_datr.as<float>(), nvec,
-                             ndat ? d_datum_r.as<float>() : nullptr, ndv, nullptr)) return rc;
-        // 2. reduced residual, time reversed (:103)
-        launch_residuals(a->dobs_dev + (size_t)s0 * nvec, d_datr.as<float>(), d_res.as<float>(), nt, (size_t)ns * nrec, g.stream);
-        // 3. MINRES on (H + fac I) de = dr (:115)
-        if (int rc = minres_batch(B, d_res.as<float>(), d_x.as<float>(), a->mu, a->maxiter, (double)a->rtol,
-                                  a->hist_host ? a->hist_host + (size_t)s0 * std::max(a->maxiter, 1) : nullptr, nullptr))
-            return rc;
-        // 4. source extension wavefield: dataesc and its datum gather (:131-132)
-        HessOut o;
-        o.sis = d_esc.as<float>(); o.sis_shot = nvec; o.reverse = 0; o.variant = 1;
-        o.datum = ndat ? d_datum_e.as<float>() : nullptr; o.datum_shot = ndv;
-        if (int rc = hessian(B, d_x.as<float>(), nvec, o)) return rc;
-        // 5. data-assimilated data and the redatumed gather of u^r + u^e (:143-145, :238)
-        if (a->datae_dev) launch_add(d_datr.as<float>(), d_esc.as<float>(), a->datae_dev + (size_t)s0 * nvec, ns * nvec, g.stream);
-        if (a->deltade_dev)
-            HIPCHK(hipMemcpyAsync(a->deltade_dev + (size_t)s0 * nvec, d_x.p, ns * nvec * 4, hipMemcpyDeviceToDevice, g.stream));
-        if (ndat > 0) {
-            float *dd = a->datum_dev + (size_t)s0 * ndv;
-            launch_add(d_datum_r.as<float>(), d_datum_e.as<float>(), dd, ns * ndv, g.stream);
-            // mode 2 band-pass of the redatumed gathers (sub_redatum_DAW.f90:240)
-            if (a->filter)
-                if (int rc = filter_device(dd, dd, ns * ndat, nt, a->dt, a->flow, a->fhig, a->llow, a->lhig)) return rc;
-        }
+        RedatumIO io;
+        io.s_dev = a->s_dev; io.type_src = a->type_src;
+        io.dobs = a->dobs_dev + (size_t)s0 * nvec;
+        io.mu = a->mu; io.maxiter = a->maxiter; io.rtol = a->rtol;
+        io.datum = ndat > 0 ? a->datum_dev + (size_t)s0 * ndv : nullptr;
+        io.datae = a->datae_dev ? a->datae_dev + (size_t)s0 * nvec : nullptr;
+        io.deltade = a->deltade_dev ? a->deltade_dev + (size_t)s0 * nvec : nullptr;
+        io.hist = a->hist_host ? a->hist_host + (size_t)s0 * std::max(a->maxiter, 1) : nullptr;
+        io.filter = a->filter; io.dt = a->dt; io.flow = a->flow; io.fhig = a->fhig; io.llow = a->llow; io.lhig = a->lhig;
+        if (int rc = redatum_run(B, io)) return rc;
         HIPCHK(hipEventRecord(e1, g.stream));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
@@ -1326,6 +1425,101 @@ int daw_redatum_batch(const daw_redatum_args *a)
         g.sweep_ms += ms;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int daw_redatum_uneven(const daw_uneven_args *a)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (!a) return fail(DAW_ERR_ARG, "null args");
+    if (int rc = check_hicks()) return rc;
+    if (!a->c0 || !a->s_dev || !a->x0 || !a->n2s || !a->nts || !a->src_tab || !a->nrecs || !a->rec_tabs ||
+        !a->dobs_dev || a->nshots < 1)
+        return fail(DAW_ERR_ARG, "redatum_uneven: missing input");
+    const int ns_all = a->nshots;
+    const bool dat = a->ndats != nullptr && a->dat_tabs != nullptr && a->datum_dev != nullptr;
+    // per shot: subdomain, tables and trace offsets in the caller's blocks
+    std::vector<size_t> rtab(ns_all + 1, 0), dtab(ns_all + 1, 0), roff(ns_all + 1, 0), doff(ns_all + 1, 0);
+    for (int s = 0; s < ns_all; ++s) {
+        const int nd = dat ? a->ndats[s] : 0;
+        if (a->x0[s] < 0 || a->n2s[s] < 1 || a->x0[s] + a->n2s[s] > a->n2 || a->nrecs[s] < 1 || nd < 0)
+            return fail(DAW_ERR_ARG, "redatum_uneven: shot %d: subdomain [%d, %d) of %d columns, %d receivers", s,
+                        a->x0[s], a->x0[s] + a->n2s[s], a->n2, a->nrecs[s]);
+        if (int rc = check_grid(a->n1, a->n2s[s], a->npml, a->nts[s], a->h, a->dt)) return rc;
+        rtab[s + 1] = rtab[s] + (size_t)5 * a->nrecs[s];
+        dtab[s + 1] = dtab[s] + (size_t)5 * nd;
+        roff[s + 1] = roff[s] + (size_t)a->nrecs[s] * a->nts[s];
+        doff[s + 1] = doff[s] + (size_t)nd * a->nts[s];
+    }
+    if (int rc = ensure_stream()) return rc;
+    size_t freeb = 0, total = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &total));
+    const double budget = a->hbm_budget_bytes > 0 ? a->hbm_budget_bytes : 0.85 * (double)freeb;
+    const int fmt = g.hess_exact ? (int)kSnapF32 : g.snap_fmt;
+    g.max_batch_used = 0;
+    for (int s0 = 0; s0 < ns_all;) {
+        // admit shots in order while they fit the HBM budget and, unless
+        // max_batch is given, keep a launch's wave state within ~128 MB (at least 8 shots)
+        int ns = 0;
+        double bytes = 0.0, state = 0.0;
+        while (s0 + ns < ns_all) {
+            const int s = s0 + ns;
+            const Grid Gs = make_grid(a->n1, a->n2s[s], a->npml, a->h, a->dt, a->ofs);
+            const double b = (double)snap_bytes(Gs, a->nts[s], fmt) + 8.0 * Gs.plane_e() * 4 +
+                             14.0 * (roff[s + 1] - roff[s]) * 4 + 3.0 * (doff[s + 1] - doff[s]) * 4 + (1 << 20);
+            const double st = 8.0 * Gs.plane_e() * 4;
+            if (ns > 0 && (bytes + b > budget || (a->max_batch > 0 ? ns >= a->max_batch : (ns >= 8 && state + st > 128e6))))
+                break;
+            bytes += b;
+            state += st;
+            ++ns;
+        }
+        g.max_batch_used = std::max(g.max_batch_used, ns);
+        Batch B;
+        B.nshots = ns;
+        B.gi.resize(ns);
+        B.inj_src.resize(ns);
+        for (int k = 0; k < ns; ++k) {
+            const int s = s0 + k;
+            auto Q = std::make_unique<Geo>();
+            Q->G = make_grid(a->n1, a->n2s[s], a->npml, a->h, a->dt, a->ofs);
+            Q->own = std::make_unique<DevModel>();
+            // the subdomain's columns x0 .. x0+n2s-1 of c0(n1,n2) are contiguous
+            if (int rc = make_model(*Q->own, Q->G, a->c0 + (size_t)a->x0[s] * a->n1)) return rc;
+            Q->M = Q->own.get();
+            Q->nt = a->nts[s];
+            if (int rc = make_geo(*Q, a->rec_tabs + rtab[s], a->nrecs[s], dat ? a->dat_tabs + dtab[s] : nullptr,
+                                  dat ? a->ndats[s] : 0)) return rc;
+            B.geo.push_back(std::move(Q));
+            B.gi[k] = k;
+            if (int rc = make_source(B, k, a->src_tab + s, ns_all)) return rc;
+        }
+        if (int rc = finish_batch(B, true)) return rc;
+        if (int rc = autotune_tiles(B)) return rc;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g.stream));
+        RedatumIO io;
+        io.s_dev = a->s_dev; io.type_src = a->type_src;
+        io.dobs = a->dobs_dev + roff[s0];
+        io.mu = a->mu; io.maxiter = a->maxiter; io.rtol = a->rtol;
+        io.datum = dat ? a->datum_dev + doff[s0] : nullptr;
+        io.datae = a->datae_dev ? a->datae_dev + roff[s0] : nullptr;
+        io.deltade = a->deltade_dev ? a->deltade_dev + roff[s0] : nullptr;
+        io.hist = a->hist_host ? a->hist_host + (size_t)s0 * std::max(a->maxiter, 1) : nullptr;
+        io.filter = a->filter; io.dt = a->dt; io.flow = a->flow; io.fhig = a->fhig; io.llow = a->llow; io.lhig = a->lhig;
+        if (int rc = redatum_run(B, io)) return rc;
+        HIPCHK(hipEventRecord(e1, g.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        g.sweep_ms += ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        s0 += ns;
     }
     HIPCHK(hipStreamSynchronize(g.stream));
     return 0;
@@ -1356,8 +1550,8 @@ int daw_modeling_batch(const daw_modeling_args *a)
         if (int rc = build_batch(B, G, M, nt, ns, srows.data(), ns, a->rec_tab, a->nrec, a->dat_tab, a->ndat, false))
             return rc;
         if (int rc = forward(B, a->s_dev, a->type_src, a->type_rec == 0 ? 1 : 0,
-                             a->nrec > 0 ? a->dobs_dev + (size_t)s0 * nvec : nullptr, nvec,
-                             a->ndat > 0 ? a->datum_dev + (size_t)s0 * ndv : nullptr, ndv, nullptr))
+                             a->nrec > 0 ? a->dobs_dev + (size_t)s0 * nvec : nullptr,
+                             a->ndat > 0 ? a->datum_dev + (size_t)s0 * ndv : nullptr, nullptr))
             return rc;
     }
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -1398,7 +1592,7 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
     if (int rc = d_g.alloc((size_t)nt * 4)) return rc;
     if (int rc = d_band.alloc((size_t)4 * std::max(sband.nslot, 1) * G.ldx * 4)) return rc;
     ShotDev &S = B.W.host[0];
-    set_inject(S, B.inj_rec_c, d_d.as<float>(), nt, 0);
+    set_inject(S, B.geo[0]->inj_rec_c, d_d.as<float>(), nt, 0);
     S.amp_rev = 0;
     S.band_row = sband.d_row.as<int>();
     for (int k = 0; k < 4; ++k) S.pband[k] = d_band.as<float>() + (size_t)k * std::max(sband.nslot, 1) * G.ldx;
