@@ -35,7 +35,7 @@ EXPORTS = [
     "daw_sublinearminres", "daw_filmartin2d", "daw_redatum_batch", "daw_set_timing", "daw_reset_stats",
     "daw_get_stats", "daw_modeling_batch", "daw_set_tile", "daw_get_tile", "daw_filmartin2d_dev",
     "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
-    "daw_set_graph", "daw_set_steps", "daw_get_steps",
+    "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
 ]
 
 
@@ -50,6 +50,20 @@ class RedatumArgs(C.Structure):
         ("ofs", C.c_int), ("nt", C.c_int), ("type_src", C.c_int),
         ("s_dev", C.c_void_p), ("nshots", C.c_int), ("nsrc_total", C.c_int), ("src_tab", C.c_void_p),
         ("nrec", C.c_int), ("rec_tab", C.c_void_p), ("ndat", C.c_int), ("dat_tab", C.c_void_p),
+        ("dobs_dev", C.c_void_p), ("mu", C.c_float), ("maxiter", C.c_int), ("rtol", C.c_float),
+        ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
+        ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
+        ("filter", C.c_int), ("flow", C.c_float), ("fhig", C.c_float), ("llow", C.c_float), ("lhig", C.c_float),
+    ]
+
+
+class UnevenArgs(C.Structure):
+    _fields_ = [
+        ("c0", C.c_void_p), ("n1", C.c_int), ("n2", C.c_int), ("npml", C.c_int),
+        ("h", C.c_float), ("dt", C.c_float), ("ofs", C.c_int), ("type_src", C.c_int),
+        ("s_dev", C.c_void_p), ("nshots", C.c_int),
+        ("x0", C.c_void_p), ("n2s", C.c_void_p), ("nts", C.c_void_p), ("src_tab", C.c_void_p),
+        ("nrecs", C.c_void_p), ("rec_tabs", C.c_void_p), ("ndats", C.c_void_p), ("dat_tabs", C.c_void_p),
         ("dobs_dev", C.c_void_p), ("mu", C.c_float), ("maxiter", C.c_int), ("rtol", C.c_float),
         ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
         ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
@@ -103,6 +117,7 @@ def lib():
         L.daw_filmartin2d.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                       C.c_float, _f32p]
         L.daw_redatum_batch.argtypes = [C.POINTER(RedatumArgs)]
+        L.daw_redatum_uneven.argtypes = [C.POINTER(UnevenArgs)]
         L.daw_get_stats.argtypes = [C.POINTER(Stats)]
         L.daw_modeling_batch.argtypes = [C.POINTER(ModelingArgs)]
         L.daw_set_timing.argtypes = [C.c_int]
@@ -372,6 +387,66 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
         a.flow, a.fhig, a.llow, a.lhig = (float(v) for v in band)
     torch.cuda.synchronize()
     _check(lib().daw_redatum_batch(C.byref(a)))
+    return hist
+
+
+def redatum_uneven(c0, npml, h, ofs, dt, s_dev, x0, n2s, nts, src_tab, rec_tabs, dobs_dev, mu, maxiter, rtol=0.0,
+                   dat_tabs=None, datum_dev=None, datae_dev=None, deltade_dev=None, want_hist=True, max_batch=0,
+                   hbm_budget_bytes=0.0, type_src=0, band=None):
+    """Redatuming of shots on their own subdomains (columns x0[s] .. x0[s]+n2s[s]-1
+    of c0), receivers and nt, advanced together.  src_tab (5, nshots) and the
+    per-shot tables rec_tabs[s] (5, nrec_s) / dat_tabs[s] (5, ndat_s) are in
+    subdomain coordinates (subacqui on the subdomain).  dobs_dev and the
+    outputs are flat float32 cuda tensors holding shot s's [nrec_s, nt_s]
+    (datum: [ndat_s, nt_s]) block after shot s-1's.  Returns the MINRES
+    histories (nshots, maxiter) when want_hist."""
+    import torch
+    _ensure_hicks(ofs)
+    c0 = _f32(c0)
+    n2, n1 = c0.shape
+    ns = len(x0)
+    x0 = _i32(x0)
+    n2s = _i32(n2s)
+    nts = _i32(nts)
+    src_tab = _i32(src_tab)
+    rec_tabs = [_i32(t) for t in rec_tabs]
+    nrecs = _i32([t.shape[1] for t in rec_tabs])
+    rec_all = _i32(np.concatenate([t.ravel() for t in rec_tabs]))
+    if dat_tabs is not None:
+        dat_tabs = [_i32(t) for t in dat_tabs]
+        ndats = _i32([t.shape[1] for t in dat_tabs])
+        dat_all = _i32(np.concatenate([t.ravel() for t in dat_tabs]))
+    nv = int(np.sum(nrecs.astype(np.int64) * nts))
+    nd = int(np.sum(ndats.astype(np.int64) * nts)) if dat_tabs is not None else 0
+    for t, n in ((dobs_dev, nv), (datum_dev, nd), (datae_dev, nv), (deltade_dev, nv)):
+        if t is not None and (t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.numel() != n):
+            raise ValueError(f"device tensor must be a contiguous float32 cuda tensor of {n} values")
+    if s_dev.dtype != torch.float32 or not s_dev.is_cuda or s_dev.numel() < int(nts.max()):
+        raise ValueError("s_dev must be a float32 cuda tensor of max(nts) samples")
+    hist = np.zeros((ns, max(int(maxiter), 1)), np.float64) if want_hist else None
+    a = UnevenArgs()
+    a.c0 = c0.ctypes.data
+    a.n1, a.n2, a.npml = n1, n2, int(npml)
+    a.h, a.dt = float(h), float(dt)
+    a.ofs, a.type_src = int(ofs), int(type_src)
+    a.s_dev = s_dev.data_ptr()
+    a.nshots = ns
+    a.x0, a.n2s, a.nts, a.src_tab = x0.ctypes.data, n2s.ctypes.data, nts.ctypes.data, src_tab.ctypes.data
+    a.nrecs, a.rec_tabs = nrecs.ctypes.data, rec_all.ctypes.data
+    if dat_tabs is not None:
+        a.ndats, a.dat_tabs = ndats.ctypes.data, dat_all.ctypes.data
+    a.dobs_dev = dobs_dev.data_ptr()
+    a.mu, a.maxiter, a.rtol = float(mu), int(maxiter), float(rtol)
+    a.datum_dev = datum_dev.data_ptr() if datum_dev is not None else None
+    a.datae_dev = datae_dev.data_ptr() if datae_dev is not None else None
+    a.deltade_dev = deltade_dev.data_ptr() if deltade_dev is not None else None
+    a.hist_host = hist.ctypes.data if hist is not None else None
+    a.max_batch = int(max_batch)
+    a.hbm_budget_bytes = float(hbm_budget_bytes)
+    if band is not None:
+        a.filter = 1
+        a.flow, a.fhig, a.llow, a.lhig = (float(b) for b in band)
+    _check(lib().daw_redatum_uneven(C.byref(a)))
     return hist
 
 
