@@ -79,6 +79,107 @@ def run_redatum(daw, P, shots=None, max_batch=0, hbm_budget=0.0):
     return datum, hist
 
 
+def setup_uneven(daw, syn, cfg, shots=None, maxiter=None):
+    """Configurations 3 and 5: shots on their own subdomains, receiver sets and
+    nt.  A sample of `shots` shots spread over the line can stand in for all."""
+    import torch
+    w = syn.uneven_workload(cfg)
+    if maxiter is not None:
+        w.maxiter = maxiter
+    idx = np.arange(w.nshots) if not shots or shots >= w.nshots else \
+        np.round(np.linspace(0, w.nshots - 1, shots)).astype(int)
+    vp, c0 = syn.uneven_models(w)
+    h, dt = np.float32(w.h), np.float32(w.dt)
+    x0, n2s, nts = w.x0[idx], w.n2s[idx], w.nts[idx]
+    s = syn.ricker(int(nts.max()), float(dt), w.f0, float(h))
+    s_dev = torch.from_numpy(s).cuda()
+    src, recs, dats, dobs, dnew = [], [], [], [], []
+    for k, i in enumerate(idx):
+        rx = np.arange(n2s[k]) * w.rec_dx
+        src.append(daw.subacqui([w.src_z], [w.src_x[i]], h, w.npml, w.ofs)[:, 0])
+        recs.append(daw.subacqui(np.full(rx.size, w.rec_z), rx, h, w.npml, w.ofs))
+        dx = rx[::4]
+        dats.append(daw.subacqui(np.full(dx.size, w.dat_z), dx, h, w.npml, w.ofs))
+        # observed data in the true model of this subdomain, and the data at the datum
+        nt = int(nts[k])
+        d = torch.zeros((1, recs[-1].shape[1], nt), dtype=torch.float32, device="cuda")
+        dd = torch.zeros((1, dats[-1].shape[1], nt), dtype=torch.float32, device="cuda")
+        daw.modeling_batch(np.ascontiguousarray(vp[x0[k]:x0[k] + n2s[k]]), w.npml, h, w.ofs, nt, dt, s_dev[:nt],
+                           src[-1][:, None], recs[-1], d, dat_tab=dats[-1], datum_dev=dd)
+        dobs.append(d.reshape(-1))
+        dnew.append(dd.reshape(-1))
+    dobs = torch.cat(dobs)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2000 + cfg)
+    rms = float(torch.sqrt(torch.mean(dobs.double() ** 2)))
+    dobs += 0.01 * rms * torch.randn(dobs.shape, generator=g, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    return dict(w=w, c0=c0, h=h, dt=dt, x0=x0, n2s=n2s, nts=nts, src=np.stack(src, axis=1), recs=recs, dats=dats,
+                s_dev=s_dev, dobs=dobs, dnew=torch.cat(dnew), nsh=len(idx))
+
+
+def run_uneven(daw, P, max_batch=0, hbm_budget=0.0):
+    import torch
+    w = P["w"]
+    nd = int(sum(d.shape[1] * int(nt) for d, nt in zip(P["dats"], P["nts"])))
+    datum = torch.zeros(nd, dtype=torch.float32, device="cuda")
+    hist = daw.redatum_uneven(P["c0"], w.npml, P["h"], w.ofs, P["dt"], P["s_dev"], P["x0"], P["n2s"], P["nts"],
+                              P["src"], P["recs"], P["dobs"], w.mu, w.maxiter, 0.0, dat_tabs=P["dats"],
+                              datum_dev=datum, max_batch=max_batch, hbm_budget_bytes=hbm_budget)
+    return datum, hist
+
+
+def main_uneven(args, daw, syn):
+    """Timed redatuming of an uneven configuration (3 or 5)."""
+    import torch
+    _log(f"setup: uneven configuration {args.config}")
+    P = setup_uneven(daw, syn, args.config, shots=args.shots, maxiter=args.maxiter)
+    w = P["w"]
+    daw.set_snapshot_format(args.snapshots if args.snapshots != "fp32" else w.snapshots)
+    snaps = args.snapshots if args.snapshots != "fp32" else w.snapshots
+    budget = args.hbm_budget_gb * 1e9
+    daw.reset_stats()
+    daw.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _log(f"timed step: {P['nsh']} shots")
+    datum, hist = run_uneven(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    daw.set_timing(False)
+    st = daw.stats()
+    n1e = w.nz + 2 * w.npml
+    n2e = P["n2s"] + 2 * w.npml
+    snap_b = 12.0 if snaps == "fp32" else 6.0 + 12.0 / 64.0
+    # algorithmic bytes of the Hessian sweeps: per shot and step, state r+w and
+    # kappa over the extended subdomain plus the snapshots of its interior
+    per_step = 36.0 * n1e * n2e + snap_b * w.nz * P["n2s"]
+    hess_bytes = float(np.sum(per_step * P["nts"])) * 2 * w.maxiter
+    hess_ms = st["sweep_ms"][1] + st["sweep_ms"][2]
+    achieved = hess_bytes / (hess_ms * 1e-3) / 1e9 if hess_ms > 0 else None
+    err = float(torch.linalg.norm(datum - P["dnew"]) / torch.linalg.norm(P["dnew"]))
+    out = {
+        "metric": "shots redatumed/hour + stencil GPts/s on 1 MI355X, uneven batch",
+        "value": P["nsh"] / elapsed * 3600.0,
+        "unit": "shots/hour",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": elapsed * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded faulted layered model, background sigma 10, Ricker, 1% seeded noise",
+        "config": {"workload": w.name, "shots_sampled": P["nsh"], "maxiter": w.maxiter, "snapshots": snaps,
+                   "subdomain_columns": [int(P["n2s"].min()), int(P["n2s"].max())],
+                   "nt": [int(P["nts"].min()), int(P["nts"].max())],
+                   "max_shots_per_launch": st["max_batch_used"], "tile": list(daw.get_tile()),
+                   "steps_per_launch": daw.get_steps()},
+        "gpts_per_s": st["point_updates"] / elapsed / 1e9,
+        "redatum_rel_l2_vs_true_datum_data": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0 if achieved else None, "traffic": None,
+                     "kernel": "k_step<1>/<2> (Hessian sweeps, average over the run)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(P, nt_sample):
     """The reference's own CPU path (oracle/_ref, built from its sources) on
     one shot: one submodeling_adjoint_esource_data call (back-propagation +
@@ -208,6 +309,8 @@ def main():
     daw.set_tile(args.tile_rows, args.tile_cols)
     daw.set_snapshot_format(args.snapshots)
     daw.set_graph(args.graph)
+    if args.config in (3, 5):
+        return main_uneven(args, daw, syn)
     _log("setup: modelling observed data")
     P = setup(daw, syn, args.config, shots=args.shots, nt=args.nt, maxiter=args.maxiter)
     w = P["w"]

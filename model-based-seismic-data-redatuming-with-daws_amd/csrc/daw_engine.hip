@@ -1476,6 +1476,10 @@ int daw_redatum_uneven(const daw_uneven_args *a)
             state += st;
             ++ns;
         }
+        {   // balance: the remaining shots in sub-batches of nearly equal count
+            const int rest = ns_all - s0, nb = (rest + ns - 1) / ns;
+            ns = (rest + nb - 1) / nb;
+        }
         g.max_batch_used = std::max(g.max_batch_used, ns);
         Batch B;
         B.nshots = ns;

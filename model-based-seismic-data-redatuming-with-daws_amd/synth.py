@@ -132,6 +132,74 @@ def workload(cfg):
     raise ValueError(f"unknown workload {cfg}")
 
 
+@dataclass
+class UnevenWorkload:
+    """Shots on their own subdomains (aperture), receiver sets and nt."""
+    name: str
+    nz: int
+    nx: int
+    h: float
+    npml: int
+    dt: float
+    f0: float
+    x0: np.ndarray          # first column of each shot's subdomain
+    n2s: np.ndarray         # subdomain widths (columns)
+    nts: np.ndarray         # time steps per shot
+    src_x: np.ndarray       # source x in subdomain coordinates (m)
+    src_z: float
+    rec_z: float
+    rec_dx: float           # receiver spacing (m): one per rec_dx over the subdomain
+    dat_z: float
+    maxiter: int = 30
+    mu: float = 1e-2
+    ofs: int = 1
+    seed: int = 0
+    faults: int = 0
+    body: bool = False
+    snapshots: str = "fp32"
+    band: tuple = field(default=(2.0, 40.0, 1.0, 5.0))
+
+    @property
+    def nshots(self):
+        return int(self.x0.size)
+
+
+def uneven_workload(cfg):
+    """Configurations with per-shot apertures, receiver counts and nt: 3 (351 x 1401,
+    h = 6.25 m, 128 shots, 3-6 km apertures, nt 4000-6000, FP32) and 5 (801 x 3201,
+    h = 3.125 m, 256 shots at 37.5 m, 400-1600 receivers, nt = 12000, 16-bit
+    snapshots)."""
+    rng = np.random.Generator(np.random.PCG64(3000 + cfg))
+    if cfg == 3:
+        nz, nx, h, ns = 351, 1401, 6.25, 128
+        ap = rng.uniform(3000.0, 6000.0, ns)
+        n2s = np.round(ap / h).astype(np.int32)                       # 480-960 columns, one receiver each
+        sx = np.linspace(0.05, 0.95, ns) * (nx - 1) * h
+        nts = rng.integers(4000, 6001, ns).astype(np.int32)
+        name, dt, f0, dx, snaps = "C3 351x1401 h=6.25 128 shots uneven apertures 3-6 km nt 4000-6000", 5e-4, 20.0, h, "fp32"
+        seed, faults, body = 1003, 3, True
+    elif cfg == 5:
+        nz, nx, h, ns = 801, 3201, 3.125, 256
+        nrec = rng.integers(400, 1601, ns)
+        n2s = nrec.astype(np.int32)                                  # one receiver per column
+        sx = 400.0 + 37.5 * np.arange(ns)
+        nts = np.full(ns, 12000, np.int32)
+        name, dt, f0, dx, snaps = "C5 801x3201 h=3.125 256 shots 400-1600 rec nt=12000", 2.5e-4, 30.0, h, "bf16"
+        seed, faults, body = 1005, 3, False
+    else:
+        raise ValueError(f"unknown uneven workload {cfg}")
+    # subdomain centred on the source, clipped to the model
+    x0 = np.clip(np.round(sx / h).astype(np.int64) - n2s // 2, 0, nx - n2s).astype(np.int32)
+    src_local = sx - x0 * h
+    return UnevenWorkload(name, nz, nx, h, 20, dt, f0, x0, n2s, nts, src_local, 10.0, 15.0, dx, 400.0,
+                          seed=seed, faults=faults, body=body, snapshots=snaps)
+
+
+def uneven_models(w):
+    vp = layered_model(w.nz, w.nx, w.seed, water_cells=min(20, w.nz // 4), faults=w.faults, body=w.body, h=w.h)
+    return vp, background(vp)
+
+
 def small_workload(nz=40, nx=90, nt=300, nshots=2, nrec=41, h=10.0, dt=1e-3, f0=15.0, maxiter=4, seed=7):
     """A small instance of the same family for parity tests."""
     sx = np.linspace(0.3, 0.7, nshots) * (nx - 1) * h
