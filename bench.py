@@ -332,8 +332,8 @@ def main():
     elapsed = time.perf_counter() - t0
     daw.set_timing(False)
     st = daw.stats()
+    launch = {m: daw.get_launch(i) for i, m in enumerate(("forward", "back-propagation", "re-injection"))}
     tile = list(daw.get_tile())
-    launch_steps = daw.get_steps()
     nsh = P["nsh"]
     shots_per_hour = nsh * args.steps / elapsed * 3600.0
     gpts = st["point_updates"] / elapsed / 1e9
@@ -341,9 +341,10 @@ def main():
     mb = st["max_batch_used"]
     # dominant kernel: back-propagation or re-injection sweep; one launch
     # advances all shots of a sub-batch by launch_steps time steps
+    dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
+    launch_steps = launch[("forward", "back-propagation", "re-injection")[dom]][0]
     kname = "k_step2" if launch_steps == 2 else "k_step"
     modes = {1: f"{kname}<1> (back-propagation)", 2: f"{kname}<2> (re-injection)"}
-    dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
     avg_ms = st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1) * launch_steps   # per launch
     # algorithmic bytes of one launch per shot: the wave state read and written
     # once (px, pz, vx, vz: 32 B) and kappa read (4 B) per extended point, plus
@@ -402,8 +403,8 @@ def main():
         "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m", "shots": nsh, "nt": w.nt,
                    "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
                    "shots_per_launch": shots_per_launch, "snapshots": args.snapshots, "graph": args.graph,
-                   "steps_per_launch": launch_steps,
-                   "tile": tile},
+                   "launch": {m: {"steps": v[0], "tile": [v[1], v[2]]} for m, v in launch.items()},
+                   "tile": [launch["back-propagation"][1], launch["back-propagation"][2]]},
         "gpts_per_s": gpts,
         "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
                               for i, k in enumerate(("forward", "back-propagation", "re-injection"))},

@@ -240,8 +240,8 @@ int daw_set_timing(int on);
 int daw_set_graph(int on);
 /* Step-kernel tile (rows x columns): 8x128, 16x64, 16x128, 32x64, 32x128 or
  * 64x64; rows = 0 lets daw_redatum_batch time every shape on a short probe
- * sweep per batch shape and keep the fastest (the default).  No reference
- * counterpart: the launch shape of the autotuner. */
+ * sweep per batch shape and sweep mode and keep the fastest (the default).
+ * No reference counterpart: the launch shape of the autotuner. */
 int daw_set_tile(int rows, int cols);
 int daw_get_tile(int *rows, int *cols);
 /* Time steps advanced per launch of the sweep kernels: 1 (one launch per
@@ -251,6 +251,9 @@ int daw_get_tile(int *rows, int *cols);
  * No reference counterpart. */
 int daw_set_steps(int steps);
 int daw_get_steps(void);
+/* Launch configuration the autotuner chose for sweep mode 0 (forward),
+ * 1 (back-propagation) or 2 (re-injection) in the last batch. */
+int daw_get_launch(int mode, int *steps, int *rows, int *cols);
 int daw_reset_stats(void);
 int daw_get_stats(daw_stats *st);
 

@@ -36,6 +36,7 @@ EXPORTS = [
     "daw_get_stats", "daw_modeling_batch", "daw_set_tile", "daw_get_tile", "daw_filmartin2d_dev",
     "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
+    "daw_get_launch",
 ]
 
 
@@ -238,6 +239,14 @@ def set_steps(steps):
 
 def get_steps():
     return int(lib().daw_get_steps())
+
+
+def get_launch(mode):
+    """(steps per launch, tile rows, tile columns) the autotuner chose for
+    sweep mode 0 (forward), 1 (back-propagation) or 2 (re-injection)."""
+    st, r, c = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().daw_get_launch(int(mode), C.byref(st), C.byref(r), C.byref(c)))
+    return st.value, r.value, c.value
 
 
 def set_graph(on):

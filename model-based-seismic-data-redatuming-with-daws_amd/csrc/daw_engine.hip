@@ -47,6 +47,11 @@ static int fail(int code, const char *fmt, ...)
     } while (0)
 
 // ------------------------------------------------------------------ state
+// Launch configuration of one sweep mode: time steps per launch and tile.
+struct LaunchCfg {
+    int steps = 1, tz = 16, txw = 1;
+};
+
 struct Global {
     std::mutex mu;
     int hicks_ofs = -1000;     // ofs given to kwsinc2d; -1000: not initialised
@@ -66,7 +71,8 @@ struct Global {
     bool hess_exact = false;   // Hessian pass 1 = exact transpose of pass 2
     int snap_fmt = kSnapF32;   // snapshot store of the reference Hessian (SnapFormat)
     bool graph = false;        // replay full sweeps as captured hipGraphs
-    std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw}
+    std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw} per mode
+    LaunchCfg last_cfg[3];     // launch configuration of each mode in the last tuned batch
 };
 static Global g;
 
@@ -325,11 +331,17 @@ static std::vector<int2> make_tiles(const std::vector<int> &tiles_per_shot)
 }
 
 // ------------------------------------------------------------------ sweeps
+struct TileList {
+    DevBuf d;        // (shot, tile) pairs in XCD-aware launch order
+    int n = 0;
+};
+
 struct SweepSet {
     int nshots = 0;
     std::vector<ShotDev> host;
-    DevBuf d_shots, d_tiles;
-    int ntiles = 0;
+    DevBuf d_shots;
+    LaunchCfg cfg[3];                                   // per SweepMode
+    std::map<std::pair<int, int>, TileList> tiles;      // per (tz, txw)
     int nt_max = 0;
     long long pts_per_step = 0;
     int snap = kSnapF32;       // SnapFormat of the shots' snapshot store
@@ -352,7 +364,7 @@ struct SweepSet {
 };
 
 // Steps per launch of the sweeps: two only where the tile has a two-step kernel.
-static int steps_per_launch() { return (g.steps == 2 && tile2_shape_ok(g.tz, g.txw)) ? 2 : 1; }
+static int steps_per_launch(const LaunchCfg &c) { return (c.steps == 2 && tile2_shape_ok(c.tz, c.txw)) ? 2 : 1; }
 
 // Copies the record items of the next sweep to the batch's item buffer
 // (stream-ordered: the previous sweep has read the old items by then).
@@ -376,21 +388,29 @@ static int set_items(SweepSet &W, const std::vector<RecItem> &items)
     return 0;
 }
 
+// Tile lists of every configured mode, sizes, and the shot descriptors on the device.
 static int finalize_sweepset(SweepSet &W)
 {
     W.clear_graphs();
-    std::vector<int> tps(W.nshots);
     W.nt_max = 0;
     W.pts_per_step = 0;
     for (int s = 0; s < W.nshots; ++s) {
-        const ShotDev &S = W.host[s];
-        tps[s] = ((S.n1e + g.tz - 1) / g.tz) * ((S.n2e + TX * g.txw - 1) / (TX * g.txw));
-        W.nt_max = std::max(W.nt_max, S.nt);
-        W.pts_per_step += (long long)S.n1e * S.n2e;
+        W.nt_max = std::max(W.nt_max, W.host[s].nt);
+        W.pts_per_step += (long long)W.host[s].n1e * W.host[s].n2e;
     }
-    std::vector<int2> tiles = make_tiles(tps);
-    W.ntiles = (int)tiles.size();
-    if (int rc = upload(W.d_tiles, tiles)) return rc;
+    for (const LaunchCfg &c : W.cfg) {
+        const std::pair<int, int> key{c.tz, c.txw};
+        if (W.tiles.count(key)) continue;
+        std::vector<int> tps(W.nshots);
+        for (int s = 0; s < W.nshots; ++s) {
+            const ShotDev &S = W.host[s];
+            tps[s] = ((S.n1e + c.tz - 1) / c.tz) * ((S.n2e + TX * c.txw - 1) / (TX * c.txw));
+        }
+        std::vector<int2> tl = make_tiles(tps);
+        TileList &T = W.tiles[key];
+        T.n = (int)tl.size();
+        if (int rc = upload(T.d, tl)) return rc;
+    }
     return upload(W.d_shots, W.host);
 }
 
@@ -404,6 +424,8 @@ static int refresh_shots(SweepSet &W)
 // step launches and the trailing record.
 static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int nsteps)
 {
+    const LaunchCfg &c = W.cfg[mode];
+    const TileList &T = W.tiles.at({c.tz, c.txw});
     for (int s = 0; s < W.nshots; ++s) {
         const ShotDev &S = W.host[s];
         const size_t pe = (size_t)S.n1e * S.ldx;
@@ -414,24 +436,24 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
     }
     StepParams P;
     P.shots = W.d_shots.as<ShotDev>();
-    P.tiles = W.d_tiles.as<int2>();
+    P.tiles = T.d.as<int2>();
     P.rec = d_rec;
     P.nrec = nrec;
     // launch k advances steps it .. it+nsub-1 and records the steps of launch k-1
-    const int per = steps_per_launch();
+    const int per = steps_per_launch(c);
     int cur = 0, prev_it = 0, prev_n = 0;
     for (int it = 0;; it += per) {
         const bool more = it < nsteps;
         const int n = more ? std::min(per, nsteps - it) : 0;
-        P.ntiles = more ? W.ntiles : 0;
+        P.ntiles = more ? T.n : 0;
         P.it = it;
         P.nsub = n;
         P.cur = cur;
         P.rec_it = prev_it;
         P.rec_nsub = nrec > 0 ? prev_n : 0;
         P.nrec_blocks = (nrec * P.rec_nsub + kRecPerBlock - 1) / kRecPerBlock;
-        if (per == 2) launch_step2(mode, W.snap, P, g.tz, g.txw, g.stream);
-        else launch_step(mode, W.snap, P, g.tz, g.txw, g.stream);
+        if (per == 2) launch_step2(mode, W.snap, P, c.tz, c.txw, g.stream);
+        else launch_step(mode, W.snap, P, c.tz, c.txw, g.stream);
         cur ^= 1;
         prev_it = it;
         prev_n = n;
@@ -453,8 +475,10 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
     }
     const int nsteps = nsteps_limit > 0 ? std::min(nsteps_limit, W.nt_max) : W.nt_max;
     if (g.graph && nsteps_limit == 0) {
-        const std::vector<long long> key{mode, nsteps, nrec, steps_per_launch(), g.tz, g.txw, W.snap, W.ntiles, (long long)(intptr_t)d_rec,
-                                         (long long)(intptr_t)W.d_tiles.p, (long long)(intptr_t)W.d_shots.p};
+        const LaunchCfg &c = W.cfg[mode];
+        const std::vector<long long> key{mode, nsteps, nrec, steps_per_launch(c), c.tz, c.txw, W.snap,
+                                         (long long)(intptr_t)d_rec, (long long)(intptr_t)W.tiles.at({c.tz, c.txw}).d.p,
+                                         (long long)(intptr_t)W.d_shots.p};
         auto hit = W.graphs.find(key);
         hipGraphExec_t ex = nullptr;
         if (hit == W.graphs.end()) {
@@ -490,7 +514,8 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
         (void)hipEventDestroy(e1);
     }
     g.pts += W.pts_per_step * nsteps;
-    g.launches += (nsteps + steps_per_launch() - 1) / steps_per_launch() + 1;
+    const int per = steps_per_launch(W.cfg[mode]);
+    g.launches += (nsteps + per - 1) / per + 1;
     return 0;
 }
 
@@ -506,9 +531,11 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
         launch_fill(S.vx[0], pe, 0.0f, g.stream);
         launch_fill(S.vz[0], pe, 0.0f, g.stream);
     }
+    const LaunchCfg &c = W.cfg[kBack];                  // the one-step tile of the back-propagation
+    const TileList &T = W.tiles.at({c.tz, c.txw});
     StepParams P;
     P.shots = W.d_shots.as<ShotDev>();
-    P.tiles = W.d_tiles.as<int2>();
+    P.tiles = T.d.as<int2>();
     P.rec = d_rec;
     const int rec_blocks = (nrec + kRecPerBlock - 1) / kRecPerBlock;
     int cur = 0;
@@ -516,14 +543,14 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
     P.nsub = 1;
     for (int k = 0; k <= W.nt_max; ++k) {
         const int it = W.nt_max - 1 - k;
-        P.ntiles = (k < W.nt_max) ? W.ntiles : 0;
+        P.ntiles = (k < W.nt_max) ? T.n : 0;
         P.it = it < 0 ? 0 : it;
         P.cur = cur;
         const bool rec_prev = k > 0 && nrec > 0;
         P.rec_nsub = rec_prev ? 1 : 0;
         P.nrec_blocks = rec_prev ? rec_blocks : 0;
         P.rec_it = it + 1;
-        launch_adjoint(hess, P, g.tz, g.txw, g.stream);
+        launch_adjoint(hess, P, c.tz, c.txw, g.stream);
         cur ^= 1;
     }
     HIPCHK(hipGetLastError());
@@ -705,85 +732,83 @@ struct HessOut {
     int record = 1;            // type_rec == 0
 };
 
-// Tile autotuner: time a short back-propagation probe (zero residuals, best of
-// three) at every compiled tile shape and steps-per-launch choice and keep the
-// fastest for this batch shape.
-// DAW_TUNE_LOG=1 prints the probe times to stderr.
+// Launch autotuner: for each sweep mode (forward, back-propagation,
+// re-injection) time a short probe (zero amplitudes, best of three) at every
+// compiled tile shape and steps-per-launch choice and keep the fastest for
+// this batch shape.  DAW_TUNE_LOG=1 prints the probe times to stderr.
 static int autotune_tiles(Batch &B)
 {
+    SweepSet &W = B.W;
     if (!g.tz_auto && !g.steps_auto) return 0;
     // candidates: {steps, rows, wavefronts across}; fixed settings are kept
     static const int shapes[][3] = {{1, 8, 2}, {1, 16, 1}, {1, 16, 2}, {1, 32, 1}, {1, 32, 2}, {1, 64, 1},
                                     {2, 16, 1}, {2, 32, 1}, {2, 16, 2}};
-    std::vector<std::vector<int>> cand;
+    std::vector<LaunchCfg> cand;
     for (const auto &sh : shapes) {
         if (!g.steps_auto && sh[0] != g.steps) continue;
         if (!g.tz_auto && (sh[1] != g.tz || sh[2] != g.txw)) continue;
         cand.push_back({sh[0], sh[1], sh[2]});
     }
-    if (cand.empty()) return 0;
-    const std::vector<int> key{(int)(B.W.pts_per_step >> 10), B.geo_of(0).G.n2e, B.nshots, g.steps_auto ? 0 : g.steps,
+    if (cand.empty() || B.store.empty() || !B.store[0].has_snap()) return 0;
+    const std::vector<int> key{(int)(W.pts_per_step >> 10), B.geo_of(0).G.n2e, B.nshots, g.steps_auto ? 0 : g.steps,
                                g.tz_auto ? 0 : g.tz, g.tz_auto ? 0 : g.txw};
     auto hit = g.tz_cache.find(key);
-    if (hit != g.tz_cache.end()) {
-        const std::vector<int> &b = hit->second;
-        if (b[0] != g.steps || b[1] != g.tz || b[2] != g.txw) {
-            g.steps = b[0];
-            g.tz = b[1];
-            g.txw = b[2];
-            return finalize_sweepset(B.W);
+    if (hit == g.tz_cache.end()) {
+        DevBuf zeros;
+        if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
+        launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
+        for (int s = 0; s < B.nshots; ++s) {
+            set_inject(W.host[s], B.geo_of(s).inj_rec, zeros.as<float>(), 0, 0);
+            W.host[s].amp_rev = 0;
         }
-        return 0;
-    }
-    if (B.store.empty() || !B.store[0].has_snap()) return 0;
-    DevBuf zeros;
-    if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
-    launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
-    for (int s = 0; s < B.nshots; ++s) {
-        set_inject(B.W.host[s], B.geo_of(s).inj_rec, zeros.as<float>(), 0, 0);
-        B.W.host[s].amp_rev = 0;
-    }
-    if (int rc = refresh_shots(B.W)) return rc;
-    const bool timing = g.timing;
-    const long long pts = g.pts, launches = g.launches;
-    g.timing = false;
-    const int probe = std::min(24, B.nt);
-    std::vector<int> best = cand[0];
-    float best_ms = 1e30f;
-    const bool log = getenv("DAW_TUNE_LOG") != nullptr;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    for (const auto &sh : cand) {
-        g.steps = sh[0];
-        g.tz = sh[1];
-        g.txw = sh[2];
-        if (int rc = finalize_sweepset(B.W)) return rc;
-        if (int rc = run_sweep(kBack, B.W, nullptr, 0, 4)) return rc;       // warm
-        float ms = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {                                  // best of three probes
-            HIPCHK(hipEventRecord(e0, g.stream));
-            if (int rc = run_sweep(kBack, B.W, nullptr, 0, probe)) return rc;
-            HIPCHK(hipEventRecord(e1, g.stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float m = 0;
-            HIPCHK(hipEventElapsedTime(&m, e0, e1));
-            ms = std::min(ms, m);
+        if (int rc = refresh_shots(W)) return rc;
+        const bool timing = g.timing;
+        const long long pts = g.pts, launches = g.launches;
+        g.timing = false;
+        const int probe = std::min(24, B.nt);
+        const bool log = getenv("DAW_TUNE_LOG") != nullptr;
+        static const char *names[3] = {"forward", "back-propagation", "re-injection"};
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        std::vector<int> best(9);
+        for (int mode : {kBack, kReinject, kFwd}) {
+            LaunchCfg pick = cand[0];
+            float best_ms = 1e30f;
+            for (const LaunchCfg &c : cand) {
+                W.cfg[mode] = c;
+                if (int rc = finalize_sweepset(W)) return rc;
+                if (int rc = run_sweep(mode, W, nullptr, 0, 4)) return rc;       // warm
+                float ms = 1e30f;
+                for (int rep = 0; rep < 3; ++rep) {                              // best of three probes
+                    HIPCHK(hipEventRecord(e0, g.stream));
+                    if (int rc = run_sweep(mode, W, nullptr, 0, probe)) return rc;
+                    HIPCHK(hipEventRecord(e1, g.stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float m = 0;
+                    HIPCHK(hipEventElapsedTime(&m, e0, e1));
+                    ms = std::min(ms, m);
+                }
+                if (log)
+                    fprintf(stderr, "[daw tune] %d shots %dx%d grid, %s: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
+                            B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], c.steps, c.tz, 64 * c.txw,
+                            1000.0f * ms / probe);
+                if (ms < best_ms) { best_ms = ms; pick = c; }
+            }
+            W.cfg[mode] = pick;
+            best[3 * mode] = pick.steps; best[3 * mode + 1] = pick.tz; best[3 * mode + 2] = pick.txw;
         }
-        if (log) fprintf(stderr, "[daw tune] %d shots %dx%d grid: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
-                         B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, sh[0], sh[1], 64 * sh[2], 1000.0f * ms / probe);
-        if (ms < best_ms) { best_ms = ms; best = sh; }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        g.timing = timing;
+        g.pts = pts;
+        g.launches = launches;
+        hit = g.tz_cache.emplace(key, best).first;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    g.timing = timing;
-    g.pts = pts;
-    g.launches = launches;
-    g.tz_cache[key] = best;
-    g.steps = best[0];
-    g.tz = best[1];
-    g.txw = best[2];
-    return finalize_sweepset(B.W);
+    const std::vector<int> &b = hit->second;
+    for (int m = 0; m < 3; ++m) W.cfg[m] = {b[3 * m], b[3 * m + 1], b[3 * m + 2]};
+    for (int m = 0; m < 3; ++m) g.last_cfg[m] = W.cfg[m];
+    return finalize_sweepset(W);
 }
 
 // Pressure extended-source range of the re-injection: 1..n (esource_data,
@@ -900,6 +925,7 @@ static int finish_batch(Batch &B, bool want_snap)
     B.store.clear();
     B.store.resize(B.nshots);
     B.W.nshots = B.nshots;
+    for (auto &c : B.W.cfg) c = {g.steps, g.tz, g.txw};
     B.W.snap = fmt < 0 ? (int)kSnapF32 : fmt;
     B.W.host.resize(B.nshots);
     B.roff.assign(B.nshots + 1, 0);
@@ -1656,6 +1682,17 @@ int daw_set_steps(int steps)
         return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel", g.tz, 64 * g.txw);
     g.steps_auto = false;
     g.steps = steps;
+    return 0;
+}
+
+int daw_get_launch(int mode, int *steps, int *rows, int *cols)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (mode < 0 || mode > 2) return fail(DAW_ERR_ARG, "sweep mode %d: 0 forward, 1 back-propagation, 2 re-injection", mode);
+    const LaunchCfg &c = g.last_cfg[mode];
+    if (steps) *steps = c.steps;
+    if (rows) *rows = c.tz;
+    if (cols) *cols = 64 * c.txw;
     return 0;
 }
 

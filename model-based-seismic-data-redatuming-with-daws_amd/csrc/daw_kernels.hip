@@ -381,7 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     constexpr int PROWS = (PH_ + NWZ - 1) / NWZ;
     constexpr int VROWS = TZ / NWZ;
     static_assert(TZ % NWZ == 0 && 3 * TZ <= kBlock, "tile rows must split evenly over the wavefront rows");
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the wavefront index is uniform: in an SGPR, row-level conditions and loads are scalar
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     constexpr int LDS_N = VXH_ * VXW_ + VZH_ * VZW_ + PH_ * PW_;
     static_assert(LDS_N >= kRecPerBlock * kRecStride, "record blocks reuse the tile's LDS");
     __shared__ __align__(16) float s_lds[LDS_N];
@@ -666,7 +667,8 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
     constexpr int VXW_ = TXT + 6, VXH_ = TZ + 3;
     constexpr int VZW_ = TXT + 3, VZH_ = TZ + 6;
     constexpr int VROWS = TZ / NWZ;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the wavefront index is uniform: in an SGPR, row-level conditions and loads are scalar
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     constexpr int LDS_N = VXH_ * VXW_ + VZH_ * VZW_ + 2 * PH_ * PW_;
     static_assert(LDS_N >= kRecPerBlock * kRecStride, "record blocks reuse the tile's LDS");
     __shared__ __align__(16) float s_lds[LDS_N];
@@ -920,7 +922,8 @@ __global__ __launch_bounds__(kBlock) void k_step2(StepParams P)
     constexpr int LDS_N = 3 * RH * RW;
     static_assert(LDS_N >= kRecPerBlock * kRecStride, "record blocks reuse the tile's LDS");
     __shared__ __align__(16) float s_lds[LDS_N];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the wavefront index is uniform: in an SGPR, row-level conditions and loads are scalar
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     if ((int)blockIdx.x < P.nrec_blocks) {
         record_block(P, blockIdx.x, s_lds);
         return;
