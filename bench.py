@@ -253,6 +253,17 @@ def f64_check(P, datum, hist, nshots):
     return {"f64_shots": nshots, "datum_rel_l2_vs_f64": e_dat, "minres_hist_max_rel_diff_vs_f64": e_hist}
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel configuration, measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (calibrated on a device copy) and recorded in
+    profiles/hbm_traffic.json; None when this configuration was not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(key, {}).get("bytes_per_launch")
+
+
 def calibrate_copy(nbytes=1 << 30):
     import torch
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -355,6 +366,11 @@ def main():
     shots_per_launch = nsh / int(np.ceil(nsh / mb))
     alg_bytes = bytes_per_shot_launch * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
+    snap_code = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snapshots]
+    traffic_key = (f"{kname}<{dom},{dcfg[1]},{dcfg[2] // 64},{snap_code}> {w.nz}x{w.nx} "
+                   f"{int(round(shots_per_launch))} shots per launch")
+    traffic = measured_traffic(traffic_key)
     # quality: redatumed (unfiltered) gathers vs data modelled at the datum in the true model
     err = float(torch.linalg.norm(datum - P["dnew"]) / torch.linalg.norm(P["dnew"]))
     # plain path: one shot at a time
@@ -414,7 +430,8 @@ def main():
         "redatum_rel_l2_vs_true_datum_data": err,
         "minres_final_rel_residual_mean": float(np.mean(hist[:, -1])) if hist is not None else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None, "kernel": modes[dom],
+                     "frac": achieved / 8000.0, "traffic": traffic, "traffic_key": traffic_key,
+                     "kernel": modes[dom],
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes,
                      "steps_per_launch": launch_steps},
         "cpu_baseline": cpu,
