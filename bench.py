@@ -368,8 +368,9 @@ def main():
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
     snap_code = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snapshots]
-    traffic_key = (f"{kname}<{dom},{dcfg[1]},{dcfg[2] // 64},{snap_code}> {w.nz}x{w.nx} "
-                   f"{int(round(shots_per_launch))} shots per launch")
+    targs = (f"{dom},{snap_code},{dcfg[1]},{dcfg[2] // 64}" if launch_steps == 2     # k_step2<MODE, SNAP, TZ, NC>
+             else f"{dom},{dcfg[1]},{dcfg[2] // 64},{snap_code}")                     # k_step<MODE, TZ, TXW, SNAP>
+    traffic_key = f"{kname}<{targs}> {w.nz}x{w.nx} {int(round(shots_per_launch))} shots per launch"
     traffic = measured_traffic(traffic_key)
     # quality: redatumed (unfiltered) gathers vs data modelled at the datum in the true model
     err = float(torch.linalg.norm(datum - P["dnew"]) / torch.linalg.norm(P["dnew"]))

@@ -48,6 +48,7 @@ struct ShotDev {
     const int *inj_ptr;
     const int *inj_r;
     const float *inj_cdt;
+    int inj_z0, inj_z1;                 // first and last row with inj_row[z] >= 0 (z1 < z0: none)
     const float *inj_amp;               // amplitude of receiver r at step it: inj_amp[r*amp_stride + it]
     int amp_stride;
     int amp_rev;                        // amplitude time index nt-1-it (adjoint of a time-reversed input)

@@ -239,6 +239,7 @@ static int make_points(PointSet &P, const int *tab, int n, const Grid &G, bool i
 // Injection gather of a point set, window clipped to the extended grid.
 struct Inject {
     std::vector<int> row;      // n1e
+    int z0 = 0, z1 = -1;       // first and last row with row[z] >= 0
     std::vector<int> ptr;      // nq*(n2e+1)
     std::vector<int> rid;
     std::vector<float> cdt;
@@ -254,8 +255,14 @@ static int make_inject(Inject &I, const PointSet &P, const Grid &G)
             const int z = P.ix1[r] + i1;
             if (z >= 0 && z < G.n1e && I.row[z] < 0) I.row[z] = 0;
         }
+    I.z0 = 0;
+    I.z1 = -1;
     for (int z = 0; z < G.n1e; ++z)
-        if (I.row[z] >= 0) I.row[z] = nq++;
+        if (I.row[z] >= 0) {
+            if (nq == 0) I.z0 = z;
+            I.z1 = z;
+            I.row[z] = nq++;
+        }
     std::vector<int> cnt((size_t)nq * (G.n2e + 1), 0);
     auto for_window = [&](int r, auto &&fn) {
         const int c1 = P.ix1[r], c2 = P.ix2[r];
@@ -635,6 +642,8 @@ static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &s
 static void set_inject(ShotDev &S, const Inject &I, const float *amp, int amp_stride, int vinj)
 {
     S.inj_row = I.d_row.as<int>();
+    S.inj_z0 = I.z0;
+    S.inj_z1 = I.z1;
     S.inj_ptr = I.d_ptr.as<int>();
     S.inj_r = I.d_rid.as<int>();
     S.inj_cdt = I.d_cdt.as<float>();

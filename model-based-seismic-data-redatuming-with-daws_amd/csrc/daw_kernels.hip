@@ -351,6 +351,183 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
     return p;
 }
 
+// Interior tile of k_step: every point of the pressure rectangle (tile plus
+// ring) lies where all four sponge profiles are exactly 1 (extended rows and
+// columns npml .. npml+n-1), inside the stencil box, below the free surface,
+// off the injection rows and inside the re-injection range.  There the
+// reference's update multiplies by damp = damp1 = 1.0f and bw = dtdx*1*1 =
+// dtdx, which are exact, so dropping those factors gives the same bits, and
+// no load needs a clamp: the tile runs without per-point branches.  Layout,
+// LDS use and statement order are those of the general body in k_step.
+template <int MODE, int TZ, int TXW, int SNAP>
+__device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float *s_vz, float *s_p, int z0, int x0, int it,
+                                              int ic, int wv, int lane, int wseg)
+{
+    constexpr int NW = kBlock / 64;
+    constexpr int TXT = 64 * TXW;
+    constexpr int NWZ = NW / TXW;
+    constexpr int PW_ = TXT + 3, PH_ = TZ + 3;
+    constexpr int VXW_ = TXT + 6, VXH_ = TZ + 3;
+    constexpr int VZW_ = TXT + 3, VZH_ = TZ + 6;
+    constexpr int PROWS = (PH_ + NWZ - 1) / NWZ;
+    constexpr int VROWS = TZ / NWZ;
+    constexpr int XR = (VXH_ + NWZ - 1) / NWZ, ZR = (VZH_ + NWZ - 1) / NWZ;
+    constexpr int HXN = (VXH_ * 6 + kBlock - 1) / kBlock, HZN = (VZH_ * 3 + kBlock - 1) / kBlock;
+    const float a0 = 1.125f;
+    const float a1 = -(1.0f / 24.0f);
+    const int tid = threadIdx.x, ldx = S.ldx;
+    const int cl = (wv % TXW) * 64 + lane, rz = wv / TXW, gx = x0 + cl;
+    const float dtdx = S.dtdx, mdt = -S.dt;
+
+    // ---- 1. loads; a wavefront row past the end of a rectangle repeats the
+    // last row (the same value lands twice in LDS, and that row is never owned)
+    float vxl[XR], vzl[ZR];
+#pragma unroll
+    for (int j = 0; j < XR; ++j) {
+        const int r = min(rz + NWZ * j, VXH_ - 1);
+        vxl[j] = ld(S.vxo, (unsigned)((z0 - 1 + r) * ldx + gx));
+    }
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) {
+        const int r = min(rz + NWZ * j, VZH_ - 1);
+        vzl[j] = ld(S.vzo, (unsigned)((z0 - 3 + r) * ldx + gx));
+    }
+    float hx[HXN], hz[HZN];
+    int hx_i[HXN], hz_i[HZN];
+#pragma unroll
+    for (int q = 0; q < HXN; ++q) {
+        const int i = tid + q * kBlock;
+        hx[q] = 0.0f;
+        hx_i[q] = -1;
+        if (i < VXH_ * 6) {
+            const int r = i / 6, k = i - 6 * r;
+            const int c = k < 3 ? k : TXT + k;
+            hx_i[q] = r * VXW_ + c;
+            hx[q] = ld(S.vxo, (unsigned)((z0 - 1 + r) * ldx + x0 - 3 + c));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < HZN; ++q) {
+        const int i = (kBlock - 1 - tid) + q * kBlock;
+        hz[q] = 0.0f;
+        hz_i[q] = -1;
+        if (i < VZH_ * 3) {
+            const int r = i / 3, k = i - 3 * r;
+            const int c = k == 0 ? 0 : TXT + k;
+            hz_i[q] = r * VZW_ + c;
+            hz[q] = ld(S.vzo, (unsigned)((z0 - 3 + r) * ldx + x0 - 1 + c));
+        }
+    }
+    float pxr[PROWS], pzr[PROWS], kar[PROWS], spr[PROWS];
+#pragma unroll
+    for (int j = 0; j < PROWS; ++j) {
+        const int r = min(rz + NWZ * j, PH_ - 1);
+        const int gz = z0 - 1 + r;
+        const unsigned g = (unsigned)(gz * ldx + gx);
+        pxr[j] = ld(S.pxo, g);
+        pzr[j] = ld(S.pzo, g);
+        kar[j] = ld(S.kappa, g);
+        spr[j] = MODE == kReinject ? snap_load_w<SNAP>(S, S.sp, S.sp16, 0, ic, gz, gx, wseg) : 0.0f;
+    }
+    const int sid = kBlock - 1 - tid;
+    const bool side = sid < 3 * TZ;
+    int s_r = 0, s_c = 0;
+    float s_px = 0.0f, s_pz = 0.0f, s_ka = 0.0f, s_sp = 0.0f;
+    if (side) {
+        const int k = sid / TZ;
+        s_r = 1 + (sid - k * TZ);
+        s_c = k == 0 ? 0 : TXT + k;
+        const int s_gz = z0 - 1 + s_r, s_gx = x0 - 1 + s_c;
+        const unsigned g = (unsigned)(s_gz * ldx + s_gx);
+        s_px = ld(S.pxo, g);
+        s_pz = ld(S.pzo, g);
+        s_ka = ld(S.kappa, g);
+        if (MODE == kReinject) s_sp = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, s_gz, s_gx);
+    }
+    float svxr[VROWS], svzr[VROWS];
+#pragma unroll
+    for (int j = 0; j < VROWS; ++j) {
+        const int gz = z0 + rz + NWZ * j;
+        svxr[j] = MODE == kReinject ? snap_load_w<SNAP>(S, S.svx, S.svx16, 1, ic, gz, gx, wseg) : 0.0f;
+        svzr[j] = MODE == kReinject ? snap_load_w<SNAP>(S, S.svz, S.svz16, 2, ic, gz, gx, wseg) : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < XR; ++j) s_vx[min(rz + NWZ * j, VXH_ - 1) * VXW_ + cl + 3] = vxl[j];
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) s_vz[min(rz + NWZ * j, VZH_ - 1) * VZW_ + cl + 1] = vzl[j];
+#pragma unroll
+    for (int q = 0; q < HXN; ++q)
+        if (hx_i[q] >= 0) s_vx[hx_i[q]] = hx[q];
+#pragma unroll
+    for (int q = 0; q < HZN; ++q)
+        if (hz_i[q] >= 0) s_vz[hz_i[q]] = hz[q];
+    __syncthreads();
+
+    // ---- 2. pressure half-step: split update, extended source, p = px + pz
+#pragma unroll
+    for (int j = 0; j < PROWS; ++j) {
+        const int r = min(rz + NWZ * j, PH_ - 1);
+        const int gz = z0 - 1 + r;
+        const float *vr = &s_vx[r * VXW_ + cl + 3];
+        const float *vc = &s_vz[(r + 2) * VZW_ + cl + 1];
+        const float der2 = a0 * (vr[0] - vr[-1]) + a1 * (vr[1] - vr[-2]);
+        const float der1 = a0 * (vc[0] - vc[-VZW_]) + a1 * (vc[VZW_] - vc[-2 * VZW_]);
+        const float tk = dtdx * kar[j];
+        float pxv = MODE == kBack ? pxr[j] - tk * der2 : pxr[j] + tk * der2;
+        const float pzv = MODE == kBack ? pzr[j] - tk * der1 : pzr[j] + tk * der1;
+        if (MODE == kReinject) pxv = pxv + (mdt * spr[j]) * kar[j];
+        const float p = pxv + pzv;
+        if (r >= 1 && r <= TZ) {
+            const unsigned g = (unsigned)(gz * ldx + gx);
+            st(S.pxn, g, pxv);
+            st(S.pzn, g, pzv);
+            if (MODE != kBack && S.sd->band_row != nullptr) {
+                const int slot = glb(S.sd->band_row)[gz];
+                if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = p;
+            }
+            if (MODE == kBack) snap_store_wave<SNAP>(S, S.sp, S.sp16, 0, it, gz, gx, p, true, true);
+        }
+        s_p[r * PW_ + cl + 1] = p;
+    }
+    if (side) {
+        const float *vr = &s_vx[s_r * VXW_ + s_c + 2];
+        const float *vc = &s_vz[(s_r + 2) * VZW_ + s_c];
+        const float der2 = a0 * (vr[0] - vr[-1]) + a1 * (vr[1] - vr[-2]);
+        const float der1 = a0 * (vc[0] - vc[-VZW_]) + a1 * (vc[VZW_] - vc[-2 * VZW_]);
+        const float tk = dtdx * s_ka;
+        float pxv = MODE == kBack ? s_px - tk * der2 : s_px + tk * der2;
+        const float pzv = MODE == kBack ? s_pz - tk * der1 : s_pz + tk * der1;
+        if (MODE == kReinject) pxv = pxv + (mdt * s_sp) * s_ka;
+        s_p[s_r * PW_ + s_c] = pxv + pzv;
+    }
+    __syncthreads();
+
+    // ---- 3. velocity half-step, snapshot re-injection or store
+#pragma unroll
+    for (int j = 0; j < VROWS; ++j) {
+        const int r = rz + NWZ * j;
+        const int gz = z0 + r;
+        const unsigned g = (unsigned)(gz * ldx + gx);
+        float vxv = s_vx[(r + 1) * VXW_ + cl + 3];
+        float vzv = s_vz[(r + 3) * VZW_ + cl + 1];
+        const float *pp = &s_p[(r + 1) * PW_ + cl + 1];
+        const float der2 = a0 * (pp[1] - pp[0]) + a1 * (pp[2] - pp[-1]);
+        const float der1 = a0 * (pp[PW_] - pp[0]) + a1 * (pp[2 * PW_] - pp[-PW_]);
+        vxv = MODE == kBack ? vxv - dtdx * der2 : vxv + dtdx * der2;
+        vzv = MODE == kBack ? vzv - dtdx * der1 : vzv + dtdx * der1;
+        if (MODE == kReinject) {
+            vxv = vxv + mdt * svxr[j];
+            vzv = vzv + mdt * svzr[j];
+        }
+        st(S.vxn, g, vxv);
+        st(S.vzn, g, vzv);
+        if (MODE == kBack) {
+            snap_store_wave<SNAP>(S, S.svx, S.svx16, 1, it, gz, gx, vxv, true, true);
+            snap_store_wave<SNAP>(S, S.svz, S.svz16, 2, it, gz, gx, vzv, true, true);
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -423,6 +600,20 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     // this wavefront's 64-column snapshot segment (lanes past the grid read
     // the last one and discard the value)
     const int wseg = __builtin_amdgcn_readfirstlane(min((x0 >> 6) + (wv % TXW), S.nseg - 1));
+    {   // interior tiles take the branch-free body (the same bits, see tile_interior)
+        const int pzl = z0 - 1, pzh = z0 + TZ + 1, pxl = x0 - 1, pxh = x0 + TXT + 1;
+        bool fast = pzl >= S.npml && pzh < S.npml + S.n1 && pxl >= S.npml && pxh < S.npml + S.n2 && pzl >= lo &&
+                    pzh <= hiz && pxl >= lo && pxh <= hix && !(S.ofs == 1 && pzl <= S.npml);
+        if (MODE != kReinject && S.sd->inj_row != nullptr && S.sd->inj_z0 <= pzh && S.sd->inj_z1 >= pzl) fast = false;
+        if (MODE != kBack && S.sd->wf != nullptr) fast = false;
+        if (MODE == kReinject &&
+            !(S.sd->xlo1 <= pzl && pzh <= S.sd->xhi1 && S.sd->xlo2 <= pxl && pxh <= S.sd->xhi2))
+            fast = false;
+        if (fast) {
+            tile_interior<MODE, TZ, TXW, SNAP>(S, s_vx, s_vz, s_p, z0, x0, it, ic, wv, lane, wseg);
+            return;
+        }
+    }
 
     // ---- 1. every global load of the step is issued here, into registers,
     // before anything waits on one: velocity halos, the pressure rectangle,
