@@ -177,7 +177,7 @@ def main_uneven(args, daw, syn):
                      "kernel": "k_step<1>/<2> (Hessian sweeps, average over the run)"},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    _emit(out)
 
 
 def cpu_baseline(P, nt_sample):
@@ -279,11 +279,28 @@ def calibrate_copy(nbytes=1 << 30):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / 10
-    print(json.dumps({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 13,
-                      "copy_GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}))
+    _emit({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 13,
+           "copy_GBps_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9})
+
+
+_JSON_OUT = None
+
+
+def _emit(obj):
+    """The one JSON line, on the process's original stdout."""
+    out = _JSON_OUT if _JSON_OUT is not None else sys.stdout
+    out.write(json.dumps(obj) + "\n")
+    out.flush()
 
 
 def main():
+    # stdout carries the JSON line only: text that native libraries write to
+    # file descriptor 1 (the reference's Fortran runtime flushes its buffered
+    # output when the process exits) goes to stderr instead
+    global _JSON_OUT
+    sys.stdout.flush()
+    _JSON_OUT = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -439,7 +456,7 @@ def main():
     }
     if mixed:
         out["precision_check"] = mixed
-    print(json.dumps(out), flush=True)
+    _emit(out)
 
 
 if __name__ == "__main__":

@@ -37,6 +37,17 @@ __device__ __forceinline__ void st(GLB T *base, unsigned idx, T v)
     *(GLB T *)((GLB char *)base + idx * (unsigned)sizeof(T)) = v;
 }
 
+// A field of a shot's state or profile block: one uniform base (px[0] of the
+// shot's 8 state planes, or the model's profile block) and the 32-bit element
+// offset of the field in it, so the kernels keep one base pointer per block.
+struct Fld {
+    GLB float *base;
+    unsigned off;
+    __device__ __forceinline__ float operator[](int i) const { return ld(base, off + (unsigned)i); }
+};
+__device__ __forceinline__ float ld(const Fld &f, unsigned idx) { return ld(f.base, f.off + idx); }
+__device__ __forceinline__ void st(const Fld &f, unsigned idx, float v) { st(f.base, f.off + idx, v); }
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi)
 {
     return v < lo ? lo : (v > hi ? hi : v);
@@ -152,6 +163,9 @@ struct View {
     GLB float *ssc;
     int nseg, spitch, sxb;
     const ShotDev *__restrict__ sd;   // the rest (injection, record band, ranges), read where used
+    // the same fields as base + offset (one base per block)
+    Fld fpxo, fpzo, fvxo, fvzo, fpxn, fpzn, fvxn, fvzn;
+    Fld fd1a, fd1a1, fd1b, fbw1, fd2a, fd2a1, fd2b, fbw2;
 };
 
 __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur, int it)
@@ -170,6 +184,16 @@ __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur
     v.sp = glb(S->sp); v.svx = glb(S->svx); v.svz = glb(S->svz);
     v.sp16 = glb(S->sp16); v.svx16 = glb(S->svx16); v.svz16 = glb(S->svz16); v.ssc = glb(S->ssc); v.nseg = S->nseg;
     v.spitch = S->spitch; v.sxb = S->sxb;
+    {
+        GLB float *st0 = glb(S->px[0]);
+        const unsigned pe = (unsigned)(S->n1e * S->ldx), ro = c1 ? 4 * pe : 0u, wo = c1 ? 0u : 4 * pe;
+        v.fpxo = {st0, ro}; v.fpzo = {st0, ro + pe}; v.fvxo = {st0, ro + 2 * pe}; v.fvzo = {st0, ro + 3 * pe};
+        v.fpxn = {st0, wo}; v.fpzn = {st0, wo + pe}; v.fvxn = {st0, wo + 2 * pe}; v.fvzn = {st0, wo + 3 * pe};
+        GLB float *pr = (GLB float *)glb(S->d1a);
+        const unsigned a = (unsigned)S->n1e, b = (unsigned)S->n2e;
+        v.fd1a = {pr, 0}; v.fd1a1 = {pr, a}; v.fd1b = {pr, 2 * a}; v.fbw1 = {pr, 3 * a};
+        v.fd2a = {pr, 4 * a}; v.fd2a1 = {pr, 4 * a + b}; v.fd2b = {pr, 4 * a + 2 * b}; v.fbw2 = {pr, 4 * a + 3 * b};
+    }
     return v;
 }
 
@@ -300,7 +324,7 @@ template <int MODE, int VZW_>
 __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, int gx, float pxv, float pzv,
                                          float kap, const float *vr, const float *vc, float d1a, float d1a1,
                                          float d2a, float d2a1, int injq, bool owned, int lo, int hiz, int hix,
-                                         float spv, GLB float *__restrict__ pxn, GLB float *__restrict__ pzn)
+                                         float spv, const Fld &pxn, const Fld &pzn)
 {
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
@@ -385,12 +409,12 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
 #pragma unroll
     for (int j = 0; j < XR; ++j) {
         const int r = min(rz + NWZ * j, VXH_ - 1);
-        vxl[j] = ld(S.vxo, (unsigned)((z0 - 1 + r) * ldx + gx));
+        vxl[j] = ld(S.fvxo, (unsigned)((z0 - 1 + r) * ldx + gx));
     }
 #pragma unroll
     for (int j = 0; j < ZR; ++j) {
         const int r = min(rz + NWZ * j, VZH_ - 1);
-        vzl[j] = ld(S.vzo, (unsigned)((z0 - 3 + r) * ldx + gx));
+        vzl[j] = ld(S.fvzo, (unsigned)((z0 - 3 + r) * ldx + gx));
     }
     float hx[HXN], hz[HZN];
     int hx_i[HXN], hz_i[HZN];
@@ -403,7 +427,7 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
             const int r = i / 6, k = i - 6 * r;
             const int c = k < 3 ? k : TXT + k;
             hx_i[q] = r * VXW_ + c;
-            hx[q] = ld(S.vxo, (unsigned)((z0 - 1 + r) * ldx + x0 - 3 + c));
+            hx[q] = ld(S.fvxo, (unsigned)((z0 - 1 + r) * ldx + x0 - 3 + c));
         }
     }
 #pragma unroll
@@ -415,7 +439,7 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
             const int r = i / 3, k = i - 3 * r;
             const int c = k == 0 ? 0 : TXT + k;
             hz_i[q] = r * VZW_ + c;
-            hz[q] = ld(S.vzo, (unsigned)((z0 - 3 + r) * ldx + x0 - 1 + c));
+            hz[q] = ld(S.fvzo, (unsigned)((z0 - 3 + r) * ldx + x0 - 1 + c));
         }
     }
     float pxr[PROWS], pzr[PROWS], kar[PROWS], spr[PROWS];
@@ -424,8 +448,8 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
         const int r = min(rz + NWZ * j, PH_ - 1);
         const int gz = z0 - 1 + r;
         const unsigned g = (unsigned)(gz * ldx + gx);
-        pxr[j] = ld(S.pxo, g);
-        pzr[j] = ld(S.pzo, g);
+        pxr[j] = ld(S.fpxo, g);
+        pzr[j] = ld(S.fpzo, g);
         kar[j] = ld(S.kappa, g);
         spr[j] = MODE == kReinject ? snap_load_w<SNAP>(S, S.sp, S.sp16, 0, ic, gz, gx, wseg) : 0.0f;
     }
@@ -439,8 +463,8 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
         s_c = k == 0 ? 0 : TXT + k;
         const int s_gz = z0 - 1 + s_r, s_gx = x0 - 1 + s_c;
         const unsigned g = (unsigned)(s_gz * ldx + s_gx);
-        s_px = ld(S.pxo, g);
-        s_pz = ld(S.pzo, g);
+        s_px = ld(S.fpxo, g);
+        s_pz = ld(S.fpzo, g);
         s_ka = ld(S.kappa, g);
         if (MODE == kReinject) s_sp = snap_load<SNAP>(S, S.sp, S.sp16, 0, ic, s_gz, s_gx);
     }
@@ -479,8 +503,8 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
         const float p = pxv + pzv;
         if (r >= 1 && r <= TZ) {
             const unsigned g = (unsigned)(gz * ldx + gx);
-            st(S.pxn, g, pxv);
-            st(S.pzn, g, pzv);
+            st(S.fpxn, g, pxv);
+            st(S.fpzn, g, pzv);
             if (MODE != kBack && S.sd->band_row != nullptr) {
                 const int slot = glb(S.sd->band_row)[gz];
                 if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = p;
@@ -519,8 +543,8 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
             vxv = vxv + mdt * svxr[j];
             vzv = vzv + mdt * svzr[j];
         }
-        st(S.vxn, g, vxv);
-        st(S.vzn, g, vzv);
+        st(S.fvxn, g, vxv);
+        st(S.fvzn, g, vzv);
         if (MODE == kBack) {
             snap_store_wave<SNAP>(S, S.svx, S.svx16, 1, it, gz, gx, vxv, true, true);
             snap_store_wave<SNAP>(S, S.svz, S.svz16, 2, it, gz, gx, vzv, true, true);
@@ -584,14 +608,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     const int lo = 2;
     const int hiz = (MODE == kFwd) ? n1e - 3 : n1e - 4;
     const int hix = (MODE == kFwd) ? n2e - 3 : n2e - 4;
-    const GLB float *__restrict__ vxo = S.vxo;
-    const GLB float *__restrict__ vzo = S.vzo;
-    const GLB float *__restrict__ pxo = S.pxo;
-    const GLB float *__restrict__ pzo = S.pzo;
-    GLB float *__restrict__ pxn = S.pxn;
-    GLB float *__restrict__ pzn = S.pzn;
-    GLB float *__restrict__ vxn = S.vxn;
-    GLB float *__restrict__ vzn = S.vzn;
+    const Fld vxo = S.fvxo, vzo = S.fvzo, pxo = S.fpxo, pzo = S.fpzo;
+    const Fld pxn = S.fpxn, pzn = S.fpzn, vxn = S.fvxn, vzn = S.fvzn;
     const int ic = S.nt - 1 - it;   // time-reversed snapshot index (icount = nt-it+1)
     const int cl = (wv % TXW) * 64 + lane;   // this lane's column in the tile
     const int rz = wv / TXW;                  // this wavefront's first row
@@ -735,7 +753,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
 #pragma unroll
     for (int q = 0; q < HZN; ++q)
         if (hz_i[q] >= 0) s_vz[hz_i[q]] = hz[q];
-    const float c_d2a = colok ? S.d2a[gxl] : 1.0f, c_d2a1 = colok ? S.d2a1[gxl] : 1.0f;
+    const float c_d2a = colok ? S.fd2a[gxl] : 1.0f, c_d2a1 = colok ? S.fd2a1[gxl] : 1.0f;
     __syncthreads();
 
     // ---- 2. pressure half-step
@@ -748,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
             if (gz >= 0 && gz < n1e && colok) {
                 const int injq = (MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj) ? glb(S.sd->inj_row)[gz] : -1;
                 p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + cl + 3],
-                                        &s_vz[(r + 2) * VZW_ + cl + 1], S.d1a[gz], S.d1a1[gz], c_d2a, c_d2a1,
+                                        &s_vz[(r + 2) * VZW_ + cl + 1], S.fd1a[gz], S.fd1a1[gz], c_d2a, c_d2a1,
                                         injq, r >= 1 && r <= TZ, lo, hiz, hix, spr[j], pxn, pzn);
             }
             s_p[r * PW_ + cl + 1] = p;
@@ -765,8 +783,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
         if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
             const int injq = (MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj) ? glb(S.sd->inj_row)[s_gz] : -1;
             p = p_point<MODE, VZW_>(S, it, ic, s_gz, s_gx, s_px, s_pz, s_ka, &s_vx[s_r * VXW_ + s_c + 2],
-                                    &s_vz[(s_r + 2) * VZW_ + s_c], S.d1a[s_gz], S.d1a1[s_gz], S.d2a[s_gx],
-                                    S.d2a1[s_gx], injq, false, lo, hiz, hix, s_sp, pxn, pzn);
+                                    &s_vz[(s_r + 2) * VZW_ + s_c], S.fd1a[s_gz], S.fd1a1[s_gz], S.fd2a[s_gx],
+                                    S.fd2a1[s_gx], injq, false, lo, hiz, hix, s_sp, pxn, pzn);
         }
         s_p[s_r * PW_ + s_c] = p;
     }
@@ -776,7 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
     // stay in the loop: the snapshot store reduces over whole wavefronts)
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
-    const float c_d2b = colok ? S.d2b[gxl] : 1.0f, c_bw2 = colok ? S.bw2[gxl] : 0.0f;
+    const float c_d2b = colok ? S.fd2b[gxl] : 1.0f, c_bw2 = colok ? S.fbw2[gxl] : 0.0f;
 #pragma unroll
     for (int j = 0; j < VROWS; ++j) {
         const int r = rz + NWZ * j;
@@ -795,10 +813,10 @@ __global__ __launch_bounds__(kBlock) void k_step(StepParams P)
                 const float der1 = a0 * (pp[PW_] - pp[0]) + a1 * (pp[2 * PW_] - pp[-PW_]);
                 if (MODE == kBack) {
                     vxv = c_d2b * vxv - c_bw2 * der2;
-                    vzv = S.d1b[gz] * vzv - S.bw1[gz] * der1;
+                    vzv = S.fd1b[gz] * vzv - S.fbw1[gz] * der1;
                 } else {
                     vxv = c_d2b * vxv + c_bw2 * der2;
-                    vzv = S.d1b[gz] * vzv + S.bw1[gz] * der1;
+                    vzv = S.fd1b[gz] * vzv + S.fbw1[gz] * der1;
                 }
             }
             if (MODE == kFwd && S.sd->vinj && S.sd->inj_row != nullptr) {
