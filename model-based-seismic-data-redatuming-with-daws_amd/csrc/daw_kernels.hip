@@ -48,6 +48,15 @@ struct Fld {
 __device__ __forceinline__ float ld(const Fld &f, unsigned idx) { return ld(f.base, f.off + idx); }
 __device__ __forceinline__ void st(const Fld &f, unsigned idx, float v) { st(f.base, f.off + idx, v); }
 
+// Non-temporal store, for the snapshot store: a snapshot is written once per
+// back-propagation and read thousands of steps later, so its writes should
+// not displace the wave state (read again at the next step) from the caches.
+template <class T>
+__device__ __forceinline__ void st_nt(GLB T *base, unsigned idx, T v)
+{
+    __builtin_nontemporal_store(v, (GLB T *)((GLB char *)base + idx * (unsigned)sizeof(T)));
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi)
 {
     return v < lo ? lo : (v > hi ? hi : v);
@@ -312,11 +321,11 @@ __device__ __forceinline__ void snap_store_wave(const View &S, GLB float *f32, G
                                                 int gz, int gx, float v, bool interior, bool row_interior)
 {
     if (F == kSnapF32) {
-        if (interior) st(splane_at(f32, S, t), snap_off(S, gz, gx), v);
+        if (interior) st_nt(splane_at(f32, S, t), snap_off(S, gz, gx), v);
         return;
     }
     const float sc = pow2_scale(wave_max_abs(interior ? v : 0.0f));
-    if (interior) st(splane_at(f16, S, t), snap_off(S, gz, gx), snap_encode<F>(v * (1.0f / sc)));
+    if (interior) st_nt(splane_at(f16, S, t), snap_off(S, gz, gx), snap_encode<F>(v * (1.0f / sc)));
     if (row_interior && (threadIdx.x & 63) == 0 && (gx >> 6) < S.nseg) st(scale_at(S, t, field), scale_off(S, gz, gx), sc);
 }
 
@@ -969,9 +978,9 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
                 const bool interior = gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2;
                 if (HESS) {
                     if (gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gx >= S.sd->xlo2 && gx <= S.sd->xhi2)
-                        st(splane_at(S.sp, S, ic), snap_off(S, gz, gx), -S.dt * kap * qpx);
+                        st_nt(splane_at(S.sp, S, ic), snap_off(S, gz, gx), -S.dt * kap * qpx);
                     else if (interior)
-                        st(splane_at(S.sp, S, ic), snap_off(S, gz, gx), 0.0f);
+                        st_nt(splane_at(S.sp, S, ic), snap_off(S, gz, gx), 0.0f);
                 } else if (S.sd->band_row != nullptr) {
                     const int slot = glb(S.sd->band_row)[gz];
                     if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = S.dt * kap * qpx;
@@ -1011,8 +1020,8 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
         S.vzn[g] = (box ? S.d1b[gz] * qvz : qvz) + az;
         if (HESS && gz >= npml && gz < npml + n1 && gx >= npml && gx < npml + n2) {
             const unsigned sidx = snap_off(S, gz, gx);
-            st(splane_at(S.svx, S, ic), sidx, -S.dt * qvx);
-            st(splane_at(S.svz, S, ic), sidx, -S.dt * qvz);
+            st_nt(splane_at(S.svx, S, ic), sidx, -S.dt * qvx);
+            st_nt(splane_at(S.svz, S, ic), sidx, -S.dt * qvz);
         }
     }
 }
