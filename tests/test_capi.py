@@ -75,3 +75,40 @@ def test_error_codes(daw):
     rc = L.daw_hicks2d_tile(0, 1, 0, 1, np.zeros(441, np.float32))
     assert rc == -1
     L.daw_deallocatekws2d()
+
+
+def _window_from_weights(w1, w2, ip, ofs):
+    """A 21x21 Hicks window [n2][n1] restated from two 1-D weight vectors
+    (kwsinc.f90:268-269, 336-355): w2*w1; with a free surface the node row ip
+    is zero and the rows ip+k (k = 1..ip) are w2*w1(ip+k) - w2*w1(ip-k); an
+    all-zero window for slices ip >= 1 without one."""
+    c = np.zeros((21, 21), np.float32)
+    if ip > 0 and abs(ofs) != 1:
+        return c
+    for n1 in range(21):
+        if ip > 0 and n1 == ip:
+            continue
+        if ip > 0 and ip < n1 <= 2 * ip:
+            c[:, n1] = (w2 * w1[n1]) - (w2 * w1[2 * ip - n1])
+        else:
+            c[:, n1] = w2 * w1[n1]
+    return c
+
+
+def test_window_is_tensor_product_of_1d_table(daw):
+    """Every 2-D window the library builds is the tensor product of two rows of
+    its 1-D table, with the free-surface slices' zero row and odd mirror, bit
+    for bit (an independent statement of kwsinc2d's construction)."""
+    t1 = daw.hicks1d()
+    rng = np.random.default_rng(11)
+    for ofs in (1, 0):
+        daw.kwsinc2d(ofs)
+        try:
+            for _ in range(60):
+                a1, a2 = (int(v) for v in rng.integers(1, 101, 2))
+                for ip in (0, int(rng.integers(1, 11)), 10):
+                    want = daw.hicks2d_tile(a1, a2, ip, ofs)
+                    got = _window_from_weights(t1[a1 - 1], t1[a2 - 1], ip, ofs)
+                    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (ofs, a1, a2, ip)
+        finally:
+            daw.deallocatekws2d()
