@@ -90,6 +90,19 @@ class Stats(C.Structure):
     ]
 
 
+def _gpu_runtime_first():
+    """PyTorch ships its own HIP runtime; libdaw.so links the system one.  Both
+    work in one process (the batched entry points take PyTorch's device
+    buffers) when PyTorch's runtime has opened the GPU first; the other order
+    leaves PyTorch without a device.  So open it before loading the library."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     """Load libdaw.so (raises DawError when it has not been built)."""
     global _lib
@@ -97,6 +110,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise DawError("libdaw.so is not built: run __graft_entry__.build() or "
                            "python model-based-seismic-data-redatuming-with-daws_amd/build.py")
+        _gpu_runtime_first()
         L = C.CDLL(LIB_PATH)
         L.daw_last_error.restype = C.c_char_p
         L.daw_hicks1d.argtypes = [_f32p]
