@@ -1375,7 +1375,7 @@ __global__ __launch_bounds__(kBlock) void k_step2(StepParams P)
 }
 
 // Tile shapes the autotuner may pick: (rows, wavefronts across x).
-#define DAW_TILE_SHAPES(X) X(8, 2) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(64, 1)
+#define DAW_TILE_SHAPES(X) X(8, 2) X(16, 1) X(16, 2) X(24, 1) X(32, 1) X(32, 2) X(64, 1)
 
 void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st)
 {

@@ -17,7 +17,7 @@ def test_tile_and_steps_knobs(daw):
         for rows, cols in [(12, 64), (16, 96), (8, 64), (64, 128), (-16, 64)]:
             assert L.daw_set_tile(rows, cols) == -1, (rows, cols)
             assert b"no compiled kernel" in L.daw_last_error()
-        for rows, cols in [(8, 128), (16, 64), (16, 128), (32, 64), (32, 128), (64, 64)]:
+        for rows, cols in [(8, 128), (16, 64), (16, 128), (24, 64), (32, 64), (32, 128), (64, 64)]:
             assert L.daw_set_tile(rows, cols) == 0, (rows, cols)
         assert L.daw_set_steps(3) == -1
         # 64x64 has no two-step kernel: fixing both must be refused either way round
