@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define DAW_ABI_VERSION 1
+#define DAW_ABI_VERSION 2
 
 #define DAW_OK 0
 #define DAW_ERR_ARG (-1)     /* invalid argument */
@@ -117,6 +117,21 @@ int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int 
                         int nrec, const int *ir, float mu, float maxiter, float error, const float *deltadr,
                         float *deltade, double *hist, double *info);
 
+/* sublinearminres with its log open (grid%oio = 1, sublinearminres.f90:51-61,
+ * 78-80): the same solve, and MINRES writes its heading, iteration lines and
+ * final status to log_path (MINRES_<shot>.txt in the reference) in the
+ * formats of minresModule.f90:346-358.  log_path NULL: no log. */
+int daw_sublinearminres_log(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                            int type_rec, int nrec, const int *ir, float mu, float maxiter, float error,
+                            const float *deltadr, float *deltade, const char *log_path);
+
+/* The MINRES log of one solve from a log block that the batched entry points
+ * return (daw_redatum_args.minres_log_host): 9 doubles (fac, Anorm, Acond,
+ * rnorm, Arnorm, ynorm, beta1, istop, itn), then 5 per iteration up to itnlim
+ * (x(1), qrnorm/beta1, Anorm, Acond, 1 if MINRES prints the line,
+ * minresModule.f90:308-325).  n is the system size, nrec*nt. */
+int daw_write_minres_log(const char *path, int n, int itnlim, float rtol, const double *block);
+
 /* filmartin2d(deltadeo,nr,nt,dt,flow,fhig,llow,lhig,deltade),
  * minresModule.f90:486-614. */
 int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
@@ -156,6 +171,11 @@ typedef struct daw_redatum_args {
     double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
     int filter;               /* 1: band-pass the gathers as mode 2 does (filmartin2d) */
     float flow, fhig, llow, lhig;
+    double *minres_log_host;  /* host out or NULL: MINRES log blocks [shot][9 + 5*maxiter]
+                                 (layout at daw_write_minres_log) */
+    int method;               /* 0 or 1: DAW (mode 1, :73-155); 3: Symes (mode 3, :157-228):
+                                 no background forward, the residual is dobs time-reversed,
+                                 datum and datae carry u^e / dataesc alone; s_dev may be NULL */
 } daw_redatum_args;
 
 int daw_redatum_batch(const daw_redatum_args *args);
@@ -194,6 +214,8 @@ typedef struct daw_uneven_args {
     double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
     int filter;
     float flow, fhig, llow, lhig;
+    double *minres_log_host;  /* host out or NULL: [shot][9 + 5*maxiter], as daw_redatum_args */
+    int method;               /* as daw_redatum_args */
 } daw_uneven_args;
 
 int daw_redatum_uneven(const daw_uneven_args *args);
@@ -221,6 +243,33 @@ typedef struct daw_modeling_args {
 } daw_modeling_args;
 
 int daw_modeling_batch(const daw_modeling_args *args);
+
+/* ---- the reference driver's files (host only, no GPU work) ------------- */
+
+/* redatum.par as subreadpar reads it (subreadpar.f90:9-21: one list-directed
+ * READ per line).  maxiter is REAL as in grid_type (main_redatum_DAW.f90:14);
+ * names are NUL-terminated. */
+typedef struct daw_par {
+    int mode;
+    char name_true[160], name_init[160], name_acqui[160], name_src[160];
+    float flow, fhig, llow, lhig;
+    int n1, n2, npml;
+    float h;
+    int nt;
+    float dt, dto;
+    int type_src, type_rec, ofs;
+    float mu;
+    int oio;
+    float error, maxiter;
+} daw_par;
+
+int daw_readpar(const char *path, daw_par *par);
+
+/* The acquisition file: "nsrc nrec" (main_redatum_DAW.f90:120-123), then one
+ * "z x" record per source and per receiver (subacqui.f90:29-57).  Counts go
+ * to *nsrc, *nrec; src_zx / rec_zx (NULL to skip) receive z,x pairs and must
+ * hold src_cap / rec_cap points.  daw_subacqui turns them into tables. */
+int daw_read_acqui(const char *path, int *nsrc, int *nrec, float *src_zx, int src_cap, float *rec_zx, int rec_cap);
 
 /* ---- measurement ------------------------------------------------------ */
 typedef struct daw_stats {

@@ -36,7 +36,7 @@ EXPORTS = [
     "daw_get_stats", "daw_modeling_batch", "daw_set_tile", "daw_get_tile", "daw_filmartin2d_dev",
     "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
-    "daw_get_launch",
+    "daw_get_launch", "daw_sublinearminres_log", "daw_write_minres_log", "daw_readpar", "daw_read_acqui",
 ]
 
 
@@ -55,6 +55,7 @@ class RedatumArgs(C.Structure):
         ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
         ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
         ("filter", C.c_int), ("flow", C.c_float), ("fhig", C.c_float), ("llow", C.c_float), ("lhig", C.c_float),
+        ("minres_log_host", C.c_void_p), ("method", C.c_int),
     ]
 
 
@@ -69,6 +70,7 @@ class UnevenArgs(C.Structure):
         ("datum_dev", C.c_void_p), ("datae_dev", C.c_void_p), ("deltade_dev", C.c_void_p),
         ("hist_host", C.c_void_p), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double),
         ("filter", C.c_int), ("flow", C.c_float), ("fhig", C.c_float), ("llow", C.c_float), ("lhig", C.c_float),
+        ("minres_log_host", C.c_void_p), ("method", C.c_int),
     ]
 
 
@@ -80,6 +82,18 @@ class ModelingArgs(C.Structure):
         ("s_dev", C.c_void_p), ("nshots", C.c_int), ("nsrc_total", C.c_int), ("src_tab", C.c_void_p),
         ("nrec", C.c_int), ("rec_tab", C.c_void_p), ("ndat", C.c_int), ("dat_tab", C.c_void_p),
         ("dobs_dev", C.c_void_p), ("datum_dev", C.c_void_p), ("max_batch", C.c_int),
+    ]
+
+
+class Par(C.Structure):
+    """daw_par: the fields of redatum.par (subreadpar.f90:9-21)."""
+    _fields_ = [
+        ("mode", C.c_int), ("name_true", C.c_char * 160), ("name_init", C.c_char * 160),
+        ("name_acqui", C.c_char * 160), ("name_src", C.c_char * 160),
+        ("flow", C.c_float), ("fhig", C.c_float), ("llow", C.c_float), ("lhig", C.c_float),
+        ("n1", C.c_int), ("n2", C.c_int), ("npml", C.c_int), ("h", C.c_float), ("nt", C.c_int),
+        ("dt", C.c_float), ("dto", C.c_float), ("type_src", C.c_int), ("type_rec", C.c_int), ("ofs", C.c_int),
+        ("mu", C.c_float), ("oio", C.c_int), ("error", C.c_float), ("maxiter", C.c_float),
     ]
 
 
@@ -136,6 +150,13 @@ def lib():
         L.daw_get_stats.argtypes = [C.POINTER(Stats)]
         L.daw_modeling_batch.argtypes = [C.POINTER(ModelingArgs)]
         L.daw_set_timing.argtypes = [C.c_int]
+        L.daw_sublinearminres_log.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                              C.c_float, C.c_int, C.c_int, _i32p, C.c_float, C.c_float, C.c_float,
+                                              _f32p, _f32p, C.c_char_p]
+        L.daw_write_minres_log.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_float, _f64p]
+        L.daw_readpar.argtypes = [C.c_char_p, C.POINTER(Par)]
+        L.daw_read_acqui.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int,
+                                     C.c_void_p, C.c_int]
         L.daw_submodeling_transpose.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                                 C.c_float, _f32p, _i32p, _i32p, C.c_int, _f32p]
         _lib = L
@@ -348,6 +369,49 @@ def sublinearminres(c0, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, maxiter, err
     return x, hist[:max(d["itn"], 0)], d
 
 
+def sublinearminres_log(c0, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, maxiter, error, log_path, type_rec=0):
+    """sublinearminres with grid%oio = 1 (sublinearminres.f90:51-61): the
+    solve, with MINRES's log written to log_path.  Returns deltade (nr, nt)."""
+    _ensure_hicks(ofs)
+    c0 = _f32(c0)
+    n2, n1 = c0.shape
+    rec_tab = _i32(rec_tab)
+    nr = rec_tab.shape[1]
+    x = np.zeros((nr, nt), np.float32)
+    _check(lib().daw_sublinearminres_log(c0, n1, n2, int(npml), float(h), int(ofs), int(nt), float(dt),
+                                         int(type_rec), nr, rec_tab, float(mu), float(maxiter), float(error),
+                                         _f32(deltadr), x, os.fsencode(log_path)))
+    return x
+
+
+def write_minres_log(path, n, itnlim, rtol, block):
+    """The MINRES log text of one solve from its log block (daw_write_minres_log)."""
+    _check(lib().daw_write_minres_log(os.fsencode(path), int(n), int(itnlim), float(rtol),
+                                      np.ascontiguousarray(block, dtype=np.float64)))
+
+
+def readpar(path="redatum.par"):
+    """subreadpar (subreadpar.f90:9-21): the parameter file as a dict."""
+    p = Par()
+    _check(lib().daw_readpar(os.fsencode(path), C.byref(p)))
+    d = {k: getattr(p, k) for k, _ in Par._fields_}
+    for k in ("name_true", "name_init", "name_acqui", "name_src"):
+        d[k] = d[k].decode()
+    return d
+
+
+def read_acqui(path):
+    """The acquisition file (main_redatum_DAW.f90:120-123, subacqui.f90:29-57):
+    source and receiver coordinates as (n, 2) float32 arrays of (z, x)."""
+    ns, nr = C.c_int(), C.c_int()
+    _check(lib().daw_read_acqui(os.fsencode(path), C.byref(ns), C.byref(nr), None, 0, None, 0))
+    src = np.zeros((ns.value, 2), np.float32)
+    rec = np.zeros((nr.value, 2), np.float32)
+    _check(lib().daw_read_acqui(os.fsencode(path), C.byref(ns), C.byref(nr), src.ctypes.data, ns.value,
+                                rec.ctypes.data, nr.value))
+    return src, rec
+
+
 def filmartin2d(din, dt, flow, fhig, llow, lhig):
     """filmartin2d (minresModule.f90:486-614) on traces (nr, nt)."""
     din = _f32(din)
@@ -361,7 +425,7 @@ def filmartin2d(din, dt, flow, fhig, llow, lhig):
 
 def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, dobs_dev, mu, maxiter, rtol=0.0,
                   datum_dev=None, datae_dev=None, deltade_dev=None, want_hist=True, max_batch=0,
-                  hbm_budget_bytes=0.0, type_src=0, band=None):
+                  hbm_budget_bytes=0.0, type_src=0, band=None, method=1, minres_log=None):
     """Mode-1 DAW plus the mode-2 datum gather of sub_redatum_DAW.f90:82-155,
     :238 for all shots of src_tab, advanced together on the GPU.
 
@@ -369,7 +433,9 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
     dobs_dev [nshots, nrec, nt]; datum_dev [nshots, ndat, nt] receives the
     redatumed gathers, band-passed as mode 2 does when band = (flow, fhig,
     llow, lhig) is given.  Returns the MINRES histories
-    (nshots, maxiter) when want_hist."""
+    (nshots, maxiter) when want_hist.  method=3 runs the Symes variant of
+    mode 3 (:157-228) instead of mode 1; minres_log, a float64 array of
+    (nshots, 9 + 5*maxiter), receives the MINRES log blocks."""
     import torch
     _ensure_hicks(ofs)
     c0 = _f32(c0)
@@ -385,15 +451,18 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
         if t is not None:
             if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
                 raise ValueError(f"device tensor must be contiguous float32 cuda of shape {shape}")
-    if s_dev.dtype != torch.float32 or not s_dev.is_cuda or s_dev.numel() != nt:
+    if s_dev is not None and (s_dev.dtype != torch.float32 or not s_dev.is_cuda or s_dev.numel() != nt):
         raise ValueError("s_dev must be a float32 cuda tensor of nt samples")
+    if minres_log is not None and (minres_log.dtype != np.float64 or not minres_log.flags.c_contiguous or
+                                   minres_log.shape != (nshots, 9 + 5 * max(int(maxiter), 1))):
+        raise ValueError("minres_log must be a C-contiguous float64 array of (nshots, 9 + 5*maxiter)")
     hist = np.zeros((nshots, max(int(maxiter), 1)), np.float64) if want_hist else None
     a = RedatumArgs()
     a.c0 = c0.ctypes.data
     a.n1, a.n2, a.npml = n1, n2, int(npml)
     a.h, a.dt = float(h), float(dt)
     a.ofs, a.nt, a.type_src = int(ofs), int(nt), int(type_src)
-    a.s_dev = s_dev.data_ptr()
+    a.s_dev = s_dev.data_ptr() if s_dev is not None else None
     a.nshots, a.nsrc_total, a.src_tab = nshots, nshots, src_tab.ctypes.data
     a.nrec, a.rec_tab = nrec, rec_tab.ctypes.data
     a.ndat, a.dat_tab = ndat, (dat_tab.ctypes.data if dat_tab is not None else None)
@@ -408,6 +477,8 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
     if band is not None:
         a.filter = 1
         a.flow, a.fhig, a.llow, a.lhig = (float(v) for v in band)
+    a.minres_log_host = minres_log.ctypes.data if minres_log is not None else None
+    a.method = int(method)
     torch.cuda.synchronize()
     _check(lib().daw_redatum_batch(C.byref(a)))
     return hist

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdaw.so")
 ARCH = os.environ.get("DAW_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["daw_host.cpp", "daw_kernels.hip", "daw_engine.hip"]
+SOURCES = ["daw_host.cpp", "daw_io.cpp", "daw_kernels.hip", "daw_engine.hip"]
 
 
 def hipcc():
@@ -54,6 +54,26 @@ def build(verbose=False):
     return LIB
 
 
+CLI = os.path.join(HERE, "dawredatum")
+
+
+def build_cli(verbose=False):
+    """dawredatum (csrc/dawredatum.cpp): the reference driver's modes over
+    libdaw.so, a host-only C++ program linked against the library and the HIP
+    runtime (it only allocates and copies device buffers)."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    tmp = CLI + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(rocm, 'include')}",
+           os.path.join(CSRC, "dawredatum.cpp"), "-L", HERE, "-ldaw", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(rocm, 'lib')}", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CLI)
+    return CLI
+
+
 def build_fortran(verbose=False):
     """The iso_c_binding drop-in (fortran/daw_shim.f90 -> fortran/_build/
     libdaw_fortran.a + .mod files) and the test driver that calls it the way
@@ -88,4 +108,5 @@ def build_fortran(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cli(verbose="-v" in sys.argv))
     print(build_fortran(verbose="-v" in sys.argv))

@@ -22,6 +22,7 @@
 #include "daw.h"
 #include "daw_device.h"
 #include "daw_host.h"
+#include "daw_io.h"
 #include "daw_kernels.h"
 
 namespace daw {
@@ -1010,14 +1011,27 @@ static int filter_device(const float *din, float *dout, int ntr, int nt, float d
 // ------------------------------------------------------------------ MINRES over a batch
 struct MinresWork {
     DevBuf vec;     // x, v, Y[3], Wv[3], hb: 9 vectors per shot
-    DevBuf off, len, part0, part1, scal, hist;
+    DevBuf off, len, part0, part1, scal, hist, trace;
 };
+
+// Per-shot MINRES log block handed to callers: kLogHead doubles (fac, Anorm,
+// Acond, rnorm, Arnorm, ynorm, beta1, istop, itn) and then kTraceStride per
+// iteration up to itnlim (x(1), qrnorm/beta1, Anorm, Acond, printed).
+constexpr int kLogHead = 9;
+static size_t log_block(int itnlim) { return kLogHead + (size_t)kTraceStride * std::max(itnlim, 1); }
+
+static void scal_info(const double *q, double *info)
+{
+    info[0] = q[kSFac]; info[1] = q[kSAnorm]; info[2] = q[kSAcond]; info[3] = q[kSRnorm];
+    info[4] = q[kSArnorm]; info[5] = q[kSYnorm]; info[6] = q[kSBeta1]; info[7] = q[kSIstop];
+    info[8] = q[kSItn];
+}
 
 // Solves (H + fac I) x = b per shot as sublinearminres/MINRES do
 // (sublinearminres.f90:64-76, minresModule.f90:83-329).  b and x are
 // device blocks [shot][nrec][nt].
 static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim, double rtol, double *hist_host,
-                        double *scal_host)
+                        double *scal_host, double *log_host = nullptr)
 {
     const int ns = B.nshots;
     const size_t blk = B.roff[ns];       // all shots' vectors, shot s at roff[s]
@@ -1040,6 +1054,12 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
     const size_t *doff = w.off.as<size_t>();
     const int *dlen = w.len.as<int>();
     launch_fill(w.hist.as<float>(), (size_t)ns * std::max(itnlim, 1) * 2, 0.0f, g.stream);
+    const size_t ntrace = (size_t)ns * std::max(itnlim, 1) * kTraceStride;
+    if (log_host) {
+        if (int rc = w.trace.alloc(ntrace * sizeof(double))) return rc;
+        launch_fill(w.trace.as<float>(), ntrace * 2, 0.0f, g.stream);
+    }
+    double *trace = log_host ? w.trace.as<double>() : nullptr;
 
     // fac from ||b|| / ||H b|| (sublinearminres.f90:64-72)
     HessOut o;
@@ -1075,9 +1095,10 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
         ir1 = ir2;
         ir2 = iy;
         iy = old_r1;
-        launch_minres_scalars(w.part1.as<double>(), w.scal.as<double>(), w.hist.as<double>(), itnlim, rtol, ns, g.stream);
+        launch_minres_scalars(w.part1.as<double>(), w.scal.as<double>(), w.hist.as<double>(), trace, dlen, itnlim, rtol,
+                              ns, g.stream);
         // w1 <- w2, w2 <- w, w <- new (written over old w1)
-        launch_minres_x(x, Wv[iw1], Wv[iw], Wv[iw2], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
+        launch_minres_x(x, Wv[iw1], Wv[iw], Wv[iw2], v, doff, dlen, w.scal.as<double>(), trace, itnlim, ns, g.stream);
         const int nw = iw1, nw2 = iw, nw1 = iw2;
         iw = nw; iw2 = nw2; iw1 = nw1;
     }
@@ -1085,6 +1106,17 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
     if (hist_host)
         HIPCHK(hipMemcpy(hist_host, w.hist.p, (size_t)ns * std::max(itnlim, 1) * sizeof(double), hipMemcpyDeviceToHost));
     if (scal_host) HIPCHK(hipMemcpy(scal_host, w.scal.p, (size_t)ns * nsc * sizeof(double), hipMemcpyDeviceToHost));
+    if (log_host) {
+        std::vector<double> tr(ntrace), sc((size_t)ns * nsc);
+        HIPCHK(hipMemcpy(tr.data(), w.trace.p, ntrace * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(sc.data(), w.scal.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
+        const size_t per = (size_t)std::max(itnlim, 1) * kTraceStride;
+        for (int s = 0; s < ns; ++s) {
+            double *blk = log_host + (size_t)s * log_block(itnlim);
+            scal_info(sc.data() + (size_t)s * nsc, blk);
+            std::memcpy(blk + kLogHead, tr.data() + (size_t)s * per, per * sizeof(double));
+        }
+    }
     return 0;
 }
 
@@ -1099,6 +1131,8 @@ struct RedatumIO {
     int maxiter = 0;
     float *datum = nullptr, *datae = nullptr, *deltade = nullptr;
     double *hist = nullptr;
+    double *log = nullptr;    // per-shot MINRES log blocks (host) or null
+    int method = 1;           // 1: DAW (mode 1); 3: Symes (mode 3)
     int filter = 0;
     float dt = 0.0f, flow = 0.0f, fhig = 0.0f, llow = 0.0f, lhig = 0.0f;
 };
@@ -1116,9 +1150,16 @@ static int redatum_run(Batch &B, const RedatumIO &io)
         if (int rc = d_datum_r.alloc(nd * 4)) return rc;
         if (int rc = d_datum_e.alloc(nd * 4)) return rc;
     }
-    // 1. background forward in c0: d^r and the datum gather of u^r (sub_redatum_DAW.f90:82-83)
-    if (int rc = forward(B, io.s_dev, io.type_src, 1, d_datr.as<float>(), nd ? d_datum_r.as<float>() : nullptr, nullptr))
+    // 1. background forward in c0: d^r and the datum gather of u^r (sub_redatum_DAW.f90:82-83).
+    // The Symes variant has no reduced wavefield: its residual is the observed
+    // data, time reversed (:175-179), and its extended wavefield is u^e alone (:219)
+    if (io.method == 3) {
+        launch_fill(d_datr.as<float>(), nv, 0.0f, g.stream);
+        if (nd) launch_fill(d_datum_r.as<float>(), nd, 0.0f, g.stream);
+    } else if (int rc = forward(B, io.s_dev, io.type_src, 1, d_datr.as<float>(), nd ? d_datum_r.as<float>() : nullptr,
+                                nullptr)) {
         return rc;
+    }
     // 2. reduced residual, time reversed (:103)
     if (B.geo.size() == 1) {
         launch_residuals(io.dobs, d_datr.as<float>(), d_res.as<float>(), B.geo[0]->nt,
@@ -1129,7 +1170,8 @@ static int redatum_run(Batch &B, const RedatumIO &io)
                              B.geo_of(s).nt, (size_t)B.geo_of(s).rec.n, g.stream);
     }
     // 3. MINRES on (H + fac I) de = dr (:115)
-    if (int rc = minres_batch(B, d_res.as<float>(), d_x.as<float>(), io.mu, io.maxiter, (double)io.rtol, io.hist, nullptr))
+    if (int rc = minres_batch(B, d_res.as<float>(), d_x.as<float>(), io.mu, io.maxiter, (double)io.rtol, io.hist, nullptr,
+                              io.log))
         return rc;
     // 4. source extension wavefield: dataesc and its datum gather (:131-132)
     HessOut o;
@@ -1348,11 +1390,10 @@ int daw_subwavefieldresample(int n1, int n2, int npml, int nt, float dt, const i
     return 0;
 }
 
-int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt, int type_rec,
-                        int nrec, const int *ir, float mu, float maxiter, float error, const float *deltadr,
-                        float *deltade, double *hist, double *info)
+static int sublinearminres(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt, int type_rec,
+                           int nrec, const int *ir, float mu, float maxiter, float error, const float *deltadr,
+                           float *deltade, double *hist, double *info, double *log)
 {
-    std::lock_guard<std::mutex> lk(g.mu);
     if (int rc = check_hicks()) return rc;
     if (int rc = check_grid(n1, n2, npml, nt, h, dt)) return rc;
     if (!c0 || nrec < 1 || !ir || !deltadr || !deltade) return fail(DAW_ERR_ARG, "sublinearminres: null argument");
@@ -1371,14 +1412,77 @@ int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int 
     HIPCHK(hipMemcpy(d_b.p, deltadr, n * 4, hipMemcpyHostToDevice));
     std::vector<double> scal(minres_scal_stride());
     std::vector<double> hh(std::max(itnlim, 1));
-    if (int rc = minres_batch(B, d_b.as<float>(), d_x.as<float>(), mu, itnlim, (double)error, hh.data(), scal.data()))
+    if (int rc = minres_batch(B, d_b.as<float>(), d_x.as<float>(), mu, itnlim, (double)error, hh.data(), scal.data(),
+                              log))
         return rc;
     HIPCHK(hipMemcpy(deltade, d_x.p, n * 4, hipMemcpyDeviceToHost));
     if (hist) std::memcpy(hist, hh.data(), (size_t)std::max(itnlim, 1) * sizeof(double));
-    if (info) {
-        info[0] = scal[kSFac]; info[1] = scal[kSAnorm]; info[2] = scal[kSAcond]; info[3] = scal[kSRnorm];
-        info[4] = scal[kSArnorm]; info[5] = scal[kSYnorm]; info[6] = scal[kSBeta1]; info[7] = scal[kSIstop];
-        info[8] = scal[kSItn];
+    if (info) scal_info(scal.data(), info);
+    return 0;
+}
+
+int daw_sublinearminres(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt, int type_rec,
+                        int nrec, const int *ir, float mu, float maxiter, float error, const float *deltadr,
+                        float *deltade, double *hist, double *info)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    return sublinearminres(c0, n1, n2, npml, h, ofs, nt, dt, type_rec, nrec, ir, mu, maxiter, error, deltadr, deltade,
+                           hist, info, nullptr);
+}
+
+static int write_text(const char *path, const std::string &text)
+{
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(DAW_ERR_ARG, "cannot open %s for writing", path);
+    const size_t w = std::fwrite(text.data(), 1, text.size(), f);
+    if (std::fclose(f) != 0 || w != text.size()) return fail(DAW_ERR_ARG, "cannot write %s", path);
+    return 0;
+}
+
+int daw_sublinearminres_log(const float *c0, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
+                            int type_rec, int nrec, const int *ir, float mu, float maxiter, float error,
+                            const float *deltadr, float *deltade, const char *log_path)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    const int itnlim = (int)maxiter;
+    std::vector<double> blk(log_block(itnlim));
+    if (int rc = sublinearminres(c0, n1, n2, npml, h, ofs, nt, dt, type_rec, nrec, ir, mu, maxiter, error, deltadr,
+                                 deltade, nullptr, nullptr, blk.data()))
+        return rc;
+    if (!log_path) return 0;
+    return write_text(log_path, minres_log_text(nrec * nt, itnlim, error, blk.data()));
+}
+
+int daw_write_minres_log(const char *path, int n, int itnlim, float rtol, const double *block)
+{
+    if (!path || !block || n < 0) return fail(DAW_ERR_ARG, "write_minres_log: bad argument");
+    return write_text(path, minres_log_text(n, itnlim, rtol, block));
+}
+
+int daw_readpar(const char *path, daw_par *par)
+{
+    if (!path || !par) return fail(DAW_ERR_ARG, "readpar: null argument");
+    std::string err;
+    if (read_par(path, par, err) != 0) return fail(DAW_ERR_ARG, "readpar: %s", err.c_str());
+    return 0;
+}
+
+int daw_read_acqui(const char *path, int *nsrc, int *nrec, float *src_zx, int src_cap, float *rec_zx, int rec_cap)
+{
+    if (!path || !nsrc || !nrec) return fail(DAW_ERR_ARG, "read_acqui: null argument");
+    std::string err;
+    std::vector<float> s, r;
+    int ns = 0, nr = 0;
+    if (read_acqui(path, ns, nr, s, r, err) != 0) return fail(DAW_ERR_ARG, "read_acqui: %s", err.c_str());
+    *nsrc = ns;
+    *nrec = nr;
+    if (src_zx) {
+        if (src_cap < ns) return fail(DAW_ERR_ARG, "read_acqui: %d sources, room for %d", ns, src_cap);
+        std::memcpy(src_zx, s.data(), s.size() * sizeof(float));
+    }
+    if (rec_zx) {
+        if (rec_cap < nr) return fail(DAW_ERR_ARG, "read_acqui: %d receivers, room for %d", nr, rec_cap);
+        std::memcpy(rec_zx, r.data(), r.size() * sizeof(float));
     }
     return 0;
 }
@@ -1399,8 +1503,11 @@ int daw_redatum_batch(const daw_redatum_args *a)
     if (!a) return fail(DAW_ERR_ARG, "null args");
     if (int rc = check_hicks()) return rc;
     if (int rc = check_grid(a->n1, a->n2, a->npml, a->nt, a->h, a->dt)) return rc;
-    if (!a->c0 || !a->s_dev || !a->src_tab || !a->rec_tab || a->nrec < 1 || a->nshots < 1 || !a->dobs_dev)
+    if (!a->c0 || (!a->s_dev && a->method != 3) || !a->src_tab || !a->rec_tab || a->nrec < 1 || a->nshots < 1 ||
+        !a->dobs_dev)
         return fail(DAW_ERR_ARG, "redatum_batch: missing input");
+    if (a->method != 0 && a->method != 1 && a->method != 3)
+        return fail(DAW_ERR_ARG, "redatum_batch: method %d (1 = DAW, 3 = Symes)", a->method);
     if (a->ndat > 0 && (!a->dat_tab || !a->datum_dev)) return fail(DAW_ERR_ARG, "redatum_batch: datum receivers need dat_tab and datum_dev");
     if (int rc = ensure_stream()) return rc;
     const Grid G = make_grid(a->n1, a->n2, a->npml, a->h, a->dt, a->ofs);
@@ -1451,6 +1558,8 @@ int daw_redatum_batch(const daw_redatum_args *a)
         io.datae = a->datae_dev ? a->datae_dev + (size_t)s0 * nvec : nullptr;
         io.deltade = a->deltade_dev ? a->deltade_dev + (size_t)s0 * nvec : nullptr;
         io.hist = a->hist_host ? a->hist_host + (size_t)s0 * std::max(a->maxiter, 1) : nullptr;
+        io.log = a->minres_log_host ? a->minres_log_host + (size_t)s0 * log_block(a->maxiter) : nullptr;
+        io.method = a->method == 3 ? 3 : 1;
         io.filter = a->filter; io.dt = a->dt; io.flow = a->flow; io.fhig = a->fhig; io.llow = a->llow; io.lhig = a->lhig;
         if (int rc = redatum_run(B, io)) return rc;
         HIPCHK(hipEventRecord(e1, g.stream));
@@ -1470,9 +1579,11 @@ int daw_redatum_uneven(const daw_uneven_args *a)
     std::lock_guard<std::mutex> lk(g.mu);
     if (!a) return fail(DAW_ERR_ARG, "null args");
     if (int rc = check_hicks()) return rc;
-    if (!a->c0 || !a->s_dev || !a->x0 || !a->n2s || !a->nts || !a->src_tab || !a->nrecs || !a->rec_tabs ||
-        !a->dobs_dev || a->nshots < 1)
+    if (!a->c0 || (!a->s_dev && a->method != 3) || !a->x0 || !a->n2s || !a->nts || !a->src_tab || !a->nrecs ||
+        !a->rec_tabs || !a->dobs_dev || a->nshots < 1)
         return fail(DAW_ERR_ARG, "redatum_uneven: missing input");
+    if (a->method != 0 && a->method != 1 && a->method != 3)
+        return fail(DAW_ERR_ARG, "redatum_uneven: method %d (1 = DAW, 3 = Symes)", a->method);
     const int ns_all = a->nshots;
     const bool dat = a->ndats != nullptr && a->dat_tabs != nullptr && a->datum_dev != nullptr;
     // per shot: subdomain, tables and trace offsets in the caller's blocks
@@ -1549,6 +1660,8 @@ int daw_redatum_uneven(const daw_uneven_args *a)
         io.datae = a->datae_dev ? a->datae_dev + roff[s0] : nullptr;
         io.deltade = a->deltade_dev ? a->deltade_dev + roff[s0] : nullptr;
         io.hist = a->hist_host ? a->hist_host + (size_t)s0 * std::max(a->maxiter, 1) : nullptr;
+        io.log = a->minres_log_host ? a->minres_log_host + (size_t)s0 * log_block(a->maxiter) : nullptr;
+        io.method = a->method == 3 ? 3 : 1;
         io.filter = a->filter; io.dt = a->dt; io.flow = a->flow; io.fhig = a->fhig; io.llow = a->llow; io.lhig = a->lhig;
         if (int rc = redatum_run(B, io)) return rc;
         HIPCHK(hipEventRecord(e1, g.stream));

@@ -42,10 +42,14 @@ void launch_minres_l1(float *y, const float *v, const float *r1, const size_t *o
                       const double *scal, double *part, int nshots, hipStream_t st);
 void launch_minres_l2(float *y, const float *r2, const size_t *off, const int *len, const double *scal,
                       double *part, int nshots, hipStream_t st);
-void launch_minres_scalars(const double *part, double *scal, double *hist, int itnlim, double rtol, int nshots,
-                           hipStream_t st);
+// trace (may be null): kTraceStride doubles per shot and iteration,
+// [shot][itn-1] = x(1), qrnorm/beta1, Anorm, Acond, printed (1/0) -- the
+// columns of the MINRES log line of that iteration and whether it is written
+void launch_minres_scalars(const double *part, double *scal, double *hist, double *trace, const int *len, int itnlim,
+                           double rtol, int nshots, hipStream_t st);
 void launch_minres_x(float *x, float *wn, const float *w2, const float *w1, const float *v, const size_t *off,
-                     const int *len, double *scal, int nshots, hipStream_t st);
+                     const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st);
+constexpr int kTraceStride = 5;
 
 // indices into a shot's scalar row (see k_minres_scalars)
 enum MinresScalIndex {

@@ -1711,8 +1711,8 @@ __global__ void k_minres_l2(float *y, const float *r2, const size_t *off, const 
 }
 
 // Scalar part of one MINRES iteration (:196-303) for every shot.
-__global__ void k_minres_scalars(const double *part, double *scal, double *hist, int itnlim, double rtol,
-                                 int nshots)
+__global__ void k_minres_scalars(const double *part, double *scal, double *hist, double *trace, const int *len,
+                                 int itnlim, double rtol, int nshots)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nshots) return;
@@ -1771,6 +1771,20 @@ __global__ void k_minres_scalars(const double *part, double *scal, double *hist,
         if (itn >= itnlim) istop = 5;
         if (phibar / q[kBeta1] <= rtol) istop = 1;
     }
+    if (trace) {
+        // the values the log line of this iteration prints and whether it is
+        // printed (:270-285, :308-325); x(1) follows from k_minres_x
+        const double an = q[kAnorm], epsx = an * q[kYnorm] * eps, epsr = an * q[kYnorm] * rtol;
+        const double rel = sqrt(gbar * gbar + dbar2 * dbar2) / an;
+        const bool prnt = len[s] <= 40 || itn <= 10 || itn >= itnlim - 10 || itn % 10 == 0 ||
+                          phibar <= 10.0 * epsx || phibar <= 10.0 * epsr || rel <= 10.0 * epsx ||
+                          rel <= 10.0 * epsr || q[kAcond] >= 1.0e-2 / eps || istop != 0;
+        double *t = trace + ((size_t)s * itnlim + (itn - 1)) * kTraceStride;
+        t[1] = phibar / q[kBeta1];
+        t[2] = an;
+        t[3] = q[kAcond];
+        t[4] = prnt ? 1.0 : 0.0;
+    }
     q[kItn] = itn;
     // the x update of this iteration still runs (:250-257 precede the exit);
     // k_minres_close applies the stop afterwards
@@ -1780,7 +1794,8 @@ __global__ void k_minres_scalars(const double *part, double *scal, double *hist,
 // w1 <- w2, w2 <- w, w <- (v - oldeps*w1 - delta*w2) * denom, x += phi*w (:252-257).
 // Buffer roles rotate on the host; here w_new is written over w1_old.
 __global__ void k_minres_x(float *x, float *w_new_over_w1, const float *w2_is_old_w, const float *w1_is_old_w2,
-                           const float *v, const size_t *off, const int *len, double *scal)
+                           const float *v, const size_t *off, const int *len, double *scal, double *trace,
+                           int itnlim)
 {
     const int s = blockIdx.y;
     double *q = scal + (size_t)s * kScal;
@@ -1792,7 +1807,9 @@ __global__ void k_minres_x(float *x, float *w_new_over_w1, const float *w2_is_ol
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock) {
         const float wi = (float)(((double)vs[i] - oldeps * (double)w1[i] - delta * (double)w2[i]) * denom);
         wn[i] = wi;
-        xs[i] = (float)((double)xs[i] + phi * (double)wi);
+        const float xi = (float)((double)xs[i] + phi * (double)wi);
+        xs[i] = xi;
+        if (trace && i == 0) trace[((size_t)s * itnlim + ((int)q[kItn] - 1)) * kTraceStride] = (double)xi;
     }
 }
 
@@ -1831,17 +1848,18 @@ void launch_minres_l2(float *y, const float *r2, const size_t *off, const int *l
     hipLaunchKernelGGL(k_minres_l2, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, y, r2, off, len, scal, part);
 }
 
-void launch_minres_scalars(const double *part, double *scal, double *hist, int itnlim, double rtol, int nshots,
-                           hipStream_t st)
+void launch_minres_scalars(const double *part, double *scal, double *hist, double *trace, const int *len, int itnlim,
+                           double rtol, int nshots, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_minres_scalars, dim3((nshots + 63) / 64), dim3(64), 0, st, part, scal, hist, itnlim, rtol,
-                       nshots);
+    hipLaunchKernelGGL(k_minres_scalars, dim3((nshots + 63) / 64), dim3(64), 0, st, part, scal, hist, trace, len,
+                       itnlim, rtol, nshots);
 }
 
 void launch_minres_x(float *x, float *wn, const float *w2, const float *w1, const float *v, const size_t *off,
-                     const int *len, double *scal, int nshots, hipStream_t st)
+                     const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_minres_x, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, x, wn, w2, w1, v, off, len, scal);
+    hipLaunchKernelGGL(k_minres_x, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, x, wn, w2, w1, v, off, len, scal,
+                       trace, itnlim);
     hipLaunchKernelGGL(k_minres_close, dim3((nshots + 63) / 64), dim3(64), 0, st, scal, nshots);
 }
 

@@ -477,6 +477,13 @@ static int SFX(blurring)(const float *vp, int n1, int n2, int npml, float h, int
  * log prints, :323).  out[0..6] = fac, Anorm, Acond, rnorm, Arnorm, ynorm,
  * beta1; iout[0..1] = istop, itn.
  */
+/* Optional per-iteration record of what MINRES's log shows (:270-285,
+ * :308-325): trace[5*(itn-1) + 0..4] = x(1), qrnorm/beta1, Anorm, Acond and 1
+ * if the line is printed.  The print rule uses the reference's REAL epsilon
+ * in both precisions.  NULL (the default) records nothing. */
+static R *SFX(minres_trace);
+void SFX(oracle_set_minres_trace)(R *trace) { SFX(minres_trace) = trace; }
+
 int SFX(oracle_linearminres)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt,
                              int type_rec, int nr, const int *ir, float mu, int itnlim, float rtol,
                              const R *deltadr, R *x, R *hist, R *out, int *iout)
@@ -566,6 +573,15 @@ int SFX(oracle_linearminres)(const float *vp, int n1, int n2, int npml, float h,
             if (istop == 0) {
                 if (itn >= itnlim) istop = 5;
                 if (qrnorm / beta1 <= (R)rtol) istop = 1;
+            }
+            if (SFX(minres_trace) && itn <= itnlim) {
+                const R feps = (R)1.1920928955078125e-07;
+                R epsx = Anorm * ynorm * feps, epsr = Anorm * ynorm * (R)rtol, rel = rootl / Anorm;
+                int prnt = n <= 40 || itn <= 10 || itn >= itnlim - 10 || itn % 10 == 0 ||
+                           qrnorm <= 10 * epsx || qrnorm <= 10 * epsr || rel <= 10 * epsx || rel <= 10 * epsr ||
+                           Acond >= (R)1.0e-2 / feps || istop != 0;
+                R *t = SFX(minres_trace) + (size_t)5 * (itn - 1);
+                t[0] = x[0]; t[1] = qrnorm / beta1; t[2] = Anorm; t[3] = Acond; t[4] = (R)prnt;
             }
             if (istop != 0) break;
         }

@@ -84,6 +84,7 @@ def lib():
             f.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float,
                           C.c_int, C.c_int, _i32p, C.c_float, C.c_int, C.c_float, rp, rp, rp, rp, _i32p]
             f.restype = C.c_int
+            getattr(L, "oracle_set_minres_trace_" + sfx).argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -238,8 +239,11 @@ def resample(wavefield, npml, rec_tab, ofs, prec="f32"):
     return sis
 
 
-def linearminres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, itnlim, rtol=0.0, type_rec=0, prec="f32"):
-    """sublinearminres: returns (deltade (nr,nt), hist (itn,), info dict)."""
+def linearminres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, itnlim, rtol=0.0, type_rec=0, prec="f32",
+                 want_trace=False):
+    """sublinearminres: returns (deltade (nr,nt), hist (itn,), info dict), and
+    with want_trace also the log trace (itn, 5): x(1), qrnorm/beta1, Anorm,
+    Acond, printed (minresModule.f90:308-325)."""
     vp = _c(vp, np.float32)
     n2, n1 = vp.shape
     rec_tab = _c(rec_tab, np.int32)
@@ -249,12 +253,20 @@ def linearminres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, itnlim, rtol=0.
     hist = np.zeros(max(int(itnlim), 1), _rdt(prec))
     out = np.zeros(7, _rdt(prec))
     iout = np.zeros(2, np.int32)
-    rc = getattr(lib(), "oracle_linearminres_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, type_rec, nr, rec_tab,
-                                                        mu, int(itnlim), rtol, b, x, hist, out, iout)
+    trace = np.zeros((max(int(itnlim), 1), 5), _rdt(prec)) if want_trace else None
+    setter = getattr(lib(), "oracle_set_minres_trace_" + prec)
+    setter(trace.ctypes.data if want_trace else None)
+    try:
+        rc = getattr(lib(), "oracle_linearminres_" + prec)(vp, n1, n2, npml, h, ofs, nt, dt, type_rec, nr, rec_tab,
+                                                            mu, int(itnlim), rtol, b, x, hist, out, iout)
+    finally:
+        setter(None)
     if rc != 0:
         raise RuntimeError("oracle linearminres failed")
     info = dict(fac=out[0], Anorm=out[1], Acond=out[2], rnorm=out[3], Arnorm=out[4], ynorm=out[5],
                 beta1=out[6], istop=int(iout[0]), itn=int(iout[1]))
+    if want_trace:
+        return x, hist[:info["itn"]], info, trace[:info["itn"]]
     return x, hist[:info["itn"]], info
 
 

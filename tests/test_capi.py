@@ -25,7 +25,7 @@ def test_exports_every_declared_symbol(daw):
     for s in syms:
         assert hasattr(L, s), s
     assert set(syms) == set(daw.EXPORTS)
-    assert L.daw_version() == 1
+    assert L.daw_version() == 2
 
 
 def test_host_geometry_bitexact(daw):

@@ -1,0 +1,257 @@
+"""The reference driver's files on the host (CPU, no GPU work):
+
+- redatum.par and the acquisition file as the library reads them
+  (daw_readpar, daw_read_acqui) against a Fortran runtime reading the same
+  files with list-directed READs (tests/fortran/list_input.f90, built with
+  flang), including the list-directed forms a hand-edited file may use;
+- the MINRES log text (daw_write_minres_log) against the same log written by
+  Fortran formatted output (tests/fortran/minres_log_fmt.f90);
+- `dawredatum --dry-run`: the src/rec/datum tables it builds from those files
+  against the oracle's subacqui.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import Case
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CLI = os.path.join(ROOT, "model-based-seismic-data-redatuming-with-daws_amd", "dawredatum")
+
+
+def _flang():
+    for c in (os.environ.get("FC"), shutil.which("flang"), "/opt/rocm/lib/llvm/bin/flang"):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+@pytest.fixture(scope="module")
+def fortran(tmp_path_factory):
+    fc = _flang()
+    if fc is None:
+        pytest.skip("no Fortran compiler (flang) to build the comparison programs")
+    d = tmp_path_factory.mktemp("fortran")
+    out = {}
+    for name in ("list_input", "minres_log_fmt"):
+        exe = str(d / name)
+        subprocess.run([fc, "-O0", os.path.join(HERE, "fortran", name + ".f90"), "-o", exe], check=True,
+                       cwd=str(d), capture_output=True)
+        out[name] = exe
+    return out
+
+
+def _fortran_values(exe, what, path):
+    r = subprocess.run([exe, what, path], capture_output=True, text=True, check=True)
+    vals = {}
+    for line in r.stdout.splitlines():
+        k, _, v = line.partition(" ")
+        if k == "zx":
+            vals.setdefault("zx", []).append([np.float32(t) for t in v.split()])
+        else:
+            vals[k] = v.strip()
+    return vals
+
+
+PAR_OK = """1                                               !modeling
+vtrue.bin vinit.bin                             !velocity model
+acqui_org_datum.txt                             !acquisition
+ricker_bl5.bin                                  !wavelet file
+3 25 1 5                                        !flow fhig llow lhig
+176 851 20 20 4801 0.002 0.002                  !n1 n2 npml h nt dt dto
+0 0                                             !source/receiver type
+1                                               !ofs
+1.0e-2                                          !penalty
+1                                               !oio
+0.001 500                                       !backward error,maximum iteration number for MINRES
+"""
+
+# other list-directed spellings of the same kind of file: commas, D
+# exponents, quoted names, a READ continued on the next record, repeat counts
+PAR_FORMS = """3,
+'v true.bin', "v0.bin"
+acq.txt, ignored
+w.bin
+2.5d0, 30.
+1.5 ,6
+121 401, 20 10.0 2500 1.0E-3 2.D-3
+2*0
+-1
+5.0e-4
+0
+1.0-4 30
+"""
+
+# the demo's files put two values where subreadpar reads four: the READ fails
+PAR_SHORT = PAR_OK.replace("3 25 1 5      ", "4 3           ")
+
+
+@pytest.mark.parametrize("text", [PAR_OK, PAR_FORMS], ids=["plain", "forms"])
+def test_readpar_matches_fortran_list_input(daw, fortran, tmp_path, text):
+    path = tmp_path / "redatum.par"
+    path.write_text(text)
+    ref = _fortran_values(fortran["list_input"], "par", str(path))
+    assert "ERROR" not in ref, ref
+    got = daw.readpar(str(path))
+    for k, v in ref.items():
+        if k.startswith("name_"):
+            assert got[k] == v, k
+        elif k in ("mode", "n1", "n2", "npml", "nt", "type_src", "type_rec", "ofs", "oio"):
+            assert got[k] == int(v), k
+        else:
+            assert np.float32(got[k]) == np.float32(v), (k, got[k], v)
+
+
+def test_readpar_rejects_what_fortran_rejects(daw, fortran, tmp_path):
+    path = tmp_path / "redatum.par"
+    path.write_text(PAR_SHORT)
+    ref = _fortran_values(fortran["list_input"], "par", str(path))
+    assert "ERROR" in ref and "flow" not in ref
+    with pytest.raises(daw.DawError, match="flow fhig llow lhig"):
+        daw.readpar(str(path))
+
+
+def test_readpar_slash_keeps_remaining_items(daw, tmp_path):
+    path = tmp_path / "redatum.par"
+    path.write_text(PAR_OK.replace("3 25 1 5      ", "3 25 /        "))
+    got = daw.readpar(str(path))
+    assert (got["flow"], got["fhig"], got["llow"], got["lhig"]) == (3.0, 25.0, 0.0, 0.0)
+    assert got["n1"] == 176 and got["maxiter"] == 500.0
+
+
+def test_readpar_missing_file(daw, tmp_path):
+    with pytest.raises(daw.DawError, match="cannot open"):
+        daw.readpar(str(tmp_path / "nope.par"))
+
+
+ACQ_TEXTS = {
+    "plain": "2 3\n10. 200.\n10. 500.\n15. 100.\n15. 300.\n15. 600.\n",
+    "icode": "1 2 extra\n450. 100. 0\n50. 0. 1\n50.,50., 1\n",
+    "split": "1 2\n\n450.\n100.\n50. 0.\n  5.0d1  1.0E2\n",
+}
+
+
+@pytest.mark.parametrize("name", sorted(ACQ_TEXTS))
+def test_read_acqui_matches_fortran_list_input(daw, fortran, tmp_path, name):
+    path = tmp_path / "acq.txt"
+    path.write_text(ACQ_TEXTS[name])
+    ref = _fortran_values(fortran["list_input"], "acqui", str(path))
+    assert "ERROR" not in ref, ref
+    ns, nr = (int(t) for t in ref["counts"].split())
+    src, rec = daw.read_acqui(str(path))
+    assert src.shape == (ns, 2) and rec.shape == (nr, 2)
+    zx = np.array(ref["zx"], np.float32)
+    assert np.array_equal(np.concatenate([src, rec]), zx)
+
+
+def test_read_acqui_short_file(daw, tmp_path):
+    path = tmp_path / "acq.txt"
+    path.write_text("2 2\n10. 20.\n")
+    with pytest.raises(daw.DawError, match="missing source"):
+        daw.read_acqui(str(path))
+
+
+def _log_cases():
+    rng = np.random.default_rng(7)
+    cases = []
+    for istop, itn, itnlim in [(1, 3, 500), (5, 25, 25), (0, 0, 30), (-1, 1, 10), (6, 12, 40), (2, 41, 60)]:
+        rows = []
+        for k in range(1, itn + 1):
+            x1 = np.float32(rng.standard_normal() * 10.0 ** rng.integers(-30, 30))
+            qr = np.float32(10.0 ** -rng.uniform(0, 8))
+            an = np.float32(10.0 ** rng.uniform(-3, 30))
+            ac = np.float32(10.0 ** rng.uniform(0, 12))
+            pr = int(k <= 10 or k % 10 == 0 or k == itn or rng.random() < 0.3)
+            rows.append((k, x1, qr, an, ac, pr))
+        head = [np.float32(v) for v in (10.0 ** rng.uniform(-8, 2), rng.uniform(1, 1e6), rng.uniform(1, 1e3),
+                                         rng.uniform(0, 1), rng.uniform(0, 1e4), rng.uniform(0, 1e-3))]
+        cases.append((istop, itn, itnlim, np.float32(10.0 ** -rng.uniform(1, 6)), head, rows))
+    return cases
+
+
+@pytest.mark.parametrize("case", _log_cases(), ids=lambda c: f"istop{c[0]}_itn{c[1]}")
+def test_minres_log_matches_fortran_formats(daw, fortran, tmp_path, case):
+    istop, itn, itnlim, rtol, (fac, anorm, acond, rnorm, arnorm, ynorm), rows = case
+    n = 401 * 3000
+    lines = [f"{n} {itnlim} {rtol:.9e} {-fac:.9e}",
+             f"{anorm:.9e} {acond:.9e} {rnorm:.9e} {ynorm:.9e} {arnorm:.9e}",
+             f"{istop} {itn}"]
+    lines += [f"{k} {x1:.9e} {qr:.9e} {an:.9e} {ac:.9e} {pr}" for k, x1, qr, an, ac, pr in rows]
+    ref_path = tmp_path / "ref.txt"
+    subprocess.run([fortran["minres_log_fmt"], str(ref_path)], input="\n".join(lines) + "\n", text=True,
+                   check=True, capture_output=True)
+    blk = np.zeros(9 + 5 * itnlim)
+    blk[:9] = [fac, anorm, acond, rnorm, arnorm, ynorm, 1.0, istop, itn]
+    for k, x1, qr, an, ac, pr in rows:
+        blk[9 + 5 * (k - 1): 14 + 5 * (k - 1)] = [x1, qr, an, ac, pr]
+    got_path = tmp_path / "got.txt"
+    daw.write_minres_log(str(got_path), n, itnlim, rtol, blk)
+    assert got_path.read_text() == ref_path.read_text()
+
+
+def test_cli_dry_run_tables(tmp_path):
+    """dawredatum reads the par file, models, wavelet and both acquisition
+    files and builds the tables subacqui builds (checked against the oracle)."""
+    assert os.path.exists(CLI), "dawredatum is not built"
+    n1, n2, nt, h, npml, ofs = 40, 90, 300, 10.0, 10, 1
+    np.full((n2, n1), 2000, np.float32).tofile(tmp_path / "vtrue.bin")
+    np.full((n2, n1), 1900, np.float32).tofile(tmp_path / "vinit.bin")
+    np.zeros(nt, np.float32).tofile(tmp_path / "w.bin")
+    src = np.array([[10.0, 200.0], [12.5, 503.7]], np.float32)
+    rec = np.array([[15.0, 100.0], [15.0, 301.2], [3.3, 600.0]], np.float32)
+    dat = np.array([[200.0, 300.0], [205.5, 400.0]], np.float32)
+    with open(tmp_path / "acq.txt", "w") as f:
+        f.write(f"{len(src)} {len(rec)}\n" + "".join(f"{z!r} {x!r}\n" for z, x in np.concatenate([src, rec]).tolist()))
+    with open(tmp_path / "dat.txt", "w") as f:
+        f.write(f"0 {len(dat)}\n" + "".join(f"{z!r} {x!r}\n" for z, x in dat.tolist()))
+    (tmp_path / "redatum.par").write_text(
+        f"1\nvtrue.bin vinit.bin\nacq.txt\nw.bin\n3 25 1 5\n{n1} {n2} {npml} {h} {nt} 0.001 0.002\n0 0\n{ofs}\n"
+        "1.0e-2\n1\n0.0 5\n")
+    r = subprocess.run([CLI, "--dry-run", "--datum", "dat.txt"], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    tabs, cur = {}, None
+    for line in r.stdout.splitlines():
+        t = line.split()
+        if t and t[0] in ("src_tab", "rec_tab", "dat_tab"):
+            cur = t[0]
+            tabs[cur] = []
+        elif cur and len(t) == 5:
+            tabs[cur].append([int(v) for v in t])
+    for name, pts in (("src_tab", src), ("rec_tab", rec), ("dat_tab", dat)):
+        ref = O.acqui_table(pts[:, 0], pts[:, 1], h, npml, ofs)
+        assert np.array_equal(np.array(tabs[name], np.int32).T, ref), name
+    assert "Niquist samping number 151" in r.stdout
+
+
+def test_cli_rejects_bad_inputs(tmp_path):
+    assert os.path.exists(CLI), "dawredatum is not built"
+    r = subprocess.run([CLI, "--dry-run"], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode != 0 and "redatum.par" in r.stderr
+    (tmp_path / "redatum.par").write_text(PAR_OK)
+    r = subprocess.run([CLI, "--dry-run"], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode != 0 and "ricker_bl5.bin" in r.stderr
+
+
+def test_oracle_minres_trace_is_the_solve_record():
+    """The oracle's MINRES log record agrees with the values its solve
+    returns: the residual column is the history, the last row carries the
+    final Anorm/Acond and x(1) of the solution, and lines 1-10, every tenth
+    and the last are printed (minresModule.f90:310-319)."""
+    c = Case(nt=120)
+    res = c.rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
+    for prec in ("f32", "f64"):
+        x, hist, info, tr = O.linearminres(c.vp0, c.npml, c.h, c.ofs, c.nt, c.dt, res, c.rec, 1e-2, 23, 0.0,
+                                           prec=prec, want_trace=True)
+        assert tr.shape == (info["itn"], 5) and info["itn"] == 23
+        assert np.array_equal(tr[:, 1], hist)
+        assert tr[-1, 0] == x[0, 0] and tr[-1, 2] == info["Anorm"] and tr[-1, 3] == info["Acond"]
+        must = [k for k in range(1, 24) if k <= 10 or k % 10 == 0 or k >= 23 - 10]
+        assert all(tr[k - 1, 4] == 1 for k in must)
+        # without a trace the solve is unchanged
+        x2, hist2, info2 = O.linearminres(c.vp0, c.npml, c.h, c.ofs, c.nt, c.dt, res, c.rec, 1e-2, 23, 0.0, prec=prec)
+        assert np.array_equal(x, x2) and np.array_equal(hist, hist2)
