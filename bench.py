@@ -286,6 +286,88 @@ def calibrate_copy(nbytes=1 << 30):
 _JSON_OUT = None
 
 
+def main_workflow(args, syn):
+    """The reference's workflow through dawredatum, files in and out: mode 0
+    (observed data, batched), then modes 1+2 fused by --datum over all shots,
+    and the plain route (mode 1 with oio = 1 writing the wavefields, then
+    mode 2 reading them back) on a bounded sample of shots.  Wall-clock time
+    of the CLI processes, including process start, file reads and writes."""
+    import shutil
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "model-based-seismic-data-redatuming-with-daws_amd", "dawredatum")
+    if not os.path.exists(cli):
+        raise SystemExit("dawredatum is not built: run __graft_entry__.build()")
+    w = syn.workload(args.config)
+    if args.nt:
+        w.nt = args.nt
+    if args.maxiter is not None:
+        w.maxiter = args.maxiter
+    nsh = w.nshots if args.shots is None else args.shots
+    kp = max(0, min(args.workflow_plain_shots, nsh))
+    vp, c0 = syn.models(w)
+    raw = syn.ricker(w.nt, w.dt, w.f0, 1.0)          # the wavelet file holds it before the division by h**2
+    work = tempfile.mkdtemp(prefix="dawredatum_", dir=os.environ.get("TMPDIR", "/tmp"))
+
+    def acq(name, ns, rz, rx):
+        pts = [(w.src_z[i], w.src_x[i]) for i in range(ns)] + list(zip(rz, rx))
+        with open(os.path.join(work, name), "w") as f:
+            f.write(f"{ns} {len(rz)}\n" + "".join(f"{float(z)!r} {float(x)!r}\n" for z, x in pts))
+
+    def run(mode, acq_name, oio, *extra):
+        b = w.band
+        with open(os.path.join(work, "redatum.par"), "w") as f:
+            f.write(f"{mode}\nvtrue.bin vinit.bin\n{acq_name}\nwavelet.bin\n{b[0]} {b[1]} {b[2]} {b[3]}\n"
+                    f"{w.nz} {w.nx} {w.npml} {w.h!r} {w.nt} {w.dt!r} {w.dt!r}\n0 0\n{w.ofs}\n{w.mu!r}\n{oio}\n"
+                    f"0.0 {w.maxiter}\n")
+        _log(f"dawredatum mode {mode} oio {oio} {' '.join(extra)}")
+        t0 = time.perf_counter()
+        r = subprocess.run([cli, "--quiet", *extra], cwd=work, capture_output=True, text=True)
+        el = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise SystemExit(f"dawredatum mode {mode} failed: {r.stderr[-2000:]}")
+        _log(f"  {el:.2f} s")
+        return el
+
+    try:
+        vp.tofile(os.path.join(work, "vtrue.bin"))
+        c0.tofile(os.path.join(work, "vinit.bin"))
+        raw.tofile(os.path.join(work, "wavelet.bin"))
+        acq("acq_org.txt", nsh, w.rec_z, w.rec_x)
+        acq("acq_new.txt", nsh, w.dat_z, w.dat_x)
+        t_model = run(0, "acq_org.txt", 0)
+        for i in range(1, nsh + 1):
+            os.replace(os.path.join(work, f"dataobs_{i:05d}.bin"), os.path.join(work, f"dataobs_org_{i:05d}.bin"))
+        t_fused = run(1, "acq_org.txt", 0, "--datum", "acq_new.txt", "--batch", str(args.workflow_batch))
+        fused = {i: np.fromfile(os.path.join(work, f"dataobs_redatum_{i:05d}.bin"), np.float32) for i in range(1, kp + 1)}
+        plain = None
+        if kp > 0:
+            acq("acq_org.txt", kp, w.rec_z, w.rec_x)
+            acq("acq_new.txt", kp, w.dat_z, w.dat_x)
+            t1 = run(1, "acq_org.txt", 1)
+            wf_bytes = os.path.getsize(os.path.join(work, "wavefield_exd00001.bin"))
+            t2 = run(2, "acq_new.txt", 1)
+            diff = 0.0
+            for i in range(1, kp + 1):
+                a = np.fromfile(os.path.join(work, f"dataobs_redatum_{i:05d}.bin"), np.float32).astype(np.float64)
+                b = fused[i].astype(np.float64)
+                diff = max(diff, float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)))
+            plain = {"shots": kp, "mode1_s": round(t1, 3), "mode2_s": round(t2, 3),
+                     "shots_per_hour": round(3600.0 * kp / (t1 + t2), 1),
+                     "wavefield_file_bytes_per_shot": 2 * wf_bytes,
+                     "rel_l2_vs_fused": diff}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    value = 3600.0 * nsh / t_fused
+    _emit({"metric": "redatumed shots/hour through dawredatum (modes 1+2 fused by --datum, files in and out)",
+           "value": round(value, 1), "unit": "shots/h", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round(1000.0 * t_fused, 1), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": w.name, "shots": nsh, "maxiter": w.maxiter, "batch": args.workflow_batch},
+           "modeling_mode0_s": round(t_model, 3),
+           "plain_sample": plain})
+
+
 def _emit(obj):
     """The one JSON line, on the process's original stdout."""
     out = _JSON_OUT if _JSON_OUT is not None else sys.stdout
@@ -323,11 +405,18 @@ def main():
     ap.add_argument("--f64-shots", type=int, default=0,
                     help="also run the FP64 oracle of the plain method on this many shots (slow, CPU)")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--workflow", action="store_true",
+                    help="time the reference's file workflow through the dawredatum CLI instead")
+    ap.add_argument("--workflow-plain-shots", type=int, default=1,
+                    help="shots of the plain route (wavefield files) timed beside the fused one")
+    ap.add_argument("--workflow-batch", type=int, default=64, help="dawredatum --batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); shots are independent and would "
               "shard across GPUs without any collective, which this build does not exercise.", file=sys.stderr)
         sys.exit(2)
+    if args.workflow:
+        return main_workflow(args, _pkg()[1])
     import torch
     if args.calibrate_copy:
         calibrate_copy()

@@ -76,6 +76,15 @@ module daw_capi
        integer(c_int) :: ir(*)
        type(c_ptr), value :: hist, info
      end function daw_sublinearminres
+     integer(c_int) function daw_sublinearminres_log(c0, n1, n2, npml, h, ofs, nt, dt, type_rec, nrec, ir, mu, &
+          maxiter, error, deltadr, deltade, log_path) bind(C, name='daw_sublinearminres_log')
+       import :: c_int, c_float, c_char
+       real(c_float) :: c0(*), deltadr(*), deltade(*)
+       integer(c_int), value :: n1, n2, npml, ofs, nt, type_rec, nrec
+       real(c_float), value :: h, dt, mu, maxiter, error
+       integer(c_int) :: ir(*)
+       character(kind=c_char) :: log_path(*)
+     end function daw_sublinearminres_log
      integer(c_int) function daw_filmartin2d(din, nr, nt, dt, flow, fhig, llow, lhig, dout) bind(C, name='daw_filmartin2d')
        import :: c_int, c_float
        real(c_float) :: din(*), dout(*)

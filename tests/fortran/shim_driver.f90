@@ -39,7 +39,7 @@ program shim_driver
 
   grid%n1 = n1; grid%n2 = n2; grid%npml = npml; grid%h = h; grid%nt = nt; grid%dt = dt
   grid%ofs = ofs; grid%type_rec = 0; grid%nrec = nr; grid%mu = mu; grid%maxiter = 4.; grid%error = 0.
-  grid%oio = 0
+  grid%oio = 1
   allocate(grid%c0(n1, n2), inv%deltadr(nt, nr), inv%deltade(nt, nr))
   grid%c0 = vp0
   inv%deltadr = res
@@ -51,3 +51,13 @@ program shim_driver
   write(u) sis, wf, sis2, sis3, wfp, sisd, filt, inv%deltade
   close(u)
 end program shim_driver
+
+! The driver's shot-name routine (subname.f90:2-20: five digits, zero padded),
+! which the shim's sublinearminres calls for the MINRES log name.
+subroutine subname(i, name)
+  implicit none
+  integer :: i
+  character(len=160) :: name
+  name = ' '
+  write(name, '(i5.5)') i
+end subroutine subname

@@ -57,5 +57,10 @@ def test_fortran_driver_through_shim(gpu_daw, tmp_path):
     assert np.array_equal(wfp.reshape(nt, n2, n1), ewf)
     assert np.array_equal(sisd.reshape(nd, nt), O.resample(owf, c.npml, c.dat, c.ofs))
     assert rel_l2(filt.reshape(nr, nt), O.filmartin2d(osis, c.dt, 3.0, 30.0, 2.0, 10.0)) < 1e-6
-    x64, _, _ = O.linearminres(c.vp0, c.npml, c.h, c.ofs, nt, c.dt, res, c.rec, mu, 4, prec="f64")
+    x64, _, info = O.linearminres(c.vp0, c.npml, c.h, c.ofs, nt, c.dt, res, c.rec, mu, 4, prec="f64")
     assert rel_l2(dde.reshape(nr, nt), x64) < 1e-4
+    # grid%oio = 1: the MINRES log the reference's sublinearminres opens (:51-61)
+    log = (tmp_path / "MINRES_00001.txt").read_text().splitlines()
+    assert log[2].startswith(" Enter MINRES.") and f"{nr * nt:7d}" in log[3]
+    assert [int(l.split()[0]) for l in log if len(l.split()) == 5 and l.split()[0].isdigit()] == [1, 2, 3, 4]
+    assert f"istop ={info['istop']:3d}" in "\n".join(log)
