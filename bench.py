@@ -368,6 +368,39 @@ def main_workflow(args, syn):
            "plain_sample": plain})
 
 
+def main_resample(args, daw):
+    """dt -> dto resampling (subresample, subresample_hicks.f90:25-83) of the
+    redatumed gathers of the 201x801 workload: 64 shots x 401 receivers,
+    nt = 3000 at 0.8 ms to 4 ms, traces resident in HBM.  Algorithmic bytes:
+    each input sample read once, each output sample written once."""
+    import torch
+    ntr, n1in, d1in, d1out = 64 * 401, 3000, 8e-4, 4e-3
+    n1out = int(np.floor(float(np.float32(np.float32(n1in - 1) * np.float32(d1in)) / np.float32(d1out)) + 0.5)) + 1
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+    xin = torch.randn((ntr, n1in), generator=g, device="cuda", dtype=torch.float32)
+    xout = torch.empty((ntr, n1out), device="cuda", dtype=torch.float32)
+    for _ in range(max(args.warmup, 2)):
+        daw.subresample_dev(xin, xout, d1in, d1out)
+    reps = max(args.steps, 20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        daw.subresample_dev(xin, xout, d1in, d1out)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / reps
+    nbytes = 4.0 * ntr * (n1in + n1out)
+    _emit({"metric": "dt->dto resampling of redatumed gathers (subresample)", "value": round(ntr / per, 1),
+           "unit": "traces/s", "n_gpus": 1, "steps": reps, "warmup": max(args.warmup, 2),
+           "ms_per_step": round(1000 * per, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{ntr} traces, {n1in} samples at {d1in} s -> {n1out} at {d1out} s"},
+           "roofline_call": {"bound": "hbm", "achieved": round(nbytes / per / 1e9, 1), "peak": 8000.0,
+                             "unit": "GB/s", "frac": round(nbytes / per / 8e12, 3), "traffic": None,
+                             "note": "whole call (host plan and table upload, launch, sync); "
+                                     "kernel alone in the rocprofv3 summary"}})
+
+
 def _emit(obj):
     """The one JSON line, on the process's original stdout."""
     out = _JSON_OUT if _JSON_OUT is not None else sys.stdout
@@ -410,6 +443,8 @@ def main():
     ap.add_argument("--workflow-plain-shots", type=int, default=1,
                     help="shots of the plain route (wavefield files) timed beside the fused one")
     ap.add_argument("--workflow-batch", type=int, default=64, help="dawredatum --batch")
+    ap.add_argument("--resample", action="store_true",
+                    help="time the dt -> dto time resampler on the workload's gathers instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); shots are independent and would "
@@ -422,6 +457,8 @@ def main():
         calibrate_copy()
         return
     daw, syn = _pkg()
+    if args.resample:
+        return main_resample(args, daw)
     daw.set_steps(args.launch_steps)
     daw.set_tile(args.tile_rows, args.tile_cols)
     daw.set_snapshot_format(args.snapshots)

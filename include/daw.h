@@ -132,6 +132,17 @@ int daw_sublinearminres_log(const float *c0, int n1, int n2, int npml, float h, 
  * minresModule.f90:308-325).  n is the system size, nrec*nt. */
 int daw_write_minres_log(const char *path, int n, int itnlim, float rtol, const double *block);
 
+/* subresample(xin,xout,n1in,n2,n1out,d1in,d1out), subresample_hicks.f90:25-83:
+ * each trace of xin(n1in,n2), sampled at d1in, resampled to n1out samples at
+ * d1out with the 9-point Kaiser-windowed sinc (irad = 4, nalpha = 10).  The
+ * reference ships it but does not link it (src/Makefile:24-25); it is the
+ * tool for the dt -> dto decimation that sub_redatum_DAW.f90:24 sizes. */
+int daw_subresample(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
+
+/* The same on device traces (xin, xout device pointers, [n2][n1in] and
+ * [n2][n1out]). */
+int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
+
 /* filmartin2d(deltadeo,nr,nt,dt,flow,fhig,llow,lhig,deltade),
  * minresModule.f90:486-614. */
 int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,

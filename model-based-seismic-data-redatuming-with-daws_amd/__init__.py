@@ -37,6 +37,7 @@ EXPORTS = [
     "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
     "daw_get_launch", "daw_sublinearminres_log", "daw_write_minres_log", "daw_readpar", "daw_read_acqui",
+    "daw_subresample", "daw_subresample_dev",
 ]
 
 
@@ -153,6 +154,8 @@ def lib():
         L.daw_sublinearminres_log.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                               C.c_float, C.c_int, C.c_int, _i32p, C.c_float, C.c_float, C.c_float,
                                               _f32p, _f32p, C.c_char_p]
+        L.daw_subresample.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.daw_subresample_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
         L.daw_write_minres_log.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_float, _f64p]
         L.daw_readpar.argtypes = [C.c_char_p, C.POINTER(Par)]
         L.daw_read_acqui.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int,
@@ -410,6 +413,33 @@ def read_acqui(path):
     _check(lib().daw_read_acqui(os.fsencode(path), C.byref(ns), C.byref(nr), src.ctypes.data, ns.value,
                                 rec.ctypes.data, nr.value))
     return src, rec
+
+
+def subresample(xin, d1in, d1out, n1out=None):
+    """subresample (subresample_hicks.f90:25-83): traces (n2, n1in) sampled at
+    d1in resampled to (n2, n1out) at d1out.  n1out defaults to the count
+    sub_redatum_DAW.f90:24 gives, nint((n1in-1)*d1in/d1out)+1."""
+    xin = _f32(xin)
+    n2, n1in = xin.shape
+    if n1out is None:
+        q = np.float32(np.float32(n1in - 1) * np.float32(d1in)) / np.float32(d1out)
+        n1out = int(np.floor(float(q) + 0.5)) + 1     # NINT of the REAL quotient
+    out = np.zeros((n2, int(n1out)), np.float32)
+    _check(lib().daw_subresample(xin, out, n1in, n2, int(n1out), float(d1in), float(d1out)))
+    return out
+
+
+def subresample_dev(xin_dev, xout_dev, d1in, d1out):
+    """subresample on contiguous float32 cuda tensors (n2, n1in) -> (n2, n1out)."""
+    import torch
+    for t in (xin_dev, xout_dev):
+        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.dim() != 2:
+            raise ValueError("subresample_dev takes contiguous 2-D float32 cuda tensors")
+    if xin_dev.shape[0] != xout_dev.shape[0]:
+        raise ValueError("subresample_dev: trace counts differ")
+    torch.cuda.synchronize()
+    _check(lib().daw_subresample_dev(xin_dev.data_ptr(), xout_dev.data_ptr(), xin_dev.shape[1], xin_dev.shape[0],
+                                     xout_dev.shape[1], float(d1in), float(d1out)))
 
 
 def filmartin2d(din, dt, flow, fhig, llow, lhig):

@@ -1276,6 +1276,59 @@ int daw_subresiduals(const float *dobs, const float *dcal, float *res, int nt, i
     return 0;
 }
 
+static int check_resample(int n1in, int n2, int n1out, float d1in, float d1out)
+{
+    if (n1in < 1 || n2 < 0 || n1out < 0 || !(d1in > 0.0f) || !(d1out > 0.0f))
+        return fail(DAW_ERR_ARG, "subresample: n1in=%d n2=%d n1out=%d d1in=%g d1out=%g", n1in, n2, n1out, (double)d1in,
+                    (double)d1out);
+    if ((double)(n1out > 0 ? n1out - 1 : 0) * d1out / d1in > 2.0e9)
+        return fail(DAW_ERR_ARG, "subresample: output times reach past 2e9 input samples");
+    return 0;
+}
+
+// subresample on device traces (caller holds the lock)
+static int time_resample_dev(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out)
+{
+    if (int rc = check_resample(n1in, n2, n1out, d1in, d1out)) return rc;
+    if ((size_t)n2 * n1out == 0) return 0;
+    if (!xin || !xout) return fail(DAW_ERR_ARG, "subresample: null trace pointer");
+    if (int rc = ensure_stream()) return rc;
+    std::vector<float> coef(9 * kRsNalpha);
+    resample_table(coef.data());
+    std::vector<int> plan;
+    resample_plan(n1out, d1in, d1out, plan);
+    DevBuf d_coef, d_plan;
+    if (int rc = upload(d_coef, coef)) return rc;
+    if (int rc = upload(d_plan, plan)) return rc;
+    launch_time_resample(xin, xout, n1in, n1out, (size_t)n2, d_plan.as<int>(), d_coef.as<float>(), g.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
+int daw_subresample(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_resample(n1in, n2, n1out, d1in, d1out)) return rc;
+    const size_t nin = (size_t)n1in * n2, nout = (size_t)n1out * n2;
+    if (nout == 0) return 0;
+    if (!xin || !xout) return fail(DAW_ERR_ARG, "subresample: null trace pointer");
+    if (int rc = ensure_stream()) return rc;
+    DevBuf a, b;
+    if (int rc = a.alloc(nin * 4)) return rc;
+    if (int rc = b.alloc(nout * 4)) return rc;
+    HIPCHK(hipMemcpy(a.p, xin, nin * 4, hipMemcpyHostToDevice));
+    if (int rc = time_resample_dev(a.as<float>(), b.as<float>(), n1in, n2, n1out, d1in, d1out)) return rc;
+    HIPCHK(hipMemcpy(xout, b.p, nout * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    return time_resample_dev(xin, xout, n1in, n2, n1out, d1in, d1out);
+}
+
 int daw_submodeling(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt, const float *s,
                     int type_src, int type_rec, const int *is5, const int *ir, int nr, float *sis, float *wavefield)
 {

@@ -31,19 +31,19 @@ float bessi0(float x)
     return static_cast<float>(static_cast<double>(pre) * q);
 }
 
-void hicks1d(float *out)
+void hicks1d_table(int irad, int nalpha, float beta, float *out)
 {
     const float pi = 3.14159265359f;          // kwsinc.f90:108 (REAL literal)
-    const float beta = 14.18f;                // bhicks0(irad = 10), kwsinc.f90:43
     const float i0_beta = bessi0(beta);
-    for (int ia = 1; ia <= kNalpha; ++ia) {
-        const float alpha = 0.5f - static_cast<float>(ia - 1) / static_cast<float>(kNalpha);
-        for (int n = -kIrad; n <= kIrad; ++n) {
+    const int w = 2 * irad + 1;
+    for (int ia = 1; ia <= nalpha; ++ia) {
+        const float alpha = 0.5f - static_cast<float>(ia - 1) / static_cast<float>(nalpha);
+        for (int n = -irad; n <= irad; ++n) {
             const float xn = static_cast<float>(n) + alpha;
             float c = 0.0f;
-            if (std::fabs(xn) <= static_cast<float>(kIrad)) {
+            if (std::fabs(xn) <= static_cast<float>(irad)) {
                 if (std::fabs(xn) > 1e-15f) {
-                    const float arg = beta * std::sqrt(1.0f - xn * xn / static_cast<float>(kIrad * kIrad));
+                    const float arg = beta * std::sqrt(1.0f - xn * xn / static_cast<float>(irad * irad));
                     const float window = bessi0(arg) / i0_beta;
                     c = window * std::sin(pi * xn) / (pi * xn);
                     if (std::fabs(c) < 1e-15f) c = 0.0f;
@@ -51,8 +51,30 @@ void hicks1d(float *out)
                     c = 1.0f;
                 }
             }
-            out[(n + kIrad) + kWin * (ia - 1)] = c;
+            out[(n + irad) + w * (ia - 1)] = c;
         }
+    }
+}
+
+// bhicks0(irad = 10), kwsinc.f90:43
+void hicks1d(float *out) { hicks1d_table(kIrad, kNalpha, 14.18f, out); }
+
+// bhicks0(irad = 4), subresample_hicks.f90:103
+void resample_table(float *out) { hicks1d_table(kRsRad, kRsNalpha, 6.31f, out); }
+
+void resample_plan(int n1out, float d1in, float d1out, std::vector<int> &plan)
+{
+    plan.resize(2 * static_cast<size_t>(n1out));
+    for (int i1 = 1; i1 <= n1out; ++i1) {
+        const float tout = static_cast<float>(i1 - 1) * d1out;
+        const int it0 = static_cast<int>(std::lround(tout / d1in)) + 1;     // NINT of a REAL
+        const float t0 = static_cast<float>(it0 - 1) * d1in;
+        const float alpha = (t0 - tout) / d1in;
+        int ia = static_cast<int>(std::lround((0.5f - alpha) * static_cast<float>(kRsNalpha))) + 1;
+        if (ia > kRsNalpha) ia = kRsNalpha;
+        if (ia < 1) ia = 1;       // alpha > 0.55 is out of reach of rounding; the reference has no guard
+        plan[2 * (size_t)(i1 - 1)] = it0;
+        plan[2 * (size_t)(i1 - 1) + 1] = ia;
     }
 }
 

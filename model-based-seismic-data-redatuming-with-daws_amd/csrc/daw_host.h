@@ -21,6 +21,21 @@ float bessi0(float x);
 // out[(n+10) + 21*(ia-1)], n = -10..10, ia = 1..100.
 void hicks1d(float *out);
 
+// Kaiser-windowed-sinc monopole table of half-width irad tabulated at nalpha
+// sub-sample shifts with Kaiser beta (the loop of kwsinc.f90:115-177 and of
+// subhicks1d, subresample_hicks.f90:110-155); out[(n+irad) + (2*irad+1)*(ia-1)].
+// Entries outside the window's support are 0.
+void hicks1d_table(int irad, int nalpha, float beta, float *out);
+
+// The 9x10 table of the time resampler: irad = 4, nalpha = 10, beta =
+// bhicks0(4) = 6.31 (subresample_hicks.f90:40-41, 103-106).
+constexpr int kRsRad = 4, kRsNalpha = 10;
+void resample_table(float *out);
+
+// Per output sample i1 (0-based) of subresample: it0 (1-based nearest input
+// sample) and ialpha, computed as subresample_hicks.f90:59-66 computes them.
+void resample_plan(int n1out, float d1in, float d1out, std::vector<int> &plan);
+
 // One 21x21 tile of the 2-D monopole table for (ialpha1, ialpha2, ip) as
 // kwsinc2d(ofs) leaves it (kwsinc.f90:212-378); out[(n1+10) + 21*(n2+10)].
 void hicks2d_tile(int ia1, int ia2, int ip, int ofs, float *out);

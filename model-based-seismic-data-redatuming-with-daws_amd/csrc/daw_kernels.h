@@ -42,6 +42,11 @@ void launch_minres_l1(float *y, const float *v, const float *r1, const size_t *o
                       const double *scal, double *part, int nshots, hipStream_t st);
 void launch_minres_l2(float *y, const float *r2, const size_t *off, const int *len, const double *scal,
                       double *part, int nshots, hipStream_t st);
+// subresample of ntr traces [ntr][n1in] -> [ntr][n1out]; plan holds (it0,
+// ialpha) per output sample (2 ints each), coef the 9x10 table
+void launch_time_resample(const float *xin, float *xout, int n1in, int n1out, size_t ntr, const int *plan,
+                          const float *coef, hipStream_t st);
+
 // trace (may be null): kTraceStride doubles per shot and iteration,
 // [shot][itn-1] = x(1), qrnorm/beta1, Anorm, Acond, printed (1/0) -- the
 // columns of the MINRES log line of that iteration and whether it is written

@@ -1506,6 +1506,46 @@ void launch_residuals(const float *dobs, const float *dcal, float *res, int nt, 
     hipLaunchKernelGGL(k_residuals, dim3((unsigned)blocks), dim3(256), 0, st, dobs, dcal, res, nt, ntr);
 }
 
+// subresample (subresample_hicks.f90:57-75): output sample i1 of every trace
+// is the sum over j = max(1-it0,-4) .. min(n1in-it0,4) of coef(j,ia) *
+// xin(it0+j), in that order from 0.0, with (it0, ia) = plan[i1] (it0 1-based).
+// A thread owns one output sample: its window, clipping and nine
+// coefficients stay in registers while it walks the traces of its grid row
+// (blockIdx.y + k*gridDim.y); lanes of a wave read one stretch of input.
+__global__ void __launch_bounds__(256) k_time_resample(const float *xin, float *xout, int n1in, int n1out, int ntr,
+                                                       const int2 *plan, const float *coef)
+{
+    const int i1 = blockIdx.x * 256 + threadIdx.x;
+    if (i1 >= n1out) return;
+    const int2 p = plan[i1];
+    const int jmin = max(1 - p.x, -4), jmax = min(n1in - p.x, 4);
+    float c[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) c[j] = coef[j + 9 * (p.y - 1)];
+    for (int tr = blockIdx.y; tr < ntr; tr += gridDim.y) {
+        const float *x = xin + (size_t)tr * n1in + (p.x - 1);
+        float acc = 0.0f;
+        if (jmin == -4 && jmax == 4) {
+#pragma unroll
+            for (int j = -4; j <= 4; ++j) acc = acc + c[j + 4] * x[j];
+        } else {
+            for (int j = jmin; j <= jmax; ++j) acc = acc + c[j + 4] * x[j];
+        }
+        xout[(size_t)tr * n1out + i1] = acc;
+    }
+}
+
+void launch_time_resample(const float *xin, float *xout, int n1in, int n1out, size_t ntr, const int *plan,
+                          const float *coef, hipStream_t st)
+{
+    if (ntr == 0 || n1out == 0) return;
+    const unsigned bx = (unsigned)((n1out + 255) / 256);
+    // enough rows that the grid holds several thousand workgroups
+    const unsigned by = (unsigned)std::min<size_t>(ntr, std::max<size_t>(1, 8192 / bx));
+    hipLaunchKernelGGL(k_time_resample, dim3(bx, by), dim3(256), 0, st, xin, xout, n1in, n1out, (int)ntr,
+                       reinterpret_cast<const int2 *>(plan), coef);
+}
+
 __global__ void k_add(const float *a, const float *b, float *c, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

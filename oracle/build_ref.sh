@@ -27,7 +27,8 @@ cd "$TMP"
 sed -n '/^module parameters/,/^end module parameters/p' "$REF/src/main_redatum_DAW.f90" > parameters.f90
 SRCS="parameters.f90 $REF/src/kwsinc.f90 $REF/src/libfourier.f90 $REF/src/minresModule.f90 \
       $REF/src/sublinearminres.f90 $REF/src/submodeling2.f90 $REF/src/subwavefieldresample.f90 \
-      $REF/src/subresiduals.f90 $REF/src/subname.f90 $REF/src/subacqui.f90 $HERE/ref_harness.f90"
+      $REF/src/subresiduals.f90 $REF/src/subname.f90 $REF/src/subacqui.f90 $REF/src/subresample_hicks.f90 \
+      $HERE/ref_harness.f90"
 OBJS=""
 for f in $SRCS; do
   o=$(basename "$f" .f90).o

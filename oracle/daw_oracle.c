@@ -82,6 +82,67 @@ void oracle_hicks1d(float *out)
     }
 }
 
+/*
+ * subhicks1d of subresample_hicks.f90:87-157 with irad = 4, nalpha = 10
+ * (:40-41): the same Kaiser-windowed sinc with Kaiser beta bhicks0(4) = 6.31
+ * (:103-106).  out[(n+4) + 9*(ia-1)], n = -4..4, ia = 1..10.  Entries with
+ * |n + alpha| > irad are never written there (:124); here they are 0, the
+ * window's value outside its support.
+ */
+void oracle_subhicks1d(float *out)
+{
+    const float PI = 3.14159265359f;
+    const float b = 6.31f;
+    float i0d = oracle_bessi0(b);
+    for (int ia = 1; ia <= 10; ia++) {
+        float alpha = 0.5f - (float)(ia - 1) / 10.0f;
+        for (int n = -4; n <= 4; n++) {
+            float xn = (float)n + alpha;
+            float c = 0.0f;
+            if (fabsf(xn) <= 4.0f) {
+                if (fabsf(xn) > 1e-15f) {
+                    float i0n = oracle_bessi0(b * sqrtf(1.0f - xn * xn / 16.0f));
+                    float kaiser = i0n / i0d;
+                    c = kaiser * sinf(PI * xn) / (PI * xn);
+                    if (fabsf(c) < 1e-15f) c = 0.0f;
+                } else {
+                    c = 1.0f;
+                }
+            }
+            out[(n + 4) + 9 * (ia - 1)] = c;
+        }
+    }
+}
+
+/*
+ * subresample (subresample_hicks.f90:25-83): xin(n1in,n2) sampled at d1in
+ * to xout(n1out,n2) at d1out, each output sample the 9-point windowed-sinc
+ * sum around the nearest input sample (:57-75), in REAL arithmetic.
+ */
+void oracle_subresample(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out)
+{
+    float c[9 * 10];
+    oracle_subhicks1d(c);
+    for (int i2 = 0; i2 < n2; i2++) {
+        const float *x = xin + (size_t)i2 * n1in;
+        float *y = xout + (size_t)i2 * n1out;
+        for (int i1 = 1; i1 <= n1out; i1++) {
+            float tout = (float)(i1 - 1) * d1out;
+            int it0 = (int)lroundf(tout / d1in) + 1;
+            float t0 = (float)(it0 - 1) * d1in;
+            float alpha = (t0 - tout) / d1in;
+            int ia = (int)lroundf((0.5f - alpha) * 10.0f) + 1;
+            if (ia > 10) ia = 10;
+            if (ia < 1) ia = 1;   /* alpha > 0.55: unreachable by rounding; the reference has no guard */
+            int jmin = 1 - it0 > -4 ? 1 - it0 : -4;
+            int jmax = n1in - it0 < 4 ? n1in - it0 : 4;
+            float acc = 0.0f;
+            for (int j = jmin; j <= jmax; j++) acc = acc + c[(j + 4) + 9 * (ia - 1)] * x[it0 + j - 1];
+            y[i1 - 1] = acc;
+        }
+    }
+}
+
 static float g_h1d[21 * 100];
 static int g_h1d_ready = 0;
 

@@ -85,6 +85,8 @@ def lib():
                           C.c_int, C.c_int, _i32p, C.c_float, C.c_int, C.c_float, rp, rp, rp, rp, _i32p]
             f.restype = C.c_int
             getattr(L, "oracle_set_minres_trace_" + sfx).argtypes = [C.c_void_p]
+        L.oracle_subhicks1d.argtypes = [_f32p]
+        L.oracle_subresample.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
         _lib = L
     return _lib
 
@@ -270,6 +272,22 @@ def linearminres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, itnlim, rtol=0.
     return x, hist[:info["itn"]], info
 
 
+def subhicks1d():
+    """The resampler's table (subresample_hicks.f90:87-157, irad 4, nalpha 10) as (10, 9)."""
+    out = np.zeros((10, 9), np.float32)
+    lib().oracle_subhicks1d(out)
+    return out
+
+
+def subresample(xin, d1in, d1out, n1out):
+    """subresample (subresample_hicks.f90:25-83) on traces (n2, n1in)."""
+    xin = _c(xin, np.float32)
+    n2, n1in = xin.shape
+    out = np.zeros((n2, int(n1out)), np.float32)
+    lib().oracle_subresample(xin, out, n1in, n2, int(n1out), float(d1in), float(d1out))
+    return out
+
+
 def set_progress(on):
     """Per-iteration progress lines on stderr from the long-running solvers."""
     lib().oracle_set_progress(1 if on else 0)
@@ -350,6 +368,9 @@ def ref():
         L.dawref_filmartin2d.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                          C.c_float, _f32p]
         L.dawref_realft2s.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
+        if hasattr(L, "dawref_subresample"):
+            L.dawref_subhicks1d.argtypes = [_f32p]
+            L.dawref_subresample.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
         _ref = L
     return _ref
 

@@ -212,4 +212,26 @@ contains
     call REALFT2S(int(n), int(isign), dre, dim_, buf)
   end subroutine dawref_realft2s
 
+  ! subhicks1d (subresample_hicks.f90:87) on a zeroed table: it writes the
+  ! entries inside the window's support and leaves the rest as they were
+  subroutine dawref_subhicks1d(out) bind(C, name='dawref_subhicks1d')
+    real(c_float), intent(out) :: out(-4:4, 10)
+    external :: subhicks1d
+    out = 0.
+    call subhicks1d(out, 4, 10)
+  end subroutine dawref_subhicks1d
+
+  ! subresample (subresample_hicks.f90:25)
+  subroutine dawref_subresample(xin, xout, n1in, n2, n1out, d1in, d1out) bind(C, name='dawref_subresample')
+    integer(c_int), value :: n1in, n2, n1out
+    real(c_float), value :: d1in, d1out
+    real(c_float), intent(in) :: xin(n1in, n2)
+    real(c_float), intent(out) :: xout(n1out, n2)
+    external :: subresample
+    real :: a, b
+    a = d1in
+    b = d1out
+    call subresample(xin, xout, int(n1in), int(n2), int(n1out), a, b)
+  end subroutine dawref_subresample
+
 end module dawref_harness

@@ -2,6 +2,8 @@
 every case fails, or succeeds, before the library touches the GPU)."""
 import ctypes as C
 
+import numpy as np
+
 
 def test_snapshot_format_knob(daw):
     assert daw.lib().daw_set_snapshot_format(3) == -1
@@ -61,3 +63,16 @@ def test_batched_entry_points_reject_bad_arguments(daw):
         assert rc == -1
     finally:
         daw.deallocatekws2d()
+
+
+def test_subresample_rejects_bad_arguments(daw):
+    L = daw.lib()
+    x = np.zeros(10, np.float32)
+    y = np.zeros(10, np.float32)
+    for n1in, n2, n1out, d1in, d1out in [(0, 1, 5, 1e-3, 1e-3), (10, -1, 5, 1e-3, 1e-3), (10, 1, -5, 1e-3, 1e-3),
+                                         (10, 1, 5, 0.0, 1e-3), (10, 1, 5, 1e-3, -1.0)]:
+        assert L.daw_subresample(x, y, n1in, n2, n1out, d1in, d1out) == -1
+        assert b"subresample" in L.daw_last_error()
+    # nothing to do is not an error
+    assert L.daw_subresample(x, y, 10, 0, 5, 1e-3, 1e-3) == 0
+    assert L.daw_subresample(x, y, 10, 1, 0, 1e-3, 1e-3) == 0
