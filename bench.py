@@ -359,13 +359,21 @@ def main_workflow(args, syn):
     finally:
         shutil.rmtree(work, ignore_errors=True)
     value = 3600.0 * nsh / t_fused
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        P = dict(w=w, c0=c0, h=np.float32(w.h), dt=np.float32(w.dt),
+                 rec=O.acqui_table(w.rec_z, w.rec_x, w.h, w.npml, w.ofs))
+        _log("reference CPU path sample")
+        cpu = cpu_baseline(P, min(args.cpu_nt, w.nt))
     _emit({"metric": "redatumed shots/hour through dawredatum (modes 1+2 fused by --datum, files in and out)",
            "value": round(value, 1), "unit": "shots/h", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": round(1000.0 * t_fused, 1), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": w.name, "shots": nsh, "maxiter": w.maxiter, "batch": args.workflow_batch},
            "modeling_mode0_s": round(t_model, 3),
-           "plain_sample": plain})
+           "plain_sample": plain, "cpu_baseline": cpu})
 
 
 def main_resample(args, daw):
@@ -390,6 +398,22 @@ def main_resample(args, daw):
     torch.cuda.synchronize()
     per = (time.perf_counter() - t0) / reps
     nbytes = 4.0 * ntr * (n1in + n1out)
+    # the reference's own subresample (oracle/_ref) on a bounded sample of the traces, 1 core
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ns = 4000
+    xs = xin[:ns].cpu().numpy()
+    ys = np.zeros((ns, n1out), np.float32)
+    R = O.ref() if O.ref_available() else None
+    kind = "reference" if R is not None and hasattr(R, "dawref_subresample") else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        R.dawref_subresample(xs, ys, n1in, ns, n1out, d1in, d1out)
+    else:
+        O.subresample(xs, d1in, d1out, n1out)
+    tc = time.perf_counter() - t0
+    cpu = {"value": round(ns / tc, 1), "unit": "traces/s", "cores": 1, "kind": kind,
+           "sample": f"{ns} of the {ntr} traces in {tc:.3f} s"}
     _emit({"metric": "dt->dto resampling of redatumed gathers (subresample)", "value": round(ntr / per, 1),
            "unit": "traces/s", "n_gpus": 1, "steps": reps, "warmup": max(args.warmup, 2),
            "ms_per_step": round(1000 * per, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -398,7 +422,8 @@ def main_resample(args, daw):
            "roofline_call": {"bound": "hbm", "achieved": round(nbytes / per / 1e9, 1), "peak": 8000.0,
                              "unit": "GB/s", "frac": round(nbytes / per / 8e12, 3), "traffic": None,
                              "note": "whole call (host plan and table upload, launch, sync); "
-                                     "kernel alone in the rocprofv3 summary"}})
+                                     "kernel alone in the rocprofv3 summary"},
+           "cpu_baseline": cpu})
 
 
 def _emit(obj):

@@ -85,6 +85,12 @@ module daw_capi
        integer(c_int) :: ir(*)
        character(kind=c_char) :: log_path(*)
      end function daw_sublinearminres_log
+     integer(c_int) function daw_subresample(xin, xout, n1in, n2, n1out, d1in, d1out) bind(C, name='daw_subresample')
+       import :: c_int, c_float
+       real(c_float) :: xin(*), xout(*)
+       integer(c_int), value :: n1in, n2, n1out
+       real(c_float), value :: d1in, d1out
+     end function daw_subresample
      integer(c_int) function daw_filmartin2d(din, nr, nt, dt, flow, fhig, llow, lhig, dout) bind(C, name='daw_filmartin2d')
        import :: c_int, c_float
        real(c_float) :: din(*), dout(*)
@@ -214,3 +220,14 @@ subroutine subwavefieldresample(n1, n2, npml, nt, dt, ir, nr, sis, wavefield)
   real :: wavefield(n1, n2, nt), sis(nt, nr)
   call daw_check(daw_subwavefieldresample(n1, n2, npml, nt, dt, ir, nr, sis, wavefield), 'subwavefieldresample')
 end subroutine subwavefieldresample
+
+! subresample_hicks.f90:25 (the time resampler the reference ships without
+! linking; a driver that adds it gets this one)
+subroutine subresample(xin, xout, n1in, n2, n1out, d1in, d1out)
+  use daw_capi
+  implicit none
+  integer :: n1in, n2, n1out
+  real :: d1in, d1out
+  real :: xin(n1in, n2), xout(n1out, n2)
+  call daw_check(daw_subresample(xin, xout, n1in, n2, n1out, d1in, d1out), 'subresample')
+end subroutine subresample

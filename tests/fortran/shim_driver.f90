@@ -10,7 +10,8 @@ program shim_driver
   integer :: hdr(7), n1, n2, npml, nt, nr, ndat, ofs, u, ix(2), ia(2), nf
   real :: fh(3), h, dt, mu, xs(2)
   real, allocatable :: vp(:, :), vp0(:, :), s(:), res(:, :), sis(:, :), wf(:, :, :), sis2(:, :), sis3(:, :)
-  real, allocatable :: wfp(:, :, :), sisd(:, :), filt(:, :)
+  real, allocatable :: wfp(:, :, :), sisd(:, :), filt(:, :), rsmp(:, :)
+  integer :: ntout
   integer, allocatable :: src(:), rec(:, :), dat(:, :)
   type(grid_type) :: grid
   type(inv_type) :: inv
@@ -36,6 +37,10 @@ program shim_driver
   call subwavefieldresample(n1, n2, npml, nt, dt, dat, ndat, sisd, wf)
   fl = 3.0; fhg = 30.0; ll = 2.0; lh = 10.0
   call filmartin2d(sis, nr, nt, dt, fl, fhg, ll, lh, filt)
+  ! dt -> 2.5 dt, nto as sub_redatum_DAW.f90:24 sizes it
+  ntout = nint(float(nt - 1) * dt / (2.5 * dt)) + 1
+  allocate(rsmp(ntout, nr))
+  call subresample(sis, rsmp, nt, nr, ntout, dt, 2.5 * dt)
 
   grid%n1 = n1; grid%n2 = n2; grid%npml = npml; grid%h = h; grid%nt = nt; grid%dt = dt
   grid%ofs = ofs; grid%type_rec = 0; grid%nrec = nr; grid%mu = mu; grid%maxiter = 4.; grid%error = 0.
@@ -48,7 +53,7 @@ program shim_driver
 
   open(newunit=u, file='out.bin', access='stream', form='unformatted', status='replace')
   write(u) ix, ia, nf
-  write(u) sis, wf, sis2, sis3, wfp, sisd, filt, inv%deltade
+  write(u) sis, wf, sis2, sis3, wfp, sisd, filt, inv%deltade, rsmp
   close(u)
 end program shim_driver
 

@@ -40,12 +40,15 @@ def test_fortran_driver_through_shim(gpu_daw, tmp_path):
     ints = np.frombuffer(raw[:20], np.int32)
     f = np.frombuffer(raw[20:], np.float32)
     nt, n1, n2 = c.nt, c.n1, c.n2
-    sizes = [nr * nt, n1 * n2 * nt, nr * nt, nr * nt, n1 * n2 * nt, nd * nt, nr * nt, nr * nt]
+    d2 = np.float32(2.5) * c.dt
+    ntout = int(np.floor(float(np.float32(np.float32(nt - 1) * c.dt) / d2) + 0.5)) + 1
+    sizes = [nr * nt, n1 * n2 * nt, nr * nt, nr * nt, n1 * n2 * nt, nd * nt, nr * nt, nr * nt, nr * ntout]
     parts, o = [], 0
     for n in sizes:
         parts.append(f[o:o + n])
         o += n
-    sis, wf, sis2, sis3, wfp, sisd, filt, dde = parts
+    assert o == f.size
+    sis, wf, sis2, sis3, wfp, sisd, filt, dde, rsmp = parts
     ix, ia, nf = O.subsourcekws(15.0, 123.4, c.h, c.ofs)
     assert list(ints) == list(ix) + list(ia) + [nf]
     osis, owf = O.forward(c.vp, c.npml, c.h, c.ofs, nt, c.dt, c.s, c.src, c.rec, want_wavefield=True)
@@ -57,6 +60,7 @@ def test_fortran_driver_through_shim(gpu_daw, tmp_path):
     assert np.array_equal(wfp.reshape(nt, n2, n1), ewf)
     assert np.array_equal(sisd.reshape(nd, nt), O.resample(owf, c.npml, c.dat, c.ofs))
     assert rel_l2(filt.reshape(nr, nt), O.filmartin2d(osis, c.dt, 3.0, 30.0, 2.0, 10.0)) < 1e-6
+    assert np.array_equal(rsmp.reshape(nr, ntout), O.subresample(osis, c.dt, d2, ntout))
     x64, _, info = O.linearminres(c.vp0, c.npml, c.h, c.ofs, nt, c.dt, res, c.rec, mu, 4, prec="f64")
     assert rel_l2(dde.reshape(nr, nt), x64) < 1e-4
     # grid%oio = 1: the MINRES log the reference's sublinearminres opens (:51-61)
