@@ -772,38 +772,66 @@ static int autotune_tiles(Batch &B)
             W.host[s].amp_rev = 0;
         }
         if (int rc = refresh_shots(W)) return rc;
+        // The probes carry the record blocks the real sweeps carry: the
+        // acquisition receivers in the re-injection (the Hessian's data
+        // record), acquisition and datum receivers in the forward sweep.  They
+        // cost 12-36 % of a step and shift which tile wins.
+        const int ns = B.nshots;
+        DevBuf rec_out, dat_out;
+        if (int rc = rec_out.alloc(std::max<size_t>(B.roff[ns], 1) * sizeof(float))) return rc;
+        if (int rc = dat_out.alloc(std::max<size_t>(B.doff[ns], 1) * sizeof(float))) return rc;
+        std::vector<RecItem> it_reinj, it_fwd;
+        for (int s = 0; s < ns; ++s) {
+            const Geo &Q = B.geo_of(s);
+            add_records(it_reinj, Q.rec, s, 0, 1, Q.d_rec_coef, rec_out.as<float>() + B.roff[s], Q.nt);
+            add_records(it_fwd, Q.rec, s, 0, 0, Q.d_rec_coef, rec_out.as<float>() + B.roff[s], Q.nt);
+            if (Q.dat.n > 0)
+                add_records(it_fwd, Q.dat, s, 1, 0, Q.d_dat_coef, dat_out.as<float>() + B.doff[s], Q.nt);
+        }
         const bool timing = g.timing;
         const long long pts = g.pts, launches = g.launches;
         g.timing = false;
-        const int probe = std::min(24, B.nt);
+        // Probes of 64 steps, three rounds over all candidates (round-robin, so
+        // that clock drift during the tuning does not favour the candidates
+        // timed first or last), median of the three per candidate.
+        const int probe = std::min(64, B.nt);
         const bool log = getenv("DAW_TUNE_LOG") != nullptr;
         static const char *names[3] = {"forward", "back-propagation", "re-injection"};
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         std::vector<int> best(9);
+        constexpr int kRounds = 3;
         for (int mode : {kBack, kReinject, kFwd}) {
-            LaunchCfg pick = cand[0];
-            float best_ms = 1e30f;
-            for (const LaunchCfg &c : cand) {
-                W.cfg[mode] = c;
-                if (int rc = finalize_sweepset(W)) return rc;
-                if (int rc = run_sweep(mode, W, nullptr, 0, 4)) return rc;       // warm
-                float ms = 1e30f;
-                for (int rep = 0; rep < 3; ++rep) {                              // best of three probes
+            const std::vector<RecItem> *items = mode == kBack ? nullptr : mode == kReinject ? &it_reinj : &it_fwd;
+            if (items)
+                if (int rc = set_items(W, *items)) return rc;
+            const RecItem *d_items = items ? W.d_items.as<RecItem>() : nullptr;
+            const int n_items = items ? (int)items->size() : 0;
+            std::vector<float> t(cand.size() * kRounds, 0.0f);
+            for (int rep = 0; rep < kRounds; ++rep) {
+                for (size_t k = 0; k < cand.size(); ++k) {
+                    W.cfg[mode] = cand[k];
+                    if (int rc = finalize_sweepset(W)) return rc;
+                    if (int rc = run_sweep(mode, W, d_items, n_items, 4)) return rc;       // warm
                     HIPCHK(hipEventRecord(e0, g.stream));
-                    if (int rc = run_sweep(mode, W, nullptr, 0, probe)) return rc;
+                    if (int rc = run_sweep(mode, W, d_items, n_items, probe)) return rc;
                     HIPCHK(hipEventRecord(e1, g.stream));
                     HIPCHK(hipEventSynchronize(e1));
-                    float m = 0;
-                    HIPCHK(hipEventElapsedTime(&m, e0, e1));
-                    ms = std::min(ms, m);
+                    HIPCHK(hipEventElapsedTime(&t[k * kRounds + rep], e0, e1));
                 }
+            }
+            LaunchCfg pick = cand[0];
+            float best_ms = 1e30f;
+            for (size_t k = 0; k < cand.size(); ++k) {
+                float *r = &t[k * kRounds];
+                std::sort(r, r + kRounds);
+                const float ms = r[kRounds / 2];
                 if (log)
                     fprintf(stderr, "[daw tune] %d shots %dx%d grid, %s: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
-                            B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], c.steps, c.tz, 64 * c.txw,
-                            1000.0f * ms / probe);
-                if (ms < best_ms) { best_ms = ms; pick = c; }
+                            B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], cand[k].steps, cand[k].tz,
+                            64 * cand[k].txw, 1000.0f * ms / probe);
+                if (ms < best_ms) { best_ms = ms; pick = cand[k]; }
             }
             W.cfg[mode] = pick;
             best[3 * mode] = pick.steps; best[3 * mode + 1] = pick.tz; best[3 * mode + 2] = pick.txw;
