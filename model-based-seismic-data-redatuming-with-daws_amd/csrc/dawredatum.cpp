@@ -25,6 +25,12 @@
 //                 MINRES_ files are written too.
 //   --batch N     shots per batched call (default 64); mode 0 with oio=0 and
 //                 --datum runs are batched.
+//   --dto         wherever dataobs_redatum_<shot>.bin is written (mode 2,
+//                 --datum), also write dataobs_redatum_dto_<shot>.bin: the
+//                 gathers resampled from dt to dto (nto samples per trace,
+//                 sub_redatum_DAW.f90:24) with subresample
+//                 (subresample_hicks.f90:25-83), which the reference ships
+//                 but does not call.
 //   --par FILE    parameter file (default redatum.par).
 //   --dry-run     read and check every input, print the geometry tables, stop
 //                 before any GPU work.
@@ -149,6 +155,9 @@ struct Run {
     Acq acq;
     int batch = 64;
     bool quiet = false;
+    bool dto = false;
+    // sub_redatum_DAW.f90:24: nto = nint(float(nt-1)*dt/dto)+1
+    int nto() const { return (int)std::lround((double)((float)(p.nt - 1) * p.dt / p.dto)) + 1; }
     size_t plane() const { return (size_t)p.n1 * p.n2; }
     size_t ntr() const { return (size_t)p.nt * acq.nrec; }
     std::string bin(const char *stem, int isrc) const { return stem + shot_name(isrc) + ".bin"; }
@@ -295,6 +304,16 @@ void mode13_fused(Run &R, const Acq &datum)
         hip(hipMemcpy(h_datum.data(), d_datum.p, (size_t)ns * nd * nt * sizeof(float), hipMemcpyDeviceToHost), "copy");
         for (int s = 0; s < ns; ++s)
             write_stream(R.bin("dataobs_redatum_", s0 + s + 1), h_datum.data() + (size_t)s * nd * nt, (size_t)nd * nt);
+        if (R.dto) {                 // on the device gathers, all shots of the batch in one call
+            const int no = R.nto();
+            Dev<float> d_dto((size_t)ns * nd * no);
+            check(daw_subresample_dev(d_datum.p, d_dto.p, nt, ns * nd, no, p.dt, p.dto), "subresample");
+            std::vector<float> h_dto((size_t)ns * nd * no);
+            hip(hipMemcpy(h_dto.data(), d_dto.p, h_dto.size() * sizeof(float), hipMemcpyDeviceToHost), "copy");
+            for (int s = 0; s < ns; ++s)
+                write_stream(R.bin("dataobs_redatum_dto_", s0 + s + 1), h_dto.data() + (size_t)s * nd * no,
+                             (size_t)nd * no);
+        }
         if (!out) continue;
         h_v.resize((size_t)ns * R.ntr());
         hip(hipMemcpy(h_v.data(), d_de.p, h_v.size() * sizeof(float), hipMemcpyDeviceToHost), "copy");
@@ -323,13 +342,18 @@ void mode2(Run &R)
               "subwavefieldresample");
         check(daw_filmartin2d(sis.data(), nr, p.nt, p.dt, p.flow, p.fhig, p.llow, p.lhig, sisf.data()), "filmartin2d");
         write_stream(R.bin("dataobs_redatum_", is), sisf.data(), sisf.size());
+        if (R.dto) {
+            std::vector<float> d((size_t)nr * R.nto());
+            check(daw_subresample(sisf.data(), d.data(), p.nt, nr, R.nto(), p.dt, p.dto), "subresample");
+            write_stream(R.bin("dataobs_redatum_dto_", is), d.data(), d.size());
+        }
     }
 }
 
 void usage()
 {
     std::fprintf(stderr,
-                 "usage: dawredatum [--par FILE] [--datum ACQUISITION_FILE] [--batch N] [--dry-run] [--quiet]\n"
+                 "usage: dawredatum [--par FILE] [--datum ACQUISITION_FILE] [--batch N] [--dto] [--dry-run] [--quiet]\n"
                  "Runs the mode that redatum.par selects in the current directory (see the header of\n"
                  "dawredatum.cpp for the files each mode reads and writes).\n");
 }
@@ -352,6 +376,7 @@ int main(int argc, char **argv)
         else if (a == "--batch") R.batch = std::atoi(next().c_str());
         else if (a == "--dry-run") dry = true;
         else if (a == "--quiet") R.quiet = true;
+        else if (a == "--dto") R.dto = true;
         else if (a == "-h" || a == "--help") { usage(); return 0; }
         else { usage(); die("unknown argument " + a); }
     }
@@ -360,6 +385,7 @@ int main(int argc, char **argv)
     daw_par &p = R.p;
     check(daw_readpar(par.c_str(), &p), "parameter file");
     if (p.n1 < 1 || p.n2 < 1 || p.nt < 1 || !(p.h > 0) || !(p.dt > 0)) die("bad grid in " + par);
+    if (R.dto && !(p.dto > 0)) die("--dto needs dto > 0 in " + par);
     if (!R.quiet) {
         std::printf(" Run mode: %d\n", p.mode);
         std::printf(" Space & time grid dimension and interval (n1 n2 h nt dt) %d %d %g %d %g\n", p.n1, p.n2, p.h, p.nt,
@@ -384,8 +410,7 @@ int main(int argc, char **argv)
     if (!R.quiet) {
         std::printf(" Number of sources and receivers %d %d\n", R.acq.nsrc, R.acq.nrec);
         // sub_redatum_DAW.f90:24
-        const int nto = (int)std::lround((double)((float)(p.nt - 1) * p.dt / p.dto)) + 1;
-        std::printf(" Niquist samping number %d\n", nto);
+        if (p.dto > 0) std::printf(" Niquist samping number %d\n", R.nto());
     }
     if (dry) {
         auto dump = [](const char *name, const std::vector<int> &t, int n) {
