@@ -362,6 +362,9 @@ def ref():
                                       _i32p, _i32p]
         L.dawref_linearminres.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_int,
                                           C.c_int, C.c_int, C.c_float, C.c_int, _f32p, _i32p, _f32p, _f32p]
+        if hasattr(L, "dawref_linearminres_log"):
+            L.dawref_linearminres_log.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_int,
+                                                  C.c_int, C.c_int, C.c_float, C.c_int, _f32p, _i32p, _f32p, _f32p]
         L.dawref_minres.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_int,
                                     C.c_int, C.c_int, C.c_float, C.c_int, _f32p, _i32p, _f32p, _f32p,
                                     C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_int), _f32p]
@@ -475,6 +478,26 @@ def ref_linearminres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, maxiter, ty
     ref().dawref_linearminres(n1, n2, npml, h, nt, dt, ofs, type_rec, nr, mu, int(maxiter), vp, rec_tab,
                               _c(deltadr, np.float32), out)
     return out
+
+
+def ref_linearminres_log(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, maxiter, workdir, type_rec=0):
+    """The reference's sublinearminres with grid%oio = 1 run in workdir:
+    (deltade, text of the MINRES_00001.txt it writes there)."""
+    ref_init(ofs)
+    vp = _c(vp, np.float32)
+    n2, n1 = vp.shape
+    rec_tab = _c(rec_tab, np.int32)
+    nr = rec_tab.shape[1]
+    out = np.zeros((nr, nt), np.float32)
+    here = os.getcwd()
+    os.chdir(workdir)
+    try:
+        ref().dawref_linearminres_log(n1, n2, npml, h, nt, dt, ofs, type_rec, nr, mu, int(maxiter), vp, rec_tab,
+                                      _c(deltadr, np.float32), out)
+    finally:
+        os.chdir(here)
+    with open(os.path.join(workdir, "MINRES_00001.txt")) as f:
+        return out, f.read()
 
 
 def ref_minres(vp, npml, h, ofs, nt, dt, deltadr, rec_tab, mu, itnlim, type_rec=0):

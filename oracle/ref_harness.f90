@@ -129,6 +129,43 @@ contains
     deltade = inv%deltade
   end subroutine dawref_linearminres
 
+  ! sublinearminres with grid%oio = 1: as dawref_linearminres, and the
+  ! reference's MINRES writes its log MINRES_00001.txt into the working
+  ! directory (sublinearminres.f90:51-61)
+  subroutine dawref_linearminres_log(n1, n2, npml, h, nt, dt, ofs, type_rec, nrec, mu, maxiter, &
+       c0, rec_tab, deltadr, deltade) bind(C, name='dawref_linearminres_log')
+    integer(c_int), value :: n1, n2, npml, nt, ofs, type_rec, nrec, maxiter
+    real(c_float), value :: h, dt, mu
+    real(c_float), intent(in) :: c0(n1, n2), deltadr(nt, nrec)
+    integer(c_int), intent(in) :: rec_tab(nrec, 5)
+    real(c_float), intent(out) :: deltade(nt, nrec)
+    type(grid_type) :: grid
+    type(inv_type) :: inv
+    integer, allocatable :: rt(:, :)
+    grid%n1 = n1
+    grid%n2 = n2
+    grid%npml = npml
+    grid%h = h
+    grid%nt = nt
+    grid%dt = dt
+    grid%ofs = ofs
+    grid%type_rec = type_rec
+    grid%nrec = nrec
+    grid%mu = mu
+    grid%maxiter = real(maxiter)
+    grid%error = 0.
+    grid%oio = 1
+    allocate(grid%c0(n1, n2))
+    grid%c0 = c0
+    allocate(inv%deltadr(nt, nrec), inv%deltade(nt, nrec))
+    inv%deltadr = deltadr
+    inv%deltade = 0.
+    allocate(rt(nrec, 5))
+    rt = rec_tab
+    call sublinearminres(1, grid, inv, rt)
+    deltade = inv%deltade
+  end subroutine dawref_linearminres_log
+
   ! MINRES stopped after itnlim iterations, returning its scalar outputs; the
   ! shift is formed the way sublinearminres.f90:64-72 forms it.
   subroutine dawref_minres(n1, n2, npml, h, nt, dt, ofs, type_rec, nrec, mu, itnlim, &

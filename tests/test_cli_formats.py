@@ -255,3 +255,33 @@ def test_oracle_minres_trace_is_the_solve_record():
         # without a trace the solve is unchanged
         x2, hist2, info2 = O.linearminres(c.vp0, c.npml, c.h, c.ofs, c.nt, c.dt, res, c.rec, 1e-2, 23, 0.0, prec=prec)
         assert np.array_equal(x, x2) and np.array_equal(hist, hist2)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference tree not built into oracle/_ref")
+@pytest.mark.parametrize("itnlim", [35, 5])
+def test_minres_log_matches_reference_log(daw, tmp_path, itnlim):
+    """End to end against the reference itself: its sublinearminres with
+    grid%oio = 1 (built from its sources into oracle/_ref) writes
+    MINRES_00001.txt; the library's writer, fed the oracle's FP32 solve of
+    the same system (bit-identical to the reference's), writes the same bytes.
+    At 35 iterations the reference prints 1-10, 20 and 25-35: the print rule
+    of minresModule.f90:308-325 is exercised both ways."""
+    R = O.ref()
+    if not hasattr(R, "dawref_linearminres_log"):
+        pytest.skip("oracle/_ref predates the log harness: rerun oracle/build_ref.sh")
+    c = Case(nt=120)
+    res = c.rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
+    x_ref, text = O.ref_linearminres_log(c.vp0, c.npml, c.h, c.ofs, c.nt, c.dt, res, c.rec, 1e-2, itnlim,
+                                         str(tmp_path))
+    x, hist, info, tr = O.linearminres(c.vp0, c.npml, c.h, c.ofs, c.nt, c.dt, res, c.rec, 1e-2, itnlim, 0.0,
+                                       prec="f32", want_trace=True)
+    assert np.array_equal(x, x_ref)
+    blk = np.zeros(9 + 5 * itnlim)
+    blk[:9] = [info["fac"], info["Anorm"], info["Acond"], info["rnorm"], info["Arnorm"], info["ynorm"],
+               info["beta1"], info["istop"], info["itn"]]
+    blk[9:9 + 5 * info["itn"]] = tr.ravel()
+    daw.write_minres_log(str(tmp_path / "ours.txt"), res.size, itnlim, 0.0, blk)
+    assert (tmp_path / "ours.txt").read_text() == text
+    printed = [int(l.split()[0]) for l in text.splitlines() if len(l.split()) == 5 and l.split()[0].isdigit()]
+    if itnlim == 35:
+        assert 15 not in printed and 20 in printed and printed[-1] == 35
