@@ -803,6 +803,10 @@ static int autotune_tiles(Batch &B)
         std::vector<int> best(9);
         constexpr int kRounds = 3;
         for (int mode : {kBack, kReinject, kFwd}) {
+            if (mode == kFwd && B.inj_src.size() == (size_t)ns) {   // the forward sweep injects the sources
+                for (int s = 0; s < ns; ++s) set_inject(W.host[s], B.inj_src[s], zeros.as<float>(), 0, 0);
+                if (int rc = refresh_shots(W)) return rc;
+            }
             const std::vector<RecItem> *items = mode == kBack ? nullptr : mode == kReinject ? &it_reinj : &it_fwd;
             if (items)
                 if (int rc = set_items(W, *items)) return rc;
