@@ -298,7 +298,7 @@ int daw_set_timing(int on);
  * 2 x itn Hessian sweeps of the MINRES solve; 0 (default): launch step by
  * step.  Results are identical.  No reference counterpart. */
 int daw_set_graph(int on);
-/* Step-kernel tile (rows x columns): 8x128, 16x64, 16x128, 24x64, 32x64, 32x128
+/* Step-kernel tile (rows x columns): 8x128, 16x64, 16x128, 20x64, 24x64, 32x64, 32x128
  * or 64x64; rows = 0 lets daw_redatum_batch time every shape on a short probe
  * sweep per batch shape and sweep mode and keep the fastest (the default).
  * No reference counterpart: the launch shape of the autotuner. */

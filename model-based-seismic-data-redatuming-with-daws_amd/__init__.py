@@ -624,7 +624,7 @@ def filmartin2d_dev(din_dev, dout_dev, dt, flow, fhig, llow, lhig):
 
 
 def set_tile(rows, cols=64):
-    """Step-kernel tile rows x cols (8x128, 16x64, 16x128, 24x64, 32x64,
+    """Step-kernel tile rows x cols (8x128, 16x64, 16x128, 20x64, 24x64, 32x64,
     32x128, 64x64), or rows = 0 for the autotuner (default)."""
     _check(lib().daw_set_tile(int(rows), int(cols)))
 

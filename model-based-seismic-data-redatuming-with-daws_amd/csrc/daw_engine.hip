@@ -751,7 +751,7 @@ static int autotune_tiles(Batch &B)
     SweepSet &W = B.W;
     if (!g.tz_auto && !g.steps_auto) return 0;
     // candidates: {steps, rows, wavefronts across}; fixed settings are kept
-    static const int shapes[][3] = {{1, 8, 2}, {1, 16, 1}, {1, 16, 2}, {1, 24, 1}, {1, 32, 1}, {1, 32, 2},
+    static const int shapes[][3] = {{1, 8, 2}, {1, 16, 1}, {1, 16, 2}, {1, 20, 1}, {1, 24, 1}, {1, 32, 1}, {1, 32, 2},
                                     {1, 64, 1}, {2, 16, 1}, {2, 32, 1}, {2, 16, 2}};
     std::vector<LaunchCfg> cand;
     for (const auto &sh : shapes) {
@@ -1871,7 +1871,7 @@ int daw_set_tile(int rows, int cols)
     std::lock_guard<std::mutex> lk(g.mu);
     if (rows == 0) { g.tz_auto = true; return 0; }
     if (cols % 64 != 0 || !tile_shape_ok(rows, cols / 64))
-        return fail(DAW_ERR_ARG, "tile %dx%d has no compiled kernel (8x128, 16x64, 16x128, 24x64, 32x64, 32x128, 64x64)", rows, cols);
+        return fail(DAW_ERR_ARG, "tile %dx%d has no compiled kernel (8x128, 16x64, 16x128, 20x64, 24x64, 32x64, 32x128, 64x64)", rows, cols);
     if (!g.steps_auto && g.steps == 2 && !tile2_shape_ok(rows, cols / 64))
         return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel (16x64, 32x64, 16x128)", rows, cols);
     g.tz_auto = false;

@@ -579,16 +579,19 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
 //   3. velocity half-step of the owned points, vz source or snapshot
 //      re-injection, the vz(npml) = vz(npml+1) mirror, writes.
 // ---------------------------------------------------------------------------
-// Waves per SIMD the compiler must leave room for: 16x64 tiles fit eight
-// (78 SGPRs, at most 64 VGPRs, no spills) where the default register budget
-// gives seven (106 SGPRs); measured 10 % faster forward steps and 1.5 %
-// faster re-injection at 201x801 with 16 shots per launch.  The other shapes
-// stay at the default: their LDS already caps them at seven or fewer, or the
-// cap would spill (the re-injection with 16-bit snapshots).
+// Waves per SIMD the compiler must leave room for: 16x64 and 20x64 tiles fit
+// eight (78 SGPRs, at most 64 VGPRs, no spills; 20x64 still fits eight
+// workgroups' LDS per CU) where the default register budget gives seven (106
+// SGPRs); measured at 201x801 with 16 shots per launch: forward 10 % faster,
+// re-injection 1.5 %, back-propagation on 20x64 1.5 % faster than the best
+// seven-wave tile.  The other shapes stay at the default: their LDS caps them
+// at seven or fewer, or the cap would spill (re-injection on 20x64, and on
+// 16x64 with 16-bit snapshots).
 template <int MODE, int TZ, int TXW, int SNAP>
 constexpr int step_min_waves()
 {
-    return TZ == 16 && TXW == 1 && (MODE != kReinject || SNAP == kSnapF32) ? 8 : 1;
+    return TXW == 1 && ((TZ == 16 && (MODE != kReinject || SNAP == kSnapF32)) || (TZ == 20 && MODE != kReinject))
+               ? 8 : 1;
 }
 
 template <int MODE, int TZ, int TXW, int SNAP>
@@ -1387,7 +1390,7 @@ __global__ __launch_bounds__(kBlock) void k_step2(StepParams P)
 }
 
 // Tile shapes the autotuner may pick: (rows, wavefronts across x).
-#define DAW_TILE_SHAPES(X) X(8, 2) X(16, 1) X(16, 2) X(24, 1) X(32, 1) X(32, 2) X(64, 1)
+#define DAW_TILE_SHAPES(X) X(8, 2) X(16, 1) X(16, 2) X(20, 1) X(24, 1) X(32, 1) X(32, 2) X(64, 1)
 
 void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st)
 {

@@ -19,7 +19,7 @@ def test_tile_and_steps_knobs(daw):
         for rows, cols in [(12, 64), (16, 96), (8, 64), (64, 128), (-16, 64)]:
             assert L.daw_set_tile(rows, cols) == -1, (rows, cols)
             assert b"no compiled kernel" in L.daw_last_error()
-        for rows, cols in [(8, 128), (16, 64), (16, 128), (24, 64), (32, 64), (32, 128), (64, 64)]:
+        for rows, cols in [(8, 128), (16, 64), (16, 128), (20, 64), (24, 64), (32, 64), (32, 128), (64, 64)]:
             assert L.daw_set_tile(rows, cols) == 0, (rows, cols)
         assert L.daw_set_steps(3) == -1
         # 64x64 has no two-step kernel: fixing both must be refused either way round
@@ -42,7 +42,7 @@ def test_get_launch_mode_range(daw):
         assert L.daw_get_launch(mode, C.byref(s), C.byref(r), C.byref(c)) == -1
     for mode in (0, 1, 2):
         assert L.daw_get_launch(mode, C.byref(s), C.byref(r), C.byref(c)) == 0
-        assert s.value in (1, 2) and r.value in (8, 16, 32, 64) and c.value in (64, 128)
+        assert s.value in (1, 2) and r.value in (8, 16, 20, 24, 32, 64) and c.value in (64, 128)
 
 
 def test_batched_entry_points_reject_bad_arguments(daw):

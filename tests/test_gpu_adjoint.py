@@ -64,7 +64,7 @@ def test_transpose_tile_shapes(gpu_daw):
     d = rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
     outs = []
     try:
-        for tile in [(8, 128), (16, 64), (16, 128), (24, 64), (32, 64), (32, 128), (64, 64)]:
+        for tile in [(8, 128), (16, 64), (16, 128), (20, 64), (24, 64), (32, 64), (32, 128), (64, 64)]:
             gpu_daw.set_tile(*tile)
             outs.append(gpu_daw.submodeling_transpose(c.vp, c.npml, c.h, 1, c.nt, c.dt, d, c.src, c.rec))
     finally:

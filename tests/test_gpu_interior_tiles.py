@@ -16,7 +16,7 @@ from cases import Case, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-TILES = [(8, 128), (16, 64), (16, 128), (24, 64), (32, 64), (32, 128), (64, 64)]
+TILES = [(8, 128), (16, 64), (16, 128), (20, 64), (24, 64), (32, 64), (32, 128), (64, 64)]
 
 
 def _interior_tiles(c, tz, tx, rows_blocked):

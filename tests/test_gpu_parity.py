@@ -92,7 +92,7 @@ def test_sublinearminres(gpu_daw):
     assert abs(info["fac"] / i64["fac"] - 1) < 1e-5
 
 
-@pytest.mark.parametrize("tile", [(8, 128), (16, 64), (16, 128), (24, 64), (32, 64), (32, 128), (64, 64)])
+@pytest.mark.parametrize("tile", [(8, 128), (16, 64), (16, 128), (20, 64), (24, 64), (32, 64), (32, 128), (64, 64)])
 def test_tile_shapes_bitexact(gpu_daw, tile):
     """Every step-kernel tile shape gives the reference's fields bit for bit."""
     c = Case(nz=45, nx=150, nt=160)

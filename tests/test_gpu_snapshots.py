@@ -36,7 +36,7 @@ def test_esource_data_16bit_bitexact(snap16, snap, ofs):
     print(snap, "rel vs f64", rel_l2(a, f64), flush=True)
 
 
-@pytest.mark.parametrize("tile", [(8, 128), (24, 64), (32, 64), (64, 64)])
+@pytest.mark.parametrize("tile", [(8, 128), (20, 64), (24, 64), (32, 64), (64, 64)])
 def test_esource_data_16bit_tile_invariant(snap16, tile):
     c = Case(nz=45, nx=150, nt=160)
     res = c.rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
