@@ -32,8 +32,11 @@
 //                 (subresample_hicks.f90:25-83), which the reference ships
 //                 but does not call.
 //   --par FILE    parameter file (default redatum.par).
-//   --dry-run     read and check every input, print the geometry tables, stop
-//                 before any GPU work.
+//   --dry-run     read and check every input (the par file, wavelet, models,
+//                 acquisition files, and every shot's dataobs_org_ or
+//                 wavefield_exd file the mode will read), print the geometry
+//                 tables, stop before any GPU work.  Every run makes the same
+//                 checks before its first shot.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -350,6 +353,29 @@ void mode2(Run &R)
     }
 }
 
+// Every per-shot input file the mode reads, present and at least as long as
+// the record it needs, before any GPU work: a missing file fails the run at
+// once, not after the shots before it.  The reference finds out at the shot.
+void preflight(const Run &R)
+{
+    const char *stem = nullptr;
+    size_t need = 0;
+    if (R.p.mode == 1 || R.p.mode == 3) { stem = "dataobs_org_"; need = R.ntr(); }
+    if (R.p.mode == 2) { stem = "wavefield_exd"; need = R.plane() * R.p.nt; }
+    if (!stem) return;
+    for (int is = 1; is <= R.acq.nsrc; ++is) {
+        const std::string path = R.bin(stem, is);
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) die("cannot open " + path);
+        const int ok = std::fseek(f, 0, SEEK_END);
+        const long len = std::ftell(f);
+        std::fclose(f);
+        if (ok != 0 || len < 0 || (size_t)len < need * sizeof(float))
+            die(path + ": expected " + std::to_string(need) + " REAL*4 values, found " +
+                std::to_string(len < 0 ? 0 : len / 4));
+    }
+}
+
 void usage()
 {
     std::fprintf(stderr,
@@ -422,8 +448,9 @@ int main(int argc, char **argv)
         dump("src_tab", R.acq.src_tab, R.acq.nsrc);
         dump("rec_tab", R.acq.rec_tab, R.acq.nrec);
         if (!datum_file.empty()) dump("dat_tab", datum.rec_tab, datum.nrec);
-        return 0;
     }
+    preflight(R);
+    if (dry) return 0;
     switch (p.mode) {
     case 0: mode0(R); break;
     case 1:

@@ -212,6 +212,15 @@ def test_cli_dry_run_tables(tmp_path):
     (tmp_path / "redatum.par").write_text(
         f"1\nvtrue.bin vinit.bin\nacq.txt\nw.bin\n3 25 1 5\n{n1} {n2} {npml} {h} {nt} 0.001 0.002\n0 0\n{ofs}\n"
         "1.0e-2\n1\n0.0 5\n")
+    # mode 1 reads each shot's observed data: --dry-run checks they are all there
+    r = subprocess.run([CLI, "--dry-run", "--datum", "dat.txt"], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode != 0 and "dataobs_org_00001.bin" in r.stderr
+    for i in (1, 2):
+        np.zeros((len(rec), nt), np.float32).tofile(tmp_path / f"dataobs_org_{i:05d}.bin")
+    np.zeros((len(rec), nt - 1), np.float32).tofile(tmp_path / "dataobs_org_00002.bin")
+    r = subprocess.run([CLI, "--dry-run", "--datum", "dat.txt"], cwd=str(tmp_path), capture_output=True, text=True)
+    assert r.returncode != 0 and "dataobs_org_00002.bin: expected 900" in r.stderr
+    np.zeros((len(rec), nt), np.float32).tofile(tmp_path / "dataobs_org_00002.bin")
     r = subprocess.run([CLI, "--dry-run", "--datum", "dat.txt"], cwd=str(tmp_path), capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     tabs, cur = {}, None
