@@ -516,14 +516,16 @@ def redatum_batch(c0, npml, h, ofs, nt, dt, s_dev, src_tab, rec_tab, dat_tab, do
 
 def redatum_uneven(c0, npml, h, ofs, dt, s_dev, x0, n2s, nts, src_tab, rec_tabs, dobs_dev, mu, maxiter, rtol=0.0,
                    dat_tabs=None, datum_dev=None, datae_dev=None, deltade_dev=None, want_hist=True, max_batch=0,
-                   hbm_budget_bytes=0.0, type_src=0, band=None):
+                   hbm_budget_bytes=0.0, type_src=0, band=None, method=1, minres_log=None):
     """Redatuming of shots on their own subdomains (columns x0[s] .. x0[s]+n2s[s]-1
     of c0), receivers and nt, advanced together.  src_tab (5, nshots) and the
     per-shot tables rec_tabs[s] (5, nrec_s) / dat_tabs[s] (5, ndat_s) are in
     subdomain coordinates (subacqui on the subdomain).  dobs_dev and the
     outputs are flat float32 cuda tensors holding shot s's [nrec_s, nt_s]
     (datum: [ndat_s, nt_s]) block after shot s-1's.  Returns the MINRES
-    histories (nshots, maxiter) when want_hist."""
+    histories (nshots, maxiter) when want_hist.  method=3 runs the Symes
+    variant (sub_redatum_DAW.f90:157-228); minres_log, float64 (nshots, 9 +
+    5*maxiter), receives the MINRES log blocks."""
     import torch
     _ensure_hicks(ofs)
     c0 = _f32(c0)
@@ -570,6 +572,11 @@ def redatum_uneven(c0, npml, h, ofs, dt, s_dev, x0, n2s, nts, src_tab, rec_tabs,
     if band is not None:
         a.filter = 1
         a.flow, a.fhig, a.llow, a.lhig = (float(b) for b in band)
+    if minres_log is not None and (minres_log.dtype != np.float64 or not minres_log.flags.c_contiguous or
+                                   minres_log.shape != (ns, 9 + 5 * max(int(maxiter), 1))):
+        raise ValueError("minres_log must be a C-contiguous float64 array of (nshots, 9 + 5*maxiter)")
+    a.minres_log_host = minres_log.ctypes.data if minres_log is not None else None
+    a.method = int(method)
     _check(lib().daw_redatum_uneven(C.byref(a)))
     return hist
 

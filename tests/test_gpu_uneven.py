@@ -69,7 +69,11 @@ def test_redatum_uneven_vs_oracle(gpu_daw):
             assert np.allclose(hist[k], hh, rtol=1e-4)
 
 
-def test_uneven_of_even_shots_equals_even_batch(gpu_daw):
+@pytest.mark.parametrize("method", [1, 3])
+def test_uneven_of_even_shots_equals_even_batch(gpu_daw, method):
+    """Shots that all span the whole model are an even batch: the uneven entry
+    point gives the even one's results bit for bit, for the DAW (mode 1) and
+    the Symes (mode 3) variants, MINRES log records included."""
     import torch
     c = Case(nt=200)
     ns, its, mu = 3, 3, 1e-2
@@ -81,13 +85,23 @@ def test_uneven_of_even_shots_equals_even_batch(gpu_daw):
     nr, nd = c.rec.shape[1], c.dat.shape[1]
     d1 = torch.zeros((ns, nd, c.nt), dtype=torch.float32, device="cuda")
     x1 = torch.zeros((ns, nr, c.nt), dtype=torch.float32, device="cuda")
+    l1 = np.zeros((ns, 9 + 5 * its))
+    l2 = np.zeros((ns, 9 + 5 * its))
+    print("uneven vs even, method", method, flush=True)
     h1 = gpu_daw.redatum_batch(c.vp0, c.npml, c.h, 1, c.nt, c.dt, s_dev, src, c.rec, c.dat, dobs, mu, its,
-                               datum_dev=d1, deltade_dev=x1)
+                               datum_dev=d1, deltade_dev=x1, method=method, minres_log=l1)
     d2 = torch.zeros(ns * nd * c.nt, dtype=torch.float32, device="cuda")
     x2 = torch.zeros(ns * nr * c.nt, dtype=torch.float32, device="cuda")
     h2 = gpu_daw.redatum_uneven(c.vp0, c.npml, c.h, 1, c.dt, s_dev, [0] * ns, [c.n2] * ns, [c.nt] * ns, src,
                                 [c.rec] * ns, dobs.reshape(-1), mu, its, dat_tabs=[c.dat] * ns, datum_dev=d2,
-                                deltade_dev=x2)
+                                deltade_dev=x2, method=method, minres_log=l2)
+    assert np.abs(d1.cpu().numpy()).max() > 0
     assert np.array_equal(d1.cpu().numpy().ravel(), d2.cpu().numpy())
     assert np.array_equal(x1.cpu().numpy().ravel(), x2.cpu().numpy())
     assert np.array_equal(h1, h2)
+    assert np.array_equal(l1, l2) and np.all(l1[:, 8] == its)
+    if method == 3:   # the Symes solve's right-hand side is the observed data itself (:175-179)
+        for s in range(ns):
+            x, hist, info = O.linearminres(c.vp0, c.npml, c.h, 1, c.nt, c.dt, dobs_np[s][:, ::-1].copy(), c.rec, mu,
+                                           its, prec="f64")
+            assert rel_l2(x1[s].cpu().numpy(), x) <= 1e-4, s
