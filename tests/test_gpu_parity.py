@@ -5,7 +5,7 @@ Single-precision propagators, Hicks gathers and resampling are compared
 bit-exactly: the kernels perform the reference's IEEE operations in the
 reference's order with FP contraction off.  MINRES uses FP64 dot products and
 Lanczos scalars where the reference uses FP32, so its solution is compared
-within 1e-4 relative L2 against both the FP32
+within 1e-4 relative L2 (the bar for MINRES solutions) against both the FP32
 oracle and the FP64 oracle.
 """
 import numpy as np

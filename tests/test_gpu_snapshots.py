@@ -4,7 +4,7 @@ scale per 64-column segment of each snapshot row).
 The Hessian-type operator with 16-bit snapshots is compared bit for bit with
 the oracle's operator using the same codec (pinned in test_snapshot_codec),
 and the batched redatuming pipeline with 16-bit snapshots against the FP64
-oracle of the plain FP32 method (north-star bound 1e-4 relative L2 for FP16;
+oracle of the plain FP32 method (bound 1e-4 relative L2 for FP16, the bar for mixed precision;
 BF16, 8 significant bits, is held to 1e-3 and its error printed).
 """
 import numpy as np
