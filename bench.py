@@ -473,7 +473,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); shots are independent and would "
-              "shard across GPUs without any collective, which this build does not exercise.", file=sys.stderr)
+              "shard across GPUs without any collective; this program drives one GPU.", file=sys.stderr)
         sys.exit(2)
     if args.workflow:
         return main_workflow(args, _pkg()[1])
