@@ -140,7 +140,7 @@ int daw_write_minres_log(const char *path, int n, int itnlim, float rtol, const 
 int daw_subresample(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
 
 /* The same on device traces (xin, xout device pointers, [n2][n1in] and
- * [n2][n1out]). */
+ * [n2][n1out], not overlapping). */
 int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
 
 /* filmartin2d(deltadeo,nr,nt,dt,flow,fhig,llow,lhig,deltade),

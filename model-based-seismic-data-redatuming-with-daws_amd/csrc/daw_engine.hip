@@ -1358,6 +1358,12 @@ int daw_subresample(const float *xin, float *xout, int n1in, int n2, int n1out, 
 int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out)
 {
     std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_resample(n1in, n2, n1out, d1in, d1out)) return rc;
+    // each output reads up to nine inputs of its trace: the two blocks must not overlap
+    const char *a0 = reinterpret_cast<const char *>(xin), *b0 = reinterpret_cast<const char *>(xout);
+    const char *a1 = a0 + (size_t)n2 * n1in * sizeof(float), *b1 = b0 + (size_t)n2 * n1out * sizeof(float);
+    if (xin && xout && (size_t)n2 * n1out > 0 && a0 < b1 && b0 < a1)
+        return fail(DAW_ERR_ARG, "subresample_dev: input and output traces overlap");
     return time_resample_dev(xin, xout, n1in, n2, n1out, d1in, d1out);
 }
 

@@ -76,3 +76,12 @@ def test_subresample_rejects_bad_arguments(daw):
     # nothing to do is not an error
     assert L.daw_subresample(x, y, 10, 0, 5, 1e-3, 1e-3) == 0
     assert L.daw_subresample(x, y, 10, 1, 0, 1e-3, 1e-3) == 0
+
+
+def test_subresample_dev_rejects_overlap(daw):
+    L = daw.lib()
+    buf = np.zeros(1000, np.float32)
+    p = buf.ctypes.data
+    # output block inside the input block: refused before any device work
+    assert L.daw_subresample_dev(p, p + 400, 100, 4, 50, 1e-3, 2e-3) == -1
+    assert b"overlap" in L.daw_last_error()
