@@ -223,13 +223,26 @@ int read_acqui(const char *path, int &nsrc, int &nrec, std::vector<float> &src_z
     if (!rd.read(2, t, err)) return fail_at("missing nsrc nrec");
     nsrc = nrec = 0;
     if (!parse_int(t[0], nsrc) || !parse_int(t[1], nrec) || nsrc < 0 || nrec < 0) return fail_at("bad nsrc nrec");
-    // then one "z x" per source and per receiver (subacqui.f90:32-33, 48-49)
+    // then one "z x" per source and per receiver (subacqui.f90:32-33, 48-49).
+    // subacqui opens the file afresh and skips one record (:30-31), so the
+    // points start on the second line even if the counts ran onto it
+    std::ifstream in2(path);
+    std::string first;
+    if (!in2 || !std::getline(in2, first)) return fail_at("cannot reread");
+    ListReader rp(in2);
     src_zx.assign((size_t)2 * nsrc, 0.0f);
     rec_zx.assign((size_t)2 * nrec, 0.0f);
     for (int i = 0; i < nsrc + nrec; ++i) {
         float *zx = i < nsrc ? &src_zx[(size_t)2 * i] : &rec_zx[(size_t)2 * (i - nsrc)];
-        if (!rd.read(2, t, err)) return fail_at(i < nsrc ? "missing source coordinates" : "missing receiver coordinates");
-        if (!parse_real(t[0], zx[0]) || !parse_real(t[1], zx[1])) return fail_at("bad coordinate");
+        if (!rp.read(2, t, err)) {
+            err = std::string(path) + " line " + std::to_string(rp.line() + 1) + ": " +
+                  (i < nsrc ? "missing source coordinates" : "missing receiver coordinates");
+            return -1;
+        }
+        if (!parse_real(t[0], zx[0]) || !parse_real(t[1], zx[1])) {
+            err = std::string(path) + " line " + std::to_string(rp.line() + 1) + ": bad coordinate";
+            return -1;
+        }
     }
     return 0;
 }

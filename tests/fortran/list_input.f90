@@ -1,6 +1,7 @@
 ! Reads a parameter file or an acquisition file with Fortran list-directed
 ! input, one READ per record group as the reference's readers do
-! (subreadpar.f90:10-20; main_redatum_DAW.f90:121 and subacqui.f90:29-57),
+! (subreadpar.f90:10-20; main_redatum_DAW.f90:121, then subacqui.f90:29-57 on
+! the reopened file),
 ! and prints what it read, so that the library's C++ reader can be checked
 ! against a Fortran runtime.  Output: one value per line; reals as ES17.9
 ! (enough digits to recover a REAL exactly); "ERROR <iostat>" on a failed READ.
@@ -58,6 +59,10 @@ program list_input
      read(10, *, iostat=ios) nsrc, nrec
      call chk(ios)
      print '(a,i0,1x,i0)', 'counts ', nsrc, nrec
+     ! the points are read after reopening the file and skipping one record
+     close(10)
+     open(10, file=path, status='old')
+     read(10, *)
      do i = 1, nsrc + nrec
         read(10, *, iostat=ios) r1, r2
         call chk(ios)

@@ -133,6 +133,8 @@ ACQ_TEXTS = {
     "plain": "2 3\n10. 200.\n10. 500.\n15. 100.\n15. 300.\n15. 600.\n",
     "icode": "1 2 extra\n450. 100. 0\n50. 0. 1\n50.,50., 1\n",
     "split": "1 2\n\n450.\n100.\n50. 0.\n  5.0d1  1.0E2\n",
+    # counts run onto the second line; the points still start there
+    "counts": "1\n2 450. 100.\n50. 0.\n50., 50.\n",
 }
 
 
