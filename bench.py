@@ -7,13 +7,17 @@ reference's redatuming workflow (sub_redatum_DAW.f90 modes 1 and 2) for all
 64 shots.  Observed data are modelled beforehand in the true model with 1 %
 seeded noise (untimed input synthesis, the reference's mode 0).
 
-One step = one redatuming pass over all shots: background forward, residual,
-fac, 30 MINRES iterations (each one back-propagation + re-injection sweep
-pair), the source-extension sweep pair, the datum gathers and their
-band-pass (filmartin2d, as mode 2 writes dataobs_redatum), with inputs
-resident in HBM.  The line also reports the plain one-shot-at-a-time GPU path,
-the roofline of the dominant step kernel and the reference's own CPU path
-timed on a bounded sample on this host.
+One step = one redatuming pass over one batch of 16 shots advanced together
+(the launch batch: the 64-shot line is four such batches, since the 64 FP32
+snapshot stores need 371 GB and a 16-shot launch keeps its wave state in the
+last-level cache): background forward, residual, fac, 30 MINRES iterations
+(each one back-propagation + re-injection sweep pair), the source-extension
+sweep pair, the datum gathers and their band-pass (filmartin2d, as mode 2
+writes dataobs_redatum), with inputs resident in HBM.  Steps walk the
+batches of the line in turn.  The line also reports the plain
+one-shot-at-a-time GPU path, the roofline of the dominant step kernel (HIP
+events on the library's stream) and the reference's own CPU path timed on
+this host's cores (one shot per core, truncated nt, scaled).
 """
 import argparse
 import json
@@ -24,6 +28,7 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+T_PROC0 = time.perf_counter()     # the time budget counts from process start (first torch import included)
 sys.path.insert(0, ROOT)
 
 
@@ -180,45 +185,131 @@ def main_uneven(args, daw, syn):
     _emit(out)
 
 
-def cpu_baseline(P, nt_sample):
-    """The reference's own CPU path (oracle/_ref, built from its sources) on
-    one shot: one submodeling_adjoint_esource_data call (back-propagation +
-    re-injection, the operator MINRES applies) truncated to nt_sample steps,
-    scaled to a redatumed shot of 7 + 2*itn sweeps at the full nt."""
+def _cpu_worker(path):
+    """One MPI rank's work in the reference's layout (para_range.f90:11-22:
+    a shot per rank): the reference's own per-shot chain of mode 1 + mode 2
+    (sub_redatum_DAW.f90:82-155, :230-240) -- submodeling, subresiduals,
+    sublinearminres, submodeling_adjoint_esource, the wavefield sum,
+    subwavefieldresample and filmartin2d -- on one shot, timed after the
+    process-wide kwsinc2d set-up.  Runs in a child process that never touches
+    the GPU; prints one JSON object."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    w = P["w"]
-    rng = np.random.default_rng(0)
-    res = rng.standard_normal((P["rec"].shape[1], nt_sample)).astype(np.float32)
+    z = np.load(path, allow_pickle=False)
+    c0, s, src, rec, dat, dobs = z["c0"], z["s"], z["src"], z["rec"], z["dat"], z["dobs"]
+    npml, ofs, nt, maxiter = (int(z[k]) for k in ("npml", "ofs", "nt", "maxiter"))
+    h, dt, mu = np.float32(z["h"]), np.float32(z["dt"]), np.float32(z["mu"])
+    band = tuple(float(v) for v in z["band"])
     kind = "reference" if O.ref_available() else "port"
-    if kind == "reference":
-        fn = lambda: O.ref_esource_data(P["c0"], w.npml, P["h"], w.ofs, nt_sample, P["dt"], res, P["rec"])
-    else:
-        fn = lambda: O.esource_data(P["c0"], w.npml, P["h"], w.ofs, nt_sample, P["dt"], res, P["rec"])
-    # the reference writes progress text to stdout (kwsinc.f90:224, minresModule.f90:512);
-    # keep this process's stdout for the one JSON line
     sys.stdout.flush()
     saved = os.dup(1)
     devnull = os.open(os.devnull, os.O_WRONLY)
-    os.dup2(devnull, 1)
+    os.dup2(devnull, 1)            # the reference's progress text (kwsinc.f90:224, minresModule.f90:512)
     try:
         if kind == "reference":
-            O.ref_init(w.ofs)
-        t0 = time.perf_counter()
-        fn()
+            O.ref_init(ofs)
+            t0 = time.perf_counter()
+            dr, wfr = O.ref_forward(c0, npml, h, ofs, nt, dt, s, src, rec)
+            res = O.ref_residuals(dobs, dr)
+            x = O.ref_linearminres(c0, npml, h, ofs, nt, dt, res, rec, mu, maxiter)
+            _, wfe = O.ref_esource(c0, npml, h, ofs, nt, dt, x, rec)
+            wfr += wfe
+            O.ref_filmartin2d(O.ref_resample(wfr, npml, dat, ofs), dt, *band)
+        else:
+            t0 = time.perf_counter()
+            dr, wfr = O.forward(c0, npml, h, ofs, nt, dt, s, src, rec, want_wavefield=True)
+            res = O.residuals(dobs, dr)
+            x, _, _ = O.linearminres(c0, npml, h, ofs, nt, dt, res, rec, mu, maxiter)
+            _, wfe = O.esource(c0, npml, h, ofs, nt, dt, x, rec, want_wavefield=True)
+            wfr += wfe
+            O.filmartin2d(O.resample(wfr, npml, dat, ofs), dt, *band)
         t = time.perf_counter() - t0
     finally:
         os.dup2(saved, 1)
         os.close(saved)
         os.close(devnull)
-    sweeps = 7 + 2 * w.maxiter
-    t_shot = t / (2.0 * nt_sample) * sweeps * w.nt
+    _emit({"seconds": t, "kind": kind})
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(P, nt_sample, cores=None, timeout=240):
+    """The reference's CPU path on this host's cores, one shot per core (the
+    para_range layout of sub_redatum_DAW.f90:30-34): `cores` child processes
+    each run one shot's full mode-1 + mode-2 chain (7 + 2*itn sweeps) with the
+    reference built from its own sources (oracle/_ref), truncated to the first
+    nt_sample time steps of the shot's observed data, all at once.  Per-shot
+    seconds scale linearly in nt to the full trace length (every sweep and
+    every MINRES vector operation is O(nt)); aggregate shots/hour = cores *
+    3600 / per-shot seconds."""
+    import resource
+    import subprocess
+    import tempfile
+    w = P["w"]
+    if cores is None:
+        try:
+            avail = len(os.sched_getaffinity(0))
+        except AttributeError:
+            avail = os.cpu_count() or 1
+        cores = max(1, min(16, avail))
+    nsh = P["nsh"]
+    idx = [int(round(i)) for i in np.linspace(0, nsh - 1, cores)]
+    d = P["dobs"]
+    dobs = d[:, :, :nt_sample].cpu().numpy() if hasattr(d, "cpu") else np.ascontiguousarray(d[:, :, :nt_sample])
+    work = tempfile.mkdtemp(prefix="daw_cpu_", dir=os.environ.get("TMPDIR", "/tmp"))
+    hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+
+    def big_stack():     # MINRES keeps its work vectors on the stack (minresModule.f90:29)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = []
+    try:
+        for k, s in enumerate(idx):
+            path = os.path.join(work, f"shot{k}.npz")
+            np.savez(path, c0=P["c0"], s=P["s"][:nt_sample], src=np.ascontiguousarray(P["src"][:, s]),
+                     rec=P["rec"], dat=P["dat"], dobs=dobs[s], npml=w.npml, ofs=w.ofs, nt=nt_sample,
+                     maxiter=w.maxiter, h=P["h"], dt=P["dt"], mu=np.float32(w.mu), band=np.array(w.band))
+            procs.append(subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", path],
+                                          stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, env=env,
+                                          preexec_fn=big_stack, text=True))
+        t_wall0 = time.perf_counter()
+        outs = []
+        for pr in procs:
+            out, _ = pr.communicate(timeout=max(1.0, timeout - (time.perf_counter() - t_wall0)))
+            if pr.returncode != 0:
+                raise RuntimeError(f"cpu worker exited with {pr.returncode}")
+            outs.append(json.loads(out.strip().splitlines()[-1]))
+        wall = time.perf_counter() - t_wall0
+    finally:
+        for pr in procs:
+            if pr.poll() is None:
+                pr.kill()
+                pr.wait()
+        import shutil
+        shutil.rmtree(work, ignore_errors=True)
+    secs = np.array([o["seconds"] for o in outs])
+    scale = w.nt / nt_sample
+    t_shot = float(np.mean(secs)) * scale
     n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml
-    return dict(value=3600.0 / t_shot, unit="shots/hour", cores=1, kind=kind,
-                sample=f"one submodeling_adjoint_esource_data call (2 sweeps) of one shot at nt={nt_sample} "
-                       f"took {t:.2f} s on 1 core; scaled x{sweeps}/2 sweeps x{w.nt}/{nt_sample} steps "
-                       f"= {t_shot:.0f} s per redatumed shot",
-                gpts_per_s=n1e * n2e * 2 * nt_sample / t / 1e9)
+    sweeps = 7 + 2 * w.maxiter
+    return dict(value=round(cores * 3600.0 / t_shot, 2), unit="shots/hour", cores=cores, kind=outs[0]["kind"],
+                cpu_model=_cpu_model(), per_shot_s=round(t_shot, 1),
+                sample=f"{cores} processes at once, one shot each (shots {idx[0] + 1}..{idx[-1] + 1} spread over the "
+                       f"line), the reference's full per-shot chain ({sweeps} sweeps, {w.maxiter} MINRES its, "
+                       f"resample + filmartin2d) truncated to nt={nt_sample} of {w.nt}: "
+                       f"{float(np.min(secs)):.1f}-{float(np.max(secs)):.1f} s per shot ({wall:.1f} s wall), "
+                       f"scaled x{scale:g} to the full nt = {t_shot:.0f} s per shot",
+                gpts_per_s=round(cores * n1e * n2e * sweeps * nt_sample / float(np.mean(secs)) / 1e9, 3))
 
 
 def f64_check(P, datum, hist, nshots):
@@ -338,6 +429,10 @@ def main_workflow(args, syn):
         t_model = run(0, "acq_org.txt", 0)
         for i in range(1, nsh + 1):
             os.replace(os.path.join(work, f"dataobs_{i:05d}.bin"), os.path.join(work, f"dataobs_org_{i:05d}.bin"))
+        # the observed data's first cpu_nt samples, for the CPU leg (dataobs(nt,nrec) per shot)
+        nts = min(args.cpu_nt, w.nt)
+        dobs_cpu = np.stack([np.fromfile(os.path.join(work, f"dataobs_org_{i:05d}.bin"), np.float32)
+                             .reshape(len(w.rec_x), w.nt)[:, :nts] for i in range(1, nsh + 1)])
         t_fused = run(1, "acq_org.txt", 0, "--datum", "acq_new.txt", "--batch", str(args.workflow_batch))
         fused = {i: np.fromfile(os.path.join(work, f"dataobs_redatum_{i:05d}.bin"), np.float32) for i in range(1, kp + 1)}
         plain = None
@@ -363,10 +458,13 @@ def main_workflow(args, syn):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        P = dict(w=w, c0=c0, h=np.float32(w.h), dt=np.float32(w.dt),
-                 rec=O.acqui_table(w.rec_z, w.rec_x, w.h, w.npml, w.ofs))
-        _log("reference CPU path sample")
-        cpu = cpu_baseline(P, min(args.cpu_nt, w.nt))
+        P = dict(w=w, c0=c0, h=np.float32(w.h), dt=np.float32(w.dt), nsh=nsh, dobs=dobs_cpu,
+                 s=syn.ricker(w.nt, w.dt, w.f0, w.h),
+                 src=O.acqui_table(w.src_z[:nsh], w.src_x[:nsh], w.h, w.npml, w.ofs),
+                 rec=O.acqui_table(w.rec_z, w.rec_x, w.h, w.npml, w.ofs),
+                 dat=O.acqui_table(w.dat_z, w.dat_x, w.h, w.npml, w.ofs))
+        _log("reference CPU path on the host cores")
+        cpu = cpu_baseline(P, nts, timeout=args.cpu_timeout)
     _emit({"metric": "redatumed shots/hour through dawredatum (modes 1+2 fused by --datum, files in and out)",
            "value": round(value, 1), "unit": "shots/h", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": round(1000.0 * t_fused, 1), "higher_is_better": True, "scaling": "weak",
@@ -443,14 +541,19 @@ def main():
     os.dup2(2, 1)
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=4, help="timed steps (batches of --batch shots)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16, help="shots per step (advanced together)")
+    ap.add_argument("--time-budget", type=float, default=540.0,
+                    help="seconds after which the plain-path and CPU legs are left out")
+    ap.add_argument("--cpu-timeout", type=float, default=180.0)
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--shots", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--maxiter", type=int, default=None)
-    ap.add_argument("--plain-shots", type=int, default=2)
-    ap.add_argument("--cpu-nt", type=int, default=1000)
+    ap.add_argument("--plain-shots", type=int, default=1)
+    ap.add_argument("--cpu-nt", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0)
     ap.add_argument("--max-batch", type=int, default=0)
@@ -471,6 +574,8 @@ def main():
     ap.add_argument("--resample", action="store_true",
                     help="time the dt -> dto time resampler on the workload's gathers instead")
     args = ap.parse_args()
+    if args.cpu_worker:
+        return _cpu_worker(args.cpu_worker)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); shots are independent and would "
               "shard across GPUs without any collective; this program drives one GPU.", file=sys.stderr)
@@ -490,36 +595,57 @@ def main():
     daw.set_graph(args.graph)
     if args.config in (3, 5):
         return main_uneven(args, daw, syn)
+    t_start = T_PROC0
     _log("setup: modelling observed data")
     P = setup(daw, syn, args.config, shots=args.shots, nt=args.nt, maxiter=args.maxiter)
     w = P["w"]
+    nsh = P["nsh"]
     budget = args.hbm_budget_gb * 1e9
+    # one step = one redatuming pass over one batch of shots advanced together
+    # (the launch batch: 16 shots at 201x801, whose wave state stays in the
+    # last-level cache); steps walk the batches of the shot line in turn
+    bsz = max(1, min(args.batch, nsh))
+    nbat = (nsh + bsz - 1) // bsz
+    batches = []
+    for b in range(nbat):
+        a, e = b * bsz, min(nsh, (b + 1) * bsz)
+        batches.append(dict(src=np.ascontiguousarray(P["src"][:, a:e]), dobs=P["dobs"][a:e],
+                            datum=torch.zeros((e - a, P["dat"].shape[1], w.nt), dtype=torch.float32, device="cuda"),
+                            shots=(a, e)))
+
+    def step(k):
+        b = batches[k % nbat]
+        return daw.redatum_batch(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], P["s_dev"], b["src"], P["rec"],
+                                 P["dat"], b["dobs"], w.mu, w.maxiter, 0.0, datum_dev=b["datum"], max_batch=bsz,
+                                 hbm_budget_bytes=budget, band=w.band)
+
     # first-use warm-up of the library (kernel loading, stream creation): one short shot
     Q = setup(daw, syn, args.config, shots=1, nt=64, maxiter=1)
     run_redatum(daw, Q)
     for i in range(args.warmup):
         _log(f"warmup step {i + 1}/{args.warmup}")
-        run_redatum(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+        step(i)
     daw.reset_stats()
     daw.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    hists = {}
     for i in range(args.steps):
-        _log(f"timed step {i + 1}/{args.steps}: {P['nsh']} shots")
-        datum, hist = run_redatum(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+        _log(f"timed step {i + 1}/{args.steps}: shots {batches[i % nbat]['shots'][0] + 1}.."
+             f"{batches[i % nbat]['shots'][1]}")
+        hists[i % nbat] = step(i)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     daw.set_timing(False)
     st = daw.stats()
     launch = {m: daw.get_launch(i) for i, m in enumerate(("forward", "back-propagation", "re-injection"))}
-    tile = list(daw.get_tile())
-    nsh = P["nsh"]
-    shots_per_hour = nsh * args.steps / elapsed * 3600.0
+    shots_timed = sum(batches[i % nbat]["shots"][1] - batches[i % nbat]["shots"][0] for i in range(args.steps))
+    shots_per_hour = shots_timed / elapsed * 3600.0
     gpts = st["point_updates"] / elapsed / 1e9
     n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml
-    mb = st["max_batch_used"]
     # dominant kernel: back-propagation or re-injection sweep; one launch
-    # advances all shots of a sub-batch by launch_steps time steps
+    # advances all shots of a batch by launch_steps time steps; its average
+    # duration comes from HIP events around the sweeps on the library's stream
     dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
     launch_steps = launch[("forward", "back-propagation", "re-injection")[dom]][0]
     kname = "k_step2" if launch_steps == 2 else "k_step"
@@ -531,47 +657,73 @@ def main():
     # one 4 B scale per 64 columns)
     snap_b = 12.0 if args.snapshots == "fp32" else 6.0 + 12.0 / 64.0
     bytes_per_shot_launch = 36.0 * n1e * n2e + launch_steps * snap_b * w.nz * w.nx
-    shots_per_launch = nsh / int(np.ceil(nsh / mb))
+    shots_per_launch = st["max_batch_used"]
     alg_bytes = bytes_per_shot_launch * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
     snap_code = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snapshots]
     targs = (f"{dom},{snap_code},{dcfg[1]},{dcfg[2] // 64}" if launch_steps == 2     # k_step2<MODE, SNAP, TZ, NC>
              else f"{dom},{dcfg[1]},{dcfg[2] // 64},{snap_code}")                     # k_step<MODE, TZ, TXW, SNAP>
-    traffic_key = f"{kname}<{targs}> {w.nz}x{w.nx} {int(round(shots_per_launch))} shots per launch"
+    traffic_key = f"{kname}<{targs}> {w.nz}x{w.nx} {int(shots_per_launch)} shots per launch"
     traffic = measured_traffic(traffic_key)
-    # quality: redatumed (unfiltered) gathers vs data modelled at the datum in the true model
-    err = float(torch.linalg.norm(datum - P["dnew"]) / torch.linalg.norm(P["dnew"]))
-    # plain path: one shot at a time
-    plain = None
-    if args.plain_shots > 0:
+    datum = torch.cat([b["datum"] for b in batches])
+    hist = np.concatenate([hists[b] for b in sorted(hists)]) if hists else None
+    # quality: redatumed gathers vs data modelled at the datum in the true model (both band-passed)
+    done = [b for b in range(nbat) if b in hists]
+    dn = torch.cat([P["dnew"][batches[b]["shots"][0]:batches[b]["shots"][1]] for b in done])
+    dg = torch.cat([batches[b]["datum"] for b in done])
+    err = float(torch.linalg.norm(dg - dn) / torch.linalg.norm(dn))
+    _log(f"timed: {shots_timed} shots in {elapsed:.1f} s = {shots_per_hour:.0f} shots/hour; "
+         f"dominant {modes[dom]} {avg_ms * 1e3:.2f} us per launch = {achieved / 8000.0:.3f} of 8 TB/s")
+
+    def time_left():
+        return args.time_budget - (time.perf_counter() - t_start)
+
+    # plain path: one shot at a time (the batch of shot 1, redatumed alone)
+    plain = diff = None
+    if args.plain_shots > 0 and time_left() > 60:
+        b0 = batches[0]
+        kp = min(args.plain_shots, b0["shots"][1] - b0["shots"][0])
         daw.reset_stats()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        _log(f"plain path: {args.plain_shots} shots one at a time")
-        dp, _ = run_redatum(daw, P, shots=args.plain_shots, max_batch=1)
+        _log(f"plain path: {kp} shots one at a time")
+        dp = torch.zeros((kp, P["dat"].shape[1], w.nt), dtype=torch.float32, device="cuda")
+        daw.redatum_batch(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], P["s_dev"],
+                          np.ascontiguousarray(b0["src"][:, :kp]), P["rec"], P["dat"], b0["dobs"][:kp], w.mu,
+                          w.maxiter, 0.0, datum_dev=dp, max_batch=1, band=w.band)
         torch.cuda.synchronize()
         tp = time.perf_counter() - t1
-        plain = args.plain_shots / tp * 3600.0
-        diff = float(torch.max(torch.abs(dp - datum[:args.plain_shots])) / torch.max(torch.abs(datum)))
-    # mixed precision: the same shots with FP32 snapshots (untimed) and, on
+        plain = kp / tp * 3600.0
+        ref_b = b0["datum"][:kp]
+        diff = float(torch.max(torch.abs(dp - ref_b)) / torch.max(torch.abs(ref_b)))
+    # mixed precision: the same batch with FP32 snapshots (untimed) and, on
     # request, the FP64 oracle of the plain method
     mixed = None
-    if args.snapshots != "fp32":
+    if args.snapshots != "fp32" and time_left() > 60:
         daw.set_snapshot_format("fp32")
-        _log("fp32-snapshot reference run")
-        d32, h32 = run_redatum(daw, P, max_batch=args.max_batch, hbm_budget=budget)
+        _log("fp32-snapshot reference run of the first batch")
+        b0 = batches[0]
+        d32 = torch.zeros_like(b0["datum"])
+        h32 = daw.redatum_batch(P["c0"], w.npml, P["h"], w.ofs, w.nt, P["dt"], P["s_dev"], b0["src"], P["rec"],
+                                P["dat"], b0["dobs"], w.mu, w.maxiter, 0.0, datum_dev=d32, max_batch=bsz,
+                                band=w.band)
         daw.set_snapshot_format(args.snapshots)
-        per = [float(torch.linalg.norm(datum[s] - d32[s]) / torch.linalg.norm(d32[s])) for s in range(nsh)]
+        per = [float(torch.linalg.norm(b0["datum"][s] - d32[s]) / torch.linalg.norm(d32[s]))
+               for s in range(d32.shape[0])]
         mixed = {"snapshots": args.snapshots,
                  "datum_rel_l2_vs_fp32_max": max(per), "datum_rel_l2_vs_fp32_mean": float(np.mean(per)),
-                 "minres_hist_max_rel_diff_vs_fp32": float(np.max(np.abs(hist / h32 - 1)))}
+                 "minres_hist_max_rel_diff_vs_fp32": float(np.max(np.abs(hists[0] / h32 - 1)))}
     if args.f64_shots > 0:
         mixed = dict(mixed or {}, **f64_check(P, datum, hist, args.f64_shots))
     cpu = None
-    if not args.no_cpu:
-        _log(f"cpu baseline sample nt={args.cpu_nt}")
-        cpu = cpu_baseline(P, args.cpu_nt)
+    if not args.no_cpu and time_left() > args.cpu_timeout + 20:
+        _log(f"cpu baseline: reference path on the host cores, nt={args.cpu_nt}")
+        try:
+            cpu = cpu_baseline(P, args.cpu_nt, timeout=args.cpu_timeout)
+        except Exception as exc:          # the GPU line stands without it
+            _log(f"cpu baseline failed: {exc}")
+            cpu = {"value": None, "error": str(exc)}
     out = {
         "metric": "shots redatumed/hour + stencil GPts/s on 1 MI355X, batched vs plain at equal error",
         "value": shots_per_hour,
@@ -585,17 +737,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded layered model (sigma 3 smoothing), background sigma 10, Ricker, 1% seeded noise",
-        "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m", "shots": nsh, "nt": w.nt,
+        "config": {"workload": f"{w.name}, {w.maxiter} MINRES its, datum z=400 m; one step = the redatuming of one "
+                               f"batch of {bsz} shots advanced together (steps walk the {nbat} batches of the line)",
+                   "shots": nsh, "shots_per_step": bsz, "shots_timed": shots_timed, "nt": w.nt,
                    "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
                    "shots_per_launch": shots_per_launch, "snapshots": args.snapshots, "graph": args.graph,
-                   "launch": {m: {"steps": v[0], "tile": [v[1], v[2]]} for m, v in launch.items()},
-                   "tile": [launch["back-propagation"][1], launch["back-propagation"][2]]},
+                   "launch": {m: {"steps": v[0], "tile": [v[1], v[2]]} for m, v in launch.items()}},
         "gpts_per_s": gpts,
         "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
                               for i, k in enumerate(("forward", "back-propagation", "re-injection"))},
         "plain_shots_per_hour": plain,
         "batched_over_plain": (shots_per_hour / plain) if plain else None,
-        "batched_vs_plain_max_rel_diff": diff if plain else None,
+        "batched_vs_plain_max_rel_diff": diff,
         "redatum_rel_l2_vs_true_datum_data": err,
         "minres_final_rel_residual_mean": float(np.mean(hist[:, -1])) if hist is not None else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
@@ -604,6 +757,7 @@ def main():
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes,
                      "steps_per_launch": launch_steps},
         "cpu_baseline": cpu,
+        "wall_s": round(time.perf_counter() - t_start, 1),
     }
     if mixed:
         out["precision_check"] = mixed

@@ -89,6 +89,17 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
  * the re-injection pass, which makes the MINRES operator symmetric. */
 int daw_set_hessian(int exact);
 
+/* Arithmetic of the MINRES solve (daw_sublinearminres, daw_redatum_batch,
+ * daw_redatum_uneven): 0 (default) = FP32 vectors with FP64 dot products,
+ * norms and Lanczos/Givens scalars; 1 = the reference's own arithmetic,
+ * REAL throughout as minresModule.f90:10-360 and sublinearminres.f90:64-72,
+ * 117-133 run it: sequential FP32 dot products in dot_product's order,
+ * datanorm in its t-outer/r-inner order, FP32 scalars and vector updates
+ * rounded as written.  Mode 1 reproduces the reference's solve (one lane
+ * per shot adds each dot product in order, ~2 ms per 1.2 M-sample product);
+ * mode 0 is the more accurate.  No reference counterpart (a knob). */
+int daw_set_minres_arith(int arith);
+
 /* Storage of the wavefield snapshots sp, svx, svz that the reference
  * Hessian-type operator keeps between its two passes (submodeling2.f90:1060-1087
  * write, :1164-1185 read): 0 = FP32 as the reference (default), 1 = BF16,

@@ -37,7 +37,7 @@ EXPORTS = [
     "daw_submodeling_transpose", "daw_set_hessian", "daw_set_snapshot_format",
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
     "daw_get_launch", "daw_sublinearminres_log", "daw_write_minres_log", "daw_readpar", "daw_read_acqui",
-    "daw_subresample", "daw_subresample_dev",
+    "daw_subresample", "daw_subresample_dev", "daw_set_minres_arith",
 ]
 
 
@@ -268,6 +268,20 @@ def submodeling_transpose(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab):
 def set_hessian(exact):
     """0: the reference's Hessian-type operator; 1: exact (symmetric) variant."""
     _check(lib().daw_set_hessian(1 if exact else 0))
+
+
+MINRES_ARITH = {"f64": 0, "ref32": 1}
+
+
+def set_minres_arith(arith):
+    """MINRES arithmetic of the solve: "f64" (default: FP64 dot products and
+    Lanczos/Givens scalars over FP32 vectors) or "ref32" (the reference's
+    own FP32 arithmetic: sequential FP32 dot products in its summation
+    order, FP32 scalars and vector updates, minresModule.f90:10-360,
+    sublinearminres.f90:64-72,117-133)."""
+    if arith not in MINRES_ARITH:
+        raise ValueError(f"minres arithmetic {arith!r}: one of {sorted(MINRES_ARITH)}")
+    _check(lib().daw_set_minres_arith(MINRES_ARITH[arith]))
 
 
 def set_steps(steps):

@@ -72,6 +72,7 @@ struct Global {
     bool hess_exact = false;   // Hessian pass 1 = exact transpose of pass 2
     int snap_fmt = kSnapF32;   // snapshot store of the reference Hessian (SnapFormat)
     bool graph = false;        // replay full sweeps as captured hipGraphs
+    int minres_arith = 0;      // 0: FP64 dot products and scalars; 1: the reference's FP32 arithmetic
     std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw} per mode
     LaunchCfg last_cfg[3];     // launch configuration of each mode in the last tuned batch
 };
@@ -1043,7 +1044,7 @@ static int filter_device(const float *din, float *dout, int ntr, int nt, float d
 // ------------------------------------------------------------------ MINRES over a batch
 struct MinresWork {
     DevBuf vec;     // x, v, Y[3], Wv[3], hb: 9 vectors per shot
-    DevBuf off, len, part0, part1, scal, hist, trace;
+    DevBuf off, len, nts, part0, part1, scal, hist, trace;
 };
 
 // Per-shot MINRES log block handed to callers: kLogHead doubles (fac, Anorm,
@@ -1079,6 +1080,11 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
     if (int rc = w.part1.alloc((size_t)ns * nb * sizeof(double))) return rc;
     if (int rc = w.scal.alloc((size_t)ns * nsc * sizeof(double))) return rc;
     if (int rc = w.hist.alloc((size_t)ns * std::max(itnlim, 1) * sizeof(double))) return rc;
+    const bool ref32 = g.minres_arith == 1;
+    std::vector<int> nts(ns);
+    for (int s = 0; s < ns; ++s) nts[s] = B.geo_of(s).nt;
+    if (int rc = upload(w.nts, nts)) return rc;
+    const int *dnts = w.nts.as<int>();
     float *base = w.vec.as<float>();
     float *v = base, *Y[3] = {base + blk, base + 2 * blk, base + 3 * blk};
     float *Wv[3] = {base + 4 * blk, base + 5 * blk, base + 6 * blk};
@@ -1097,9 +1103,18 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
     HessOut o;
     o.sis = hb; o.reverse = 0; o.variant = 0;
     if (int rc = hessian(B, b, o)) return rc;
-    launch_dot(b, b, doff, dlen, w.part0.as<double>(), ns, g.stream);
-    launch_dot(hb, hb, doff, dlen, w.part1.as<double>(), ns, g.stream);
-    launch_minres_init(w.part0.as<double>(), w.part1.as<double>(), w.scal.as<double>(), mu, ns, g.stream);
+    double *dscal = w.scal.as<double>();
+    if (ref32) {
+        // datanorm of b and H b in its t-outer order, then b.b as dot_product sums it
+        launch_seqdot32(b, b, doff, dlen, dnts, dscal, 0, 1, false, ns, g.stream);
+        launch_seqdot32(hb, hb, doff, dlen, dnts, dscal, 1, 1, false, ns, g.stream);
+        launch_seqdot32(b, b, doff, dlen, dnts, dscal, 2, 0, false, ns, g.stream);
+        launch_minres32_init(dscal, mu, ns, g.stream);
+    } else {
+        launch_dot(b, b, doff, dlen, w.part0.as<double>(), ns, g.stream);
+        launch_dot(hb, hb, doff, dlen, w.part1.as<double>(), ns, g.stream);
+        launch_minres_init(w.part0.as<double>(), w.part1.as<double>(), dscal, mu, ns, g.stream);
+    }
     // y = r1 = r2 = b; x = w = w2 = 0
     for (int k = 0; k < 3; ++k) HIPCHK(hipMemcpyAsync(Y[k], b, blk * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
     launch_fill(x, blk, 0.0f, g.stream);
@@ -1116,21 +1131,34 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
         if (!any) break;
         // y of the previous iteration lives in the r2 buffer (r2 = y, :193); the
         // free buffer Y[iy] receives the new operator output
-        launch_minres_v(Y[ir2], v, doff, dlen, w.scal.as<double>(), ns, g.stream);
+        if (ref32) launch_minres32_v(Y[ir2], v, doff, dlen, dscal, ns, g.stream);
+        else launch_minres_v(Y[ir2], v, doff, dlen, dscal, ns, g.stream);
         HessOut h;
         h.sis = Y[iy]; h.reverse = 1; h.variant = 0;     // blurring: time-reversed output (:394-396)
         if (int rc = hessian(B, v, h)) return rc;
-        launch_minres_l1(Y[iy], v, Y[ir1], doff, dlen, w.scal.as<double>(), w.part0.as<double>(), ns, g.stream);
-        launch_minres_l2(Y[iy], Y[ir2], doff, dlen, w.scal.as<double>(), w.part1.as<double>(), ns, g.stream);
+        if (ref32) {
+            launch_minres32_l1(Y[iy], v, Y[ir1], doff, dlen, dscal, ns, g.stream);
+            launch_seqdot32(v, Y[iy], doff, dlen, dnts, dscal, 3, 0, true, ns, g.stream);        // alfa
+            launch_minres32_l2(Y[iy], Y[ir2], doff, dlen, dscal, ns, g.stream);
+            launch_seqdot32(Y[iy], Y[iy], doff, dlen, dnts, dscal, 0, 0, true, ns, g.stream);    // r2.y
+        } else {
+            launch_minres_l1(Y[iy], v, Y[ir1], doff, dlen, dscal, w.part0.as<double>(), ns, g.stream);
+            launch_minres_l2(Y[iy], Y[ir2], doff, dlen, dscal, w.part1.as<double>(), ns, g.stream);
+        }
         // r1 <- r2, r2 <- y, y <- free
         const int old_r1 = ir1;
         ir1 = ir2;
         ir2 = iy;
         iy = old_r1;
-        launch_minres_scalars(w.part1.as<double>(), w.scal.as<double>(), w.hist.as<double>(), trace, dlen, itnlim, rtol,
-                              ns, g.stream);
         // w1 <- w2, w2 <- w, w <- new (written over old w1)
-        launch_minres_x(x, Wv[iw1], Wv[iw], Wv[iw2], v, doff, dlen, w.scal.as<double>(), trace, itnlim, ns, g.stream);
+        if (ref32) {
+            launch_minres32_scalars(dscal, w.hist.as<double>(), trace, dlen, itnlim, (float)rtol, ns, g.stream);
+            launch_minres32_x(x, Wv[iw1], Wv[iw], Wv[iw2], v, doff, dlen, dscal, trace, itnlim, ns, g.stream);
+        } else {
+            launch_minres_scalars(w.part1.as<double>(), dscal, w.hist.as<double>(), trace, dlen, itnlim, rtol, ns,
+                                  g.stream);
+            launch_minres_x(x, Wv[iw1], Wv[iw], Wv[iw2], v, doff, dlen, dscal, trace, itnlim, ns, g.stream);
+        }
         const int nw = iw1, nw2 = iw, nw1 = iw2;
         iw = nw; iw2 = nw2; iw1 = nw1;
     }
@@ -1869,6 +1897,14 @@ int daw_set_hessian(int exact)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     g.hess_exact = exact != 0;
+    return 0;
+}
+
+int daw_set_minres_arith(int arith)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (arith != 0 && arith != 1) return fail(DAW_ERR_ARG, "minres arithmetic %d: 0 (FP64) or 1 (the reference's FP32)", arith);
+    g.minres_arith = arith;
     return 0;
 }
 

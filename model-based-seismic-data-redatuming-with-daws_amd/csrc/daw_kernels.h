@@ -56,6 +56,26 @@ void launch_minres_x(float *x, float *wn, const float *w2, const float *w1, cons
                      const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st);
 constexpr int kTraceStride = 5;
 
+// MINRES in the reference's FP32 arithmetic.  launch_seqdot32 writes one
+// sequential FP32 sum per shot into its scalar row: which = 0, 1, 2 (the
+// initialisation's datanorm(b)^2, datanorm(Hb)^2, b.b) or 3 (alfa);
+// order 0 sums a[i]*b[i] for ascending i (dot_product), order 1 in datanorm's
+// t-outer, r-inner order over [r][t] traces of nts[s] samples.
+void launch_seqdot32(const float *a, const float *b, const size_t *off, const int *len, const int *nts, double *scal,
+                     int which, int order, bool skip_inactive, int nshots, hipStream_t st);
+void launch_minres32_init(double *scal, float mu, int nshots, hipStream_t st);
+void launch_minres32_v(const float *y, float *v, const size_t *off, const int *len, const double *scal, int nshots,
+                       hipStream_t st);
+void launch_minres32_l1(float *y, const float *v, const float *r1, const size_t *off, const int *len,
+                        const double *scal, int nshots, hipStream_t st);
+void launch_minres32_l2(float *y, const float *r2, const size_t *off, const int *len, const double *scal,
+                        int nshots, hipStream_t st);
+// r2.y of the iteration is read from slot which = 0
+void launch_minres32_scalars(double *scal, double *hist, double *trace, const int *len, int itnlim, float rtol,
+                             int nshots, hipStream_t st);
+void launch_minres32_x(float *x, float *wn, const float *w2, const float *w1, const float *v, const size_t *off,
+                       const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st);
+
 // indices into a shot's scalar row (see k_minres_scalars)
 enum MinresScalIndex {
     kSBeta = 0, kSOldb, kSBeta1, kSDbar, kSEpsln, kSQrnorm, kSPhibar, kSRhs1, kSRhs2, kSTnorm2, kSYnorm2, kSCs,

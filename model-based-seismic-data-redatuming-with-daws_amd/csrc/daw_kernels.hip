@@ -1641,7 +1641,7 @@ constexpr int kVecBlocks = 64;
 enum {
     kBeta = 0, kOldb, kBeta1, kDbar, kEpsln, kQrnorm, kPhibar, kRhs1, kRhs2, kTnorm2, kYnorm2, kCs, kSn,
     kGmax, kGmin, kAnorm, kYnorm, kRnorm, kAcond, kAlfa, kShift, kOldeps, kDelta, kPhi, kDenom, kIstop,
-    kItn, kArnorm, kFac, kStopNext, kScal
+    kItn, kArnorm, kFac, kStopNext, kD0, kD1, kD2, kScal
 };
 
 __global__ void k_dot(const float *a, const float *b, const size_t *off, const int *len, double *part)
@@ -1876,6 +1876,313 @@ __global__ void k_minres_close(double *scal, int nshots)
     double *q = scal + (size_t)s * kScal;
     if (q[kIstop] != 0.0 || q[kItn] < 0.0) return;
     q[kIstop] = q[kStopNext];
+}
+
+// ---------------------------------------------------------------------------
+// MINRES in the reference's own arithmetic (minresModule.f90:10-360 with
+// sublinearminres.f90:64-72,117-133, all REAL): the opt-in parity mode.
+// Vector updates are the reference's FP32 expressions, rounded after every
+// operation in its order (-ffp-contract=off).  Its dot products and datanorm
+// are sequential FP32 sums, which no parallel reduction reproduces: one
+// workgroup per shot forms the rounded products in LDS (waves 1-3, one chunk
+// ahead) while one lane of wave 0 adds them in the reference's order.
+// Scalars are FP32, kept in the shot's scalar row as exactly representable
+// doubles so that the host-side stop test and the log readers are shared
+// with the FP64 path.
+// ---------------------------------------------------------------------------
+constexpr int kSeqChunk = 2048;
+
+// ORDER 0: sum over i ascending of a[i]*b[i] (dot_product, minresModule.f90:87,189,195).
+// ORDER 1: datanorm's order over [r][t] traces, t outer and r inner
+// (sublinearminres.f90:125-129).  Result to scal[s*kScal + slot].
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_seqdot32(const float *a, const float *b, const size_t *off, const int *len,
+                                                  const int *nts, double *scal, int slot, int skip_inactive)
+{
+    __shared__ float4 buf4[2][kSeqChunk / 4];
+    const int s = blockIdx.x;
+    double *q = scal + (size_t)s * kScal;
+    if (skip_inactive && !active(q)) return;
+    const unsigned n = (unsigned)len[s];
+    const float *as = a + off[s], *bs = b + off[s];
+    const unsigned nt = (unsigned)nts[s];
+    const unsigned nr = n / nt;
+    const int tid = threadIdx.x;
+    const unsigned nchunks = (n + kSeqChunk - 1) / kSeqChunk;
+    auto fill = [&](unsigned c) {
+        float *p = reinterpret_cast<float *>(buf4[c & 1]);
+        const unsigned k0 = c * kSeqChunk;
+        const int cnt = (int)min((unsigned)kSeqChunk, n - k0);
+        for (int j = tid - 64; j < cnt; j += kBlock - 64) {
+            unsigned i = k0 + (unsigned)j;
+            if (ORDER == 1) {
+                const unsigned t = i / nr, r = i - t * nr;
+                i = r * nt + t;
+            }
+            p[j] = as[i] * bs[i];
+        }
+    };
+    float acc = 0.0f;
+    if (tid >= 64 && nchunks > 0) fill(0);
+    __syncthreads();
+    for (unsigned c = 0; c < nchunks; ++c) {
+        if (tid >= 64) {
+            if (c + 1 < nchunks) fill(c + 1);
+        } else if (tid == 0) {
+            const int cnt = (int)min((unsigned)kSeqChunk, n - c * kSeqChunk);
+            const float4 *p4 = buf4[c & 1];
+            int j = 0;
+#pragma unroll 8
+            for (; j + 4 <= cnt; j += 4) {
+                const float4 v = p4[j >> 2];
+                acc = acc + v.x;
+                acc = acc + v.y;
+                acc = acc + v.z;
+                acc = acc + v.w;
+            }
+            const float *p = reinterpret_cast<const float *>(p4);
+            for (; j < cnt; ++j) acc = acc + p[j];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) q[slot] = (double)acc;
+}
+
+// fac = datanorm(deltadr) / datanorm(deltade) * mu, shift = -fac
+// (sublinearminres.f90:64-72), beta1 = sqrt(b.b) (minresModule.f90:83-153),
+// from kD0 = datanorm(b)^2, kD1 = datanorm(H b)^2, kD2 = b.b.
+__global__ void k_minres32_init(double *scal, float mu, int nshots)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    double *q = scal + (size_t)s * kScal;
+    const float ndr = sqrtf((float)q[kD0]), nde = sqrtf((float)q[kD1]);
+    const float fac = ndr / nde * mu;
+    float beta1 = (float)q[kD2];
+    for (int i = 0; i < kScal; ++i) q[i] = 0.0;
+    q[kFac] = fac;
+    q[kShift] = -fac;
+    if (beta1 < 0.0f) { q[kIstop] = 8; return; }
+    if (beta1 == 0.0f) { q[kIstop] = 0; q[kItn] = -1; return; }
+    beta1 = sqrtf(beta1);
+    q[kBeta1] = beta1;
+    q[kBeta] = beta1;
+    q[kQrnorm] = beta1;
+    q[kPhibar] = beta1;
+    q[kRhs1] = beta1;
+    q[kCs] = -1.0;
+}
+
+// v = (1/beta) y (:181-182)
+__global__ void k_minres32_v(const float *y, float *v, const size_t *off, const int *len, const double *scal)
+{
+    const int s = blockIdx.y;
+    const double *q = scal + (size_t)s * kScal;
+    if (!active(q)) return;
+    const float sc = 1.0f / (float)q[kBeta];
+    const size_t n = len[s];
+    const float *ys = y + off[s];
+    float *vs = v + off[s];
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock)
+        vs[i] = sc * ys[i];
+}
+
+// y = y - shift v, then y = y - (beta/oldb) r1 from the second iteration (:185-188)
+__global__ void k_minres32_l1(float *y, const float *v, const float *r1, const size_t *off, const int *len,
+                              const double *scal)
+{
+    const int s = blockIdx.y;
+    const double *q = scal + (size_t)s * kScal;
+    if (!active(q)) return;
+    const float shift = (float)q[kShift];
+    const bool second = q[kItn] >= 1.0;    // kItn counts finished iterations
+    const float c = second ? (float)q[kBeta] / (float)q[kOldb] : 0.0f;
+    const size_t n = len[s];
+    float *ys = y + off[s];
+    const float *vs = v + off[s], *rs = r1 + off[s];
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock) {
+        float yi = ys[i] - shift * vs[i];
+        if (second) yi = yi - c * rs[i];
+        ys[i] = yi;
+    }
+}
+
+// y = y - (alfa/beta) r2 (:191)
+__global__ void k_minres32_l2(float *y, const float *r2, const size_t *off, const int *len, const double *scal)
+{
+    const int s = blockIdx.y;
+    const double *q = scal + (size_t)s * kScal;
+    if (!active(q)) return;
+    const float c = (float)q[kAlfa] / (float)q[kBeta];
+    const size_t n = len[s];
+    float *ys = y + off[s];
+    const float *rs = r2 + off[s];
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock)
+        ys[i] = ys[i] - c * rs[i];
+}
+
+// Scalar part of one iteration (:196-303) in FP32; alfa in kAlfa, r2.y in kD0.
+__global__ void k_minres32_scalars(double *scal, double *hist, double *trace, const int *len, int itnlim, float rtol,
+                                   int nshots)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    double *q = scal + (size_t)s * kScal;
+    if (!active(q)) return;
+    const float eps = 1.1920928955078125e-07f;   // epsilon(real), minresModule.f90:64
+    const int itn = (int)q[kItn] + 1;
+    const float alfa = (float)q[kAlfa];
+    const float oldb = (float)q[kBeta];
+    float beta = (float)q[kD0];
+    q[kOldb] = oldb;
+    if (beta < 0.0f) { q[kIstop] = 6; q[kItn] = itn; return; }
+    beta = sqrtf(beta);
+    const float tnorm2 = (float)q[kTnorm2] + alfa * alfa + oldb * oldb + beta * beta;
+    const float beta1 = (float)q[kBeta1];
+    float gmax = (float)q[kGmax], gmin = (float)q[kGmin];
+    int istop = 0;
+    if (itn == 1) {
+        if (beta / beta1 <= 10.0f * eps) istop = -1;
+        gmax = alfa < 0.0f ? -alfa : alfa;
+        gmin = gmax;
+    }
+    const float cs = (float)q[kCs], sn = (float)q[kSn], dbar = (float)q[kDbar];
+    const float oldeps = (float)q[kEpsln];
+    const float delta = cs * dbar + sn * alfa;
+    const float gbar = sn * dbar - cs * alfa;
+    const float epsln = sn * beta;
+    const float dbar2 = -cs * beta;
+    const float gamma = sqrtf(gbar * gbar + beta * beta);
+    const float cs2 = gbar / gamma, sn2 = beta / gamma;
+    const float phibar0 = (float)q[kPhibar];
+    const float phi = cs2 * phibar0;
+    const float phibar = sn2 * phibar0;
+    const float denom = 1.0f / gamma;
+    gmax = gmax > gamma ? gmax : gamma;
+    gmin = gmin < gamma ? gmin : gamma;
+    const float z = (float)q[kRhs1] / gamma;
+    const float ynorm2 = z * z + (float)q[kYnorm2];
+    const float rhs1 = (float)q[kRhs2] - delta * z;
+    const float rhs2 = -epsln * z;
+    const float anorm = sqrtf(tnorm2);
+    const float ynorm = sqrtf(ynorm2);
+    const float rnorml = (float)q[kRnorm];
+    const float rootl = sqrtf(gbar * gbar + dbar2 * dbar2);
+    const float acond = gmax / gmin;
+    const float rel_r = phibar / beta1;
+    q[kTnorm2] = tnorm2;
+    q[kGmax] = gmax;
+    q[kGmin] = gmin;
+    q[kOldeps] = oldeps;
+    q[kDelta] = delta;
+    q[kEpsln] = epsln;
+    q[kDbar] = dbar2;
+    q[kCs] = cs2;
+    q[kSn] = sn2;
+    q[kPhi] = phi;
+    q[kPhibar] = phibar;
+    q[kDenom] = denom;
+    q[kBeta] = beta;
+    q[kYnorm2] = ynorm2;
+    q[kRhs1] = rhs1;
+    q[kRhs2] = rhs2;
+    q[kAnorm] = anorm;
+    q[kYnorm] = ynorm;
+    q[kQrnorm] = phibar;
+    q[kRnorm] = phibar;
+    q[kArnorm] = rnorml * rootl;
+    q[kAcond] = acond;
+    if (hist) hist[(size_t)s * itnlim + (itn - 1)] = rel_r;
+    if (istop == 0) {
+        if (itn >= itnlim) istop = 5;
+        if (rel_r <= rtol) istop = 1;
+    }
+    if (trace) {
+        const float epsx = anorm * ynorm * eps, epsr = anorm * ynorm * rtol, rel = rootl / anorm;
+        const bool prnt = len[s] <= 40 || itn <= 10 || itn >= itnlim - 10 || itn % 10 == 0 ||
+                          phibar <= 10.0f * epsx || phibar <= 10.0f * epsr || rel <= 10.0f * epsx ||
+                          rel <= 10.0f * epsr || acond >= 1.0e-2f / eps || istop != 0;
+        double *t = trace + ((size_t)s * itnlim + (itn - 1)) * kTraceStride;
+        t[1] = rel_r;
+        t[2] = anorm;
+        t[3] = acond;
+        t[4] = prnt ? 1.0 : 0.0;
+    }
+    q[kItn] = itn;
+    q[kStopNext] = istop;
+}
+
+// w1 <- w2, w2 <- w, w <- (v - oldeps w1 - delta w2) denom, x <- x + phi w (:252-257), FP32
+__global__ void k_minres32_x(float *x, float *w_new_over_w1, const float *w2_is_old_w, const float *w1_is_old_w2,
+                             const float *v, const size_t *off, const int *len, double *scal, double *trace,
+                             int itnlim)
+{
+    const int s = blockIdx.y;
+    double *q = scal + (size_t)s * kScal;
+    if (!(q[kIstop] == 0.0 && q[kItn] >= 1.0)) return;
+    const float oldeps = (float)q[kOldeps], delta = (float)q[kDelta], denom = (float)q[kDenom];
+    const float phi = (float)q[kPhi];
+    const size_t n = len[s];
+    float *xs = x + off[s], *wn = w_new_over_w1 + off[s];
+    const float *w2 = w2_is_old_w + off[s], *w1 = w1_is_old_w2 + off[s], *vs = v + off[s];
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock) {
+        const float wi = (vs[i] - oldeps * w1[i] - delta * w2[i]) * denom;
+        wn[i] = wi;
+        const float xi = xs[i] + phi * wi;
+        xs[i] = xi;
+        if (trace && i == 0) trace[((size_t)s * itnlim + ((int)q[kItn] - 1)) * kTraceStride] = (double)xi;
+    }
+}
+
+void launch_seqdot32(const float *a, const float *b, const size_t *off, const int *len, const int *nts, double *scal,
+                     int which, int order, bool skip_inactive, int nshots, hipStream_t st)
+{
+    static const int slots[4] = {kD0, kD1, kD2, kAlfa};
+    const int slot = slots[which & 3];
+    if (order == 1)
+        hipLaunchKernelGGL(k_seqdot32<1>, dim3(nshots), dim3(kBlock), 0, st, a, b, off, len, nts, scal, slot,
+                           skip_inactive ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_seqdot32<0>, dim3(nshots), dim3(kBlock), 0, st, a, b, off, len, nts, scal, slot,
+                           skip_inactive ? 1 : 0);
+}
+
+void launch_minres32_init(double *scal, float mu, int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_minres32_init, dim3((nshots + 63) / 64), dim3(64), 0, st, scal, mu, nshots);
+}
+
+void launch_minres32_v(const float *y, float *v, const size_t *off, const int *len, const double *scal, int nshots,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_minres32_v, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, y, v, off, len, scal);
+}
+
+void launch_minres32_l1(float *y, const float *v, const float *r1, const size_t *off, const int *len,
+                        const double *scal, int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_minres32_l1, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, y, v, r1, off, len, scal);
+}
+
+void launch_minres32_l2(float *y, const float *r2, const size_t *off, const int *len, const double *scal,
+                        int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_minres32_l2, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, y, r2, off, len, scal);
+}
+
+void launch_minres32_scalars(double *scal, double *hist, double *trace, const int *len, int itnlim, float rtol,
+                             int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_minres32_scalars, dim3((nshots + 63) / 64), dim3(64), 0, st, scal, hist, trace, len, itnlim,
+                       rtol, nshots);
+}
+
+void launch_minres32_x(float *x, float *wn, const float *w2, const float *w1, const float *v, const size_t *off,
+                       const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_minres32_x, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, x, wn, w2, w1, v, off, len, scal,
+                       trace, itnlim);
+    hipLaunchKernelGGL(k_minres_close, dim3((nshots + 63) / 64), dim3(64), 0, st, scal, nshots);
 }
 
 void launch_minres_init(const double *part0, const double *part1, double *scal, float mu, int nshots, hipStream_t st)

@@ -6,7 +6,8 @@
  * checker.  It is never part of the product path (the HIP library does not
  * link it and the Python package never imports it).
  *
- * Built by oracle/Makefile with -O2 -ffp-contract=off and no -march, so the
+ * Built by oracle/Makefile with -O2 -ffp-contract=off, OpenMP over grid
+ * columns (each point written by one thread) and no -march, so the
  * single-precision functions perform the same IEEE operations in the same
  * order as the reference compiled for x86-64 (no FMA).  The propagators,
  * resampling and MINRES are instantiated twice from daw_oracle_prop.h:
@@ -29,6 +30,14 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Grid loops run in parallel (OpenMP) only where every grid point is written
+ * by one thread from values no other thread writes in the same loop, so the
+ * results are those of the serial loops; small grids stay serial. */
+#define OMP_BIG(S) ((long)(S)->n1e * (S)->n2e > 20000)
 
 /* ------------------------------------------------------------------------ */
 /* Hicks (Kaiser-windowed sinc) tables, kwsinc.f90                          */

@@ -123,12 +123,27 @@ static void SFX(setup_make)(SFX(setup_t) *S, const float *vp, int n1, int n2, in
 #define E(a, i1, i2) (a)[(size_t)((i1) - 1) + (size_t)((i2) - 1) * n1e]
 #define I3(a, i1, i2, it) (a)[(size_t)((i1) - 1) + (size_t)((i2) - 1) * n1 + (size_t)((it) - 1) * n1 * n2]
 
+/* This thread's share c0..c1 of the columns lo..hi (all of them outside a
+ * parallel region). */
+static void SFX(my_cols)(int lo, int hi, int *c0, int *c1)
+{
+#ifdef _OPENMP
+    const long nth = omp_get_num_threads(), th = omp_get_thread_num();
+#else
+    const long nth = 1, th = 0;
+#endif
+    const long n = hi - lo + 1;
+    *c0 = lo + (int)(n * th / nth);
+    *c1 = lo + (int)(n * (th + 1) / nth) - 1;
+}
+
 /* pressure update, split PML; sgn=+1 forward (:148-155), -1 back-prop (:657-664) */
-static void SFX(p_update)(SFX(setup_t) *S, R *px, R *pz, const R *vx, const R *vz, int lo, int hi1, int hi2, int sgn)
+static void SFX(p_update_cols)(const SFX(setup_t) *S, R *px, R *pz, const R *vx, const R *vz, int lo, int hi1,
+                               int c0, int c1, int sgn)
 {
     const int n1e = S->n1e;
     const R a0 = (R)1.125, a1 = -((R)1 / (R)24);   /* :48-49 */
-    for (int i2 = lo; i2 <= hi2; i2++)
+    for (int i2 = c0; i2 <= c1; i2++)
         for (int i1 = lo; i1 <= hi1; i1++) {
             R der2 = a0 * (E(vx, i1, i2) - E(vx, i1, i2 - 1)) + a1 * (E(vx, i1, i2 + 1) - E(vx, i1, i2 - 2));
             R der1 = a0 * (E(vz, i1, i2) - E(vz, i1 - 1, i2)) + a1 * (E(vz, i1 + 1, i2) - E(vz, i1 - 2, i2));
@@ -142,12 +157,23 @@ static void SFX(p_update)(SFX(setup_t) *S, R *px, R *pz, const R *vx, const R *v
         }
 }
 
+static void SFX(p_update)(SFX(setup_t) *S, R *px, R *pz, const R *vx, const R *vz, int lo, int hi1, int hi2, int sgn)
+{
+#pragma omp parallel if (OMP_BIG(S))
+    {
+        int c0, c1;
+        SFX(my_cols)(lo, hi2, &c0, &c1);
+        SFX(p_update_cols)(S, px, pz, vx, vz, lo, hi1, c0, c1, sgn);
+    }
+}
+
 /* velocity update; sgn=+1 forward (:229-236), -1 back-prop (:710-717) */
-static void SFX(v_update)(SFX(setup_t) *S, R *vx, R *vz, const R *p, int lo, int hi1, int hi2, int sgn)
+static void SFX(v_update_cols)(const SFX(setup_t) *S, R *vx, R *vz, const R *p, int lo, int hi1, int c0, int c1,
+                               int sgn)
 {
     const int n1e = S->n1e;
     const R a0 = (R)1.125, a1 = -((R)1 / (R)24);   /* :48-49 */
-    for (int i2 = lo; i2 <= hi2; i2++)
+    for (int i2 = c0; i2 <= c1; i2++)
         for (int i1 = lo; i1 <= hi1; i1++) {
             R der2 = a0 * (E(p, i1, i2 + 1) - E(p, i1, i2)) + a1 * (E(p, i1, i2 + 2) - E(p, i1, i2 - 1));
             R der1 = a0 * (E(p, i1 + 1, i2) - E(p, i1, i2)) + a1 * (E(p, i1 + 2, i2) - E(p, i1 - 1, i2));
@@ -161,12 +187,67 @@ static void SFX(v_update)(SFX(setup_t) *S, R *vx, R *vz, const R *p, int lo, int
         }
 }
 
+static void SFX(v_update)(SFX(setup_t) *S, R *vx, R *vz, const R *p, int lo, int hi1, int hi2, int sgn)
+{
+#pragma omp parallel if (OMP_BIG(S))
+    {
+        int c0, c1;
+        SFX(my_cols)(lo, hi2, &c0, &c1);
+        SFX(v_update_cols)(S, vx, vz, p, lo, hi1, c0, c1, sgn);
+    }
+}
+
+/* dst(i1,i2) = src(i1+npml, i2+npml) over the interior (a snapshot or
+ * wavefield slice of n1*n2 values from the extended grid), columns c0..c1 */
+static void SFX(grab_cols)(R *dst, const R *src, int n1, int npml, int n1e, int c0, int c1)
+{
+    for (int i2 = c0; i2 <= c1; i2++)
+        for (int i1 = 1; i1 <= n1; i1++)
+            dst[(size_t)(i1 - 1) + (size_t)(i2 - 1) * n1] = src[(size_t)(i1 + npml - 1) + (size_t)(i2 + npml - 1) * n1e];
+}
+
+static void SFX(grab_interior)(const SFX(setup_t) *S, R *dst, const R *src)
+{
+#pragma omp parallel if (OMP_BIG(S))
+    {
+        int c0, c1;
+        SFX(my_cols)(1, S->n2, &c0, &c1);
+        SFX(grab_cols)(dst, src, S->n1, S->npml, S->n1e, c0, c1);
+    }
+}
+
+/* Extended source of the re-injection pass (:1121-1125, :1175-1186):
+ * f(i1+npml, i2+npml) = f + ((ss*dt)*snap(i1,i2))*w(i1+npml,i2+npml) over
+ * i1 = lo..hi1, i2 = lo..hi2, with w = kappa, or w = 1 (bue) when kap is
+ * NULL; snap is the n1*n2 slice of the step. */
+static void SFX(reinject_cols)(R *f, const R *snap, const R *kap, R ssdt, int n1, int npml, int n1e, int lo, int hi1,
+                               int c0, int c1)
+{
+    for (int i2 = c0; i2 <= c1; i2++)
+        for (int i1 = lo; i1 <= hi1; i1++) {
+            const size_t e = (size_t)(i1 + npml - 1) + (size_t)(i2 + npml - 1) * n1e;
+            const R sv = snap[(size_t)(i1 - 1) + (size_t)(i2 - 1) * n1];
+            f[e] = f[e] + ssdt * sv * (kap ? kap[e] : (R)1);
+        }
+}
+
+static void SFX(reinject)(const SFX(setup_t) *S, R *f, const R *snap, const R *kap, R ssdt, int lo, int hi1, int hi2)
+{
+#pragma omp parallel if (OMP_BIG(S))
+    {
+        int c0, c1;
+        SFX(my_cols)(lo, hi2, &c0, &c1);
+        SFX(reinject_cols)(f, snap, kap, ssdt, S->n1, S->npml, S->n1e, lo, hi1, c0, c1);
+    }
+}
+
 /* p = px + pz; free surface p(1:npml+1,:) = 0 (:183-189) */
 static void SFX(p_sum)(SFX(setup_t) *S, R *p, const R *px, const R *pz, int ofs)
 {
     const int n1e = S->n1e, n2e = S->n2e;
-    size_t ne = (size_t)n1e * n2e;
-    for (size_t k = 0; k < ne; k++) p[k] = px[k] + pz[k];
+    long ne = (long)n1e * n2e;
+#pragma omp parallel for schedule(static) if (OMP_BIG(S))
+    for (long k = 0; k < ne; k++) p[k] = px[k] + pz[k];
     if (ofs == 1)
         for (int i2 = 1; i2 <= n2e; i2++)
             for (int i1 = 1; i1 <= S->npml + 1; i1++) E(p, i1, i2) = 0;
@@ -198,6 +279,42 @@ static void SFX(hicks_inject)(SFX(setup_t) *S, R *f, const int *tab5, int stride
             else if (mode == 1) E(f, ix1 + i1, ix2 + i2) = E(f, ix1 + i1, ix2 + i2) - t * E(S->kappa, ix1 + i1, ix2 + i2);
             else E(f, ix1 + i1, ix2 + i2) = E(f, ix1 + i1, ix2 + i2) + t * (R)1;
         }
+}
+
+/* hicks_inject mode 1 for every receiver r = 0..nr-1 of (nr,5) table ir with
+ * amplitudes amp[r*amp_stride].  Windows of neighbouring receivers overlap,
+ * so each grid point must take its adds in receiver order: threads own
+ * disjoint column ranges and walk all receivers in order, each applying the
+ * part of the window in its columns. */
+static void SFX(hicks_inject_receivers)(SFX(setup_t) *S, R *f, const int *ir, int nr, const float *rt,
+                                        const R *amp, size_t amp_stride)
+{
+#pragma omp parallel if (OMP_BIG(S))
+    {
+#ifdef _OPENMP
+        const int nth = omp_get_num_threads(), th = omp_get_thread_num();
+#else
+        const int nth = 1, th = 0;
+#endif
+        const int n1e = S->n1e, n2e = S->n2e;
+        const int c0 = 1 + (int)((long)n2e * th / nth), c1 = (int)((long)n2e * (th + 1) / nth);
+        for (int r = 0; r < nr; r++) {
+            const int *tab5 = ir + r;
+            const float *coef441 = rt + (size_t)r * 441;
+            const R amp_r = amp[(size_t)r * amp_stride];
+            int ix1 = tab5[0], ix2 = tab5[nr];
+            int r1min = 1 - ix1 > -10 ? 1 - ix1 : -10, r1max = n1e - ix1 < 10 ? n1e - ix1 : 10;
+            int r2min = 1 - ix2 > -10 ? 1 - ix2 : -10, r2max = n2e - ix2 < 10 ? n2e - ix2 : 10;
+            if (ix2 + r2min < c0) r2min = c0 - ix2;
+            if (ix2 + r2max > c1) r2max = c1 - ix2;
+            for (int i1 = r1min; i1 <= r1max; i1++)
+                for (int i2 = r2min; i2 <= r2max; i2++) {
+                    R c = (R)coef441[(i1 + 10) + 21 * (i2 + 10)];
+                    R t = c * S->dt * amp_r;
+                    E(f, ix1 + i1, ix2 + i2) = E(f, ix1 + i1, ix2 + i2) - t * E(S->kappa, ix1 + i1, ix2 + i2);
+                }
+        }
+    }
 }
 
 /* Hicks record sis(it,r) += c*p over the extended-grid-clipped window (:199-224) */
@@ -265,9 +382,8 @@ int SFX(oracle_forward)(const float *vp, int n1, int n2, int npml, float h, int 
         if (type_src == 0) SFX(hicks_inject)(&S, F.px, is5, 1, stile, (R)s[it - 1], 0);
         SFX(p_sum)(&S, F.p, F.px, F.pz, ofs);
         if (type_rec == 0) {
-            if (wavefield)
-                for (int i2 = 1; i2 <= n2; i2++)
-                    for (int i1 = 1; i1 <= n1; i1++) I3(wavefield, i1, i2, it) = E(F.p, i1 + npml, i2 + npml);
+            if (wavefield) SFX(grab_interior)(&S, wavefield + (size_t)(it - 1) * n1 * n2, F.p);
+#pragma omp parallel for schedule(static) if (OMP_BIG(&S))
             for (int r = 0; r < nr; r++)
                 sis[(size_t)(it - 1) + (size_t)r * nt] = SFX(hicks_record)(&S, F.p, ir + r, nr, rt + (size_t)r * 441);
         }
@@ -313,6 +429,9 @@ void SFX(snap_quantize)(R *a, int n1, int n2, int npml, int nt, int fmt)
             }
 }
 
+static _Thread_local R *SFX(snap_buf);
+static _Thread_local size_t SFX(snap_cap);
+
 static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt,
                                     float dt, const R *res, int type_rec, int nr, const int *ir, R *sis,
                                     R *wavefieldp, int variant, R *snap_cost, int exact, int snap_fmt)
@@ -322,8 +441,18 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
     const int n1e = S.n1e, n2e = S.n2e;
     size_t ne = (size_t)n1e * n2e;
     size_t nint3 = (size_t)n1 * n2 * nt;
-    R *sp = calloc(nint3, sizeof(R)), *svx = calloc(nint3, sizeof(R)), *svz = calloc(nint3, sizeof(R));
-    if (!sp || !svx || !svz) { free(sp); free(svx); free(svz); SFX(setup_free)(&S); return -1; }
+    /* The snapshot store is kept per calling thread between calls (pass 1
+     * writes every element before pass 2 reads it; the exact variant clears
+     * it first): first-touch page faults on ~1.5 GB per call otherwise
+     * dominate the operator's time at the 121x401, nt = 2500 size. */
+    if (3 * nint3 > SFX(snap_cap)) {
+        free(SFX(snap_buf));
+        SFX(snap_buf) = malloc(3 * nint3 * sizeof(R));
+        SFX(snap_cap) = SFX(snap_buf) ? 3 * nint3 : 0;
+    }
+    if (!SFX(snap_buf)) { SFX(setup_free)(&S); return -1; }
+    R *sp = SFX(snap_buf), *svx = sp + nint3, *svz = svx + nint3;
+    if (exact) memset(sp, 0, 3 * nint3 * sizeof(R));
     SFX(fields_t) F;
     SFX(fields_alloc)(&F, ne);
     float *rt = SFX(tiles_for)(ir, nr, ofs);
@@ -331,18 +460,13 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
     if (exact) SFX(exact_pass1)(&S, ofs, nt, res, nr, ir, rt, variant, sp, svx, svz);
     for (int it = 1; it <= (exact ? 0 : nt); it++) {
         SFX(p_update)(&S, F.px, F.pz, F.vx, F.vz, 3, n1e - 3, n2e - 3, -1);
-        for (int r = 0; r < nr; r++)
-            SFX(hicks_inject)(&S, F.px, ir + r, nr, rt + (size_t)r * 441, res[(size_t)(it - 1) + (size_t)r * nt], 1);
+        SFX(hicks_inject_receivers)(&S, F.px, ir, nr, rt, res + (it - 1), (size_t)nt);
         SFX(p_sum)(&S, F.p, F.px, F.pz, ofs);
-        for (int i2 = 1; i2 <= n2; i2++)
-            for (int i1 = 1; i1 <= n1; i1++) I3(sp, i1, i2, it) = E(F.p, i1 + npml, i2 + npml);
+        SFX(grab_interior)(&S, sp + (size_t)(it - 1) * n1 * n2, F.p);
         SFX(v_update)(&S, F.vx, F.vz, F.p, 3, n1e - 3, n2e - 3, -1);
         SFX(vz_mirror)(&S, F.vz, ofs);
-        for (int i2 = 1; i2 <= n2; i2++)
-            for (int i1 = 1; i1 <= n1; i1++) {
-                I3(svx, i1, i2, it) = E(F.vx, i1 + npml, i2 + npml);
-                I3(svz, i1, i2, it) = E(F.vz, i1 + npml, i2 + npml);
-            }
+        SFX(grab_interior)(&S, svx + (size_t)(it - 1) * n1 * n2, F.vx);
+        SFX(grab_interior)(&S, svz + (size_t)(it - 1) * n1 * n2, F.vz);
     }
     if (snap_fmt == 1 || snap_fmt == 2) {
         SFX(snap_quantize)(sp, n1, n2, npml, nt, snap_fmt);
@@ -363,29 +487,22 @@ static int SFX(oracle_esource_impl)(const float *vp, int n1, int n2, int npml, f
     for (int it = 1; it <= nt; it++) {
         SFX(p_update)(&S, F.px, F.pz, F.vx, F.vz, 3, n1e - 3, n2e - 3, +1);
         int ic = nt - it + 1;
-        for (int i2 = plo; i2 <= phi2; i2++)
-            for (int i1 = plo; i1 <= phi1; i1++)
-                E(F.px, i1 + npml, i2 + npml) = E(F.px, i1 + npml, i2 + npml) +
-                    ss * S.dt * I3(sp, i1, i2, ic) * E(S.kappa, i1 + npml, i2 + npml);
+        const R ssdt = ss * S.dt;
+        const size_t slice = (size_t)(ic - 1) * n1 * n2;
+        SFX(reinject)(&S, F.px, sp + slice, S.kappa, ssdt, plo, phi1, phi2);
         SFX(p_sum)(&S, F.p, F.px, F.pz, ofs);
         if (type_rec == 0) {
-            if (wavefieldp)
-                for (int i2 = 1; i2 <= n2; i2++)
-                    for (int i1 = 1; i1 <= n1; i1++) I3(wavefieldp, i1, i2, it) = E(F.p, i1 + npml, i2 + npml);
+            if (wavefieldp) SFX(grab_interior)(&S, wavefieldp + (size_t)(it - 1) * n1 * n2, F.p);
+#pragma omp parallel for schedule(static) if (OMP_BIG(&S))
             for (int r = 0; r < nr; r++)
                 sis[(size_t)(it - 1) + (size_t)r * nt] = SFX(hicks_record)(&S, F.p, ir + r, nr, rt + (size_t)r * 441);
         }
         SFX(v_update)(&S, F.vx, F.vz, F.p, 3, n1e - 3, n2e - 3, +1);
-        for (int i2 = 1; i2 <= n2; i2++)
-            for (int i1 = 1; i1 <= n1; i1++)
-                E(F.vx, i1 + npml, i2 + npml) = E(F.vx, i1 + npml, i2 + npml) + ss * S.dt * I3(svx, i1, i2, ic) * (R)1;
-        for (int i2 = 1; i2 <= n2; i2++)
-            for (int i1 = 1; i1 <= n1; i1++)
-                E(F.vz, i1 + npml, i2 + npml) = E(F.vz, i1 + npml, i2 + npml) + ss * S.dt * I3(svz, i1, i2, ic) * (R)1;
+        SFX(reinject)(&S, F.vx, svx + slice, NULL, ssdt, 1, n1, n2);
+        SFX(reinject)(&S, F.vz, svz + slice, NULL, ssdt, 1, n1, n2);
         SFX(vz_mirror)(&S, F.vz, ofs);
     }
     free(rt);
-    free(sp); free(svx); free(svz);
     SFX(fields_free)(&F);
     SFX(setup_free)(&S);
     return 0;
@@ -426,6 +543,7 @@ int SFX(oracle_resample)(int n1, int n2, int npml, int nt, const int *ir, int nr
                          const R *wavefield)
 {
     float *rt = SFX(tiles_for)(ir, nr, ofs);
+#pragma omp parallel for schedule(static) if ((long)nt * nr > 4096)
     for (int it = 1; it <= nt; it++)
         for (int r = 0; r < nr; r++) {
             int ix1 = ir[r] - npml, ix2 = ir[r + nr] - npml;
