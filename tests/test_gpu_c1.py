@@ -6,11 +6,14 @@ at z = 400 m) through the batched GPU pipeline, twice:
    products summed in dot_product's and datanorm's order, REAL scalars and
    vector updates) against the FP32 oracle, the reference's own CPU path
    restated operation for operation (pinned bit-exact against the reference
-   build by tests/test_oracle.py).  The propagators are bit-exact, so the
-   whole solve is expected bit for bit; the bound asserted is the 1e-4
-   parity bar and the measured figures are printed.
+   build by tests/test_oracle.py).  The propagators are bit-exact and so
+   is the solve: deltade and the MINRES history must equal the oracle's bit
+   for bit.  The gathers are within 1e-6 (measured 1.3e-7): the GPU records
+   u^r and u^e at the datum on the fly and adds the two gathers, the
+   reference adds the wavefields and then resamples.
 2. The default MINRES (FP64 dot products and scalars) against the FP64
-   oracle of the plain method, within 1e-4.
+   oracle of the plain method, within 1e-4 (measured: deltade 2.3e-5,
+   gathers 7.0e-6, history 8.1e-6 max relative).
 
 Quantities: deltade (the MINRES solution), the redatumed gathers of
 u^r + u^e at the datum (unfiltered; the oracle resamples the summed
@@ -52,6 +55,7 @@ def _errors(got, want):
             "hist_bitexact": bool(np.array_equal(np.asarray(hist, np.float32), np.asarray(ho, np.float32)))}
 
 
+@pytest.mark.timeout(600)
 def test_c1_pipeline_both_arithmetics(gpu_daw):
     import torch
     syn = importlib.import_module("daw_amd.synth")
@@ -93,5 +97,6 @@ def test_c1_pipeline_both_arithmetics(gpu_daw):
     print("C1 f64 GPU vs FP64 oracle:", e64, flush=True)
     print("C1 ref32 GPU vs FP64 oracle:", _errors(g32, o64), flush=True)
     print("C1 FP32 oracle vs FP64 oracle:", _errors(o32, o64), flush=True)
-    for e in (e32, e64):
-        assert max(e["deltade_rel_l2"], e["datum_rel_l2"], e["hist_rel_l2"], e["hist_max_rel"]) <= BAR, e
+    assert e32["deltade_bitexact"] and e32["hist_bitexact"], e32
+    assert e32["datum_rel_l2"] <= 1e-6, e32
+    assert max(e64["deltade_rel_l2"], e64["datum_rel_l2"], e64["hist_rel_l2"], e64["hist_max_rel"]) <= BAR, e64
