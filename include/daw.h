@@ -154,6 +154,17 @@ int daw_subresample(const float *xin, float *xout, int n1in, int n2, int n1out, 
  * [n2][n1out], not overlapping). */
 int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
 
+/* daw_subresample_dev that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The resampling plan and table go to a fixed device array of
+ * the library (one per GPU, held until the process ends) instead of
+ * buffers allocated per call, so n1out is limited to 65536 output samples
+ * (DAW_ERR_ARG beyond).  The library's lock serializes calls within a
+ * process; the array is the last call's, so calls that overlapped on one
+ * GPU would race.  Results are identical to daw_subresample_dev. */
+int daw_subresample_dev_noalloc(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
+
 /* filmartin2d(deltadeo,nr,nt,dt,flow,fhig,llow,lhig,deltade),
  * minresModule.f90:486-614. */
 int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,

@@ -38,6 +38,7 @@ EXPORTS = [
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
     "daw_get_launch", "daw_sublinearminres_log", "daw_write_minres_log", "daw_readpar", "daw_read_acqui",
     "daw_subresample", "daw_subresample_dev", "daw_set_minres_arith",
+    "daw_subresample_dev_noalloc",
 ]
 
 
@@ -156,6 +157,9 @@ def lib():
                                               _f32p, _f32p, C.c_char_p]
         L.daw_subresample.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
         L.daw_subresample_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
+        L.daw_subresample_dev_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                                  C.c_float]
+        L.daw_set_minres_arith.argtypes = [C.c_int]
         L.daw_write_minres_log.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_float, _f64p]
         L.daw_readpar.argtypes = [C.c_char_p, C.POINTER(Par)]
         L.daw_read_acqui.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int,

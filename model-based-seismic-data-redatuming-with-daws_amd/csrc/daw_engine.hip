@@ -1395,6 +1395,65 @@ int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1o
     return time_resample_dev(xin, xout, n1in, n2, n1out, d1in, d1out);
 }
 
+// ------------------------------------------------------------------ subresample without allocation
+// Scratch of daw_subresample_dev_noalloc: the per-output-sample plan (it0,
+// ialpha) of up to kNoAllocMaxOut output samples, then the 9 x 10 table.
+// Each GPU holds its own copy of this array for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress gives
+// for that device (the array's name is a host-side stand-in).
+namespace {
+constexpr int kNoAllocMaxOut = 65536;
+constexpr int kNoAllocCoef = 2 * kNoAllocMaxOut;            // int offset of the table
+constexpr int kNoAllocInts = kNoAllocCoef + 9 * kRsNalpha;
+constexpr int kMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) int scratch[kNoAllocInts];
+static int *g_scratch_dev[kMaxDevices];          // per device: device address of scratch
+static int g_scratch_host[kNoAllocInts];         // staging of the plan and table (under g.mu)
+
+// The device address of this device's scratch, looked up once per device.
+static int scratch_ptr(int *&ptr)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(DAW_ERR_ARG, "subresample_dev_noalloc: device %d", dev);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<int *>(p);
+    }
+    ptr = g_scratch_dev[dev];
+    return 0;
+}
+
+int daw_subresample_dev_noalloc(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (int rc = check_resample(n1in, n2, n1out, d1in, d1out)) return rc;
+    if (n1out > kNoAllocMaxOut)
+        return fail(DAW_ERR_ARG, "subresample_dev_noalloc: n1out=%d exceeds the %d output samples its scratch holds",
+                    n1out, kNoAllocMaxOut);
+    const char *a0 = reinterpret_cast<const char *>(xin), *b0 = reinterpret_cast<const char *>(xout);
+    const char *a1 = a0 + (size_t)n2 * n1in * sizeof(float), *b1 = b0 + (size_t)n2 * n1out * sizeof(float);
+    if (xin && xout && (size_t)n2 * n1out > 0 && a0 < b1 && b0 < a1)
+        return fail(DAW_ERR_ARG, "subresample_dev_noalloc: input and output traces overlap");
+    if ((size_t)n2 * n1out == 0) return 0;
+    if (!xin || !xout) return fail(DAW_ERR_ARG, "subresample_dev_noalloc: null trace pointer");
+    if (int rc = ensure_stream()) return rc;
+    int *ptr = nullptr;
+    if (int rc = scratch_ptr(ptr)) return rc;
+    resample_plan_into(n1out, d1in, d1out, g_scratch_host);
+    resample_table(reinterpret_cast<float *>(g_scratch_host + kNoAllocCoef));
+    HIPCHK(hipMemcpyAsync(ptr, g_scratch_host, (size_t)2 * n1out * sizeof(int), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(ptr + kNoAllocCoef, g_scratch_host + kNoAllocCoef, 9 * kRsNalpha * sizeof(int),
+                          hipMemcpyHostToDevice, g.stream));
+    launch_time_resample(xin, xout, n1in, n1out, (size_t)n2, ptr, reinterpret_cast<const float *>(ptr + kNoAllocCoef),
+                         g.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return 0;
+}
+
 int daw_submodeling(const float *vp, int n1, int n2, int npml, float h, int ofs, int nt, float dt, const float *s,
                     int type_src, int type_rec, const int *is5, const int *ir, int nr, float *sis, float *wavefield)
 {

@@ -65,6 +65,11 @@ void resample_table(float *out) { hicks1d_table(kRsRad, kRsNalpha, 6.31f, out); 
 void resample_plan(int n1out, float d1in, float d1out, std::vector<int> &plan)
 {
     plan.resize(2 * static_cast<size_t>(n1out));
+    resample_plan_into(n1out, d1in, d1out, plan.data());
+}
+
+void resample_plan_into(int n1out, float d1in, float d1out, int *plan)
+{
     for (int i1 = 1; i1 <= n1out; ++i1) {
         const float tout = static_cast<float>(i1 - 1) * d1out;
         const int it0 = static_cast<int>(std::lround(tout / d1in)) + 1;     // NINT of a REAL

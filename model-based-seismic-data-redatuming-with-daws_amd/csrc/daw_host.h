@@ -35,6 +35,8 @@ void resample_table(float *out);
 // Per output sample i1 (0-based) of subresample: it0 (1-based nearest input
 // sample) and ialpha, computed as subresample_hicks.f90:59-66 computes them.
 void resample_plan(int n1out, float d1in, float d1out, std::vector<int> &plan);
+// the same into plan[2*n1out] (no allocation)
+void resample_plan_into(int n1out, float d1in, float d1out, int *plan);
 
 // One 21x21 tile of the 2-D monopole table for (ialpha1, ialpha2, ip) as
 // kwsinc2d(ofs) leaves it (kwsinc.f90:212-378); out[(n1+10) + 21*(n2+10)].
