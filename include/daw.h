@@ -109,6 +109,15 @@ int daw_set_minres_arith(int arith);
  * (daw_set_hessian(1)) always keeps FP32.  No reference counterpart. */
 int daw_set_snapshot_format(int fmt);
 
+/* Storage of the MINRES Krylov vectors r1, r2 and w, w1, w2 (the reference's
+ * work vectors, minresModule.f90:29) in the default (FP64) MINRES arithmetic:
+ * 0 = FP32 (default), 1 = BF16, 2 = FP16, each 256-element chunk scaled by a
+ * power of two at or above its max |value|.  Dot products, norms and scalars
+ * stay FP64, v, y and x FP32; the 16-bit vectors are decoded where they are
+ * read.  Ignored by the reference arithmetic (daw_set_minres_arith(1)).  No
+ * reference counterpart. */
+int daw_set_krylov_format(int fmt);
+
 /* subwavefieldresample(n1,n2,npml,nt,dt,ir,nr,sis,wavefield),
  * subwavefieldresample.f90:1-41. */
 int daw_subwavefieldresample(int n1, int n2, int npml, int nt, float dt, const int *ir, int nr, float *sis,
@@ -166,12 +175,15 @@ int daw_subresample_dev(const float *xin, float *xout, int n1in, int n2, int n1o
 int daw_subresample_dev_noalloc(const float *xin, float *xout, int n1in, int n2, int n1out, float d1in, float d1out);
 
 /* filmartin2d(deltadeo,nr,nt,dt,flow,fhig,llow,lhig,deltade),
- * minresModule.f90:486-614. */
+ * minresModule.f90:486-614, on host traces: copied to the GPU, filtered as
+ * daw_filmartin2d_dev does, copied back. */
 int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
                     float *dout);
 
-/* filmartin2d on device traces din/dout [nr][nt] (may alias); FFT lengths
- * up to 8192 run on the GPU, longer ones on the host. */
+/* filmartin2d on device traces din/dout [nr][nt] (may alias).  Every FFT
+ * length of the reference's list (up to 65536) runs on the GPU in FP64:
+ * in LDS up to 8192, as a four-step transform through an HBM work area of up
+ * to 256 MB for 16384-65536. */
 int daw_filmartin2d_dev(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
                         float *dout);
 

@@ -38,7 +38,7 @@ EXPORTS = [
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
     "daw_get_launch", "daw_sublinearminres_log", "daw_write_minres_log", "daw_readpar", "daw_read_acqui",
     "daw_subresample", "daw_subresample_dev", "daw_set_minres_arith",
-    "daw_subresample_dev_noalloc",
+    "daw_subresample_dev_noalloc", "daw_set_krylov_format",
 ]
 
 
@@ -160,6 +160,7 @@ def lib():
         L.daw_subresample_dev_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float,
                                                   C.c_float]
         L.daw_set_minres_arith.argtypes = [C.c_int]
+        L.daw_set_krylov_format.argtypes = [C.c_int]
         L.daw_write_minres_log.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_float, _f64p]
         L.daw_readpar.argtypes = [C.c_char_p, C.POINTER(Par)]
         L.daw_read_acqui.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int,
@@ -272,6 +273,15 @@ def submodeling_transpose(vp, npml, h, ofs, nt, dt, d, src_row, rec_tab):
 def set_hessian(exact):
     """0: the reference's Hessian-type operator; 1: exact (symmetric) variant."""
     _check(lib().daw_set_hessian(1 if exact else 0))
+
+
+def set_krylov_format(fmt):
+    """Storage of the MINRES Krylov vectors r1, r2, w, w1, w2 in the FP64
+    MINRES: "fp32" (default), "bf16" or "fp16" (per-256-element power-of-two
+    scales; dot products and scalars stay FP64)."""
+    if fmt not in SNAPSHOT_FORMATS:
+        raise ValueError(f"Krylov vector format {fmt!r}: one of {sorted(SNAPSHOT_FORMATS)}")
+    _check(lib().daw_set_krylov_format(SNAPSHOT_FORMATS[fmt]))
 
 
 MINRES_ARITH = {"f64": 0, "ref32": 1}

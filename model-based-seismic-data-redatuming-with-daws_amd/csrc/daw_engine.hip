@@ -73,6 +73,7 @@ struct Global {
     int snap_fmt = kSnapF32;   // snapshot store of the reference Hessian (SnapFormat)
     bool graph = false;        // replay full sweeps as captured hipGraphs
     int minres_arith = 0;      // 0: FP64 dot products and scalars; 1: the reference's FP32 arithmetic
+    int krylov_fmt = kSnapF32; // storage of MINRES r1, r2, w, w1, w2 in the FP64 path (SnapFormat)
     std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw} per mode
     LaunchCfg last_cfg[3];     // launch configuration of each mode in the last tuned batch
 };
@@ -1008,8 +1009,9 @@ static int build_batch(Batch &B, const Grid &G, const DevModel *M, int nt, int n
 }
 
 // ------------------------------------------------------------------ band-pass on device traces
-// filmartin2d (minresModule.f90:486-614) for ntr traces [tr][t] in HBM.  FFT
-// lengths up to 8192 run in LDS on the GPU; longer ones on the host.
+// filmartin2d (minresModule.f90:486-614) for ntr traces [tr][t] in HBM, FP64
+// transforms on the GPU for the whole FFT length list (nto <= 65536): in LDS
+// up to 8192, as a four-step FFT through an HBM work area beyond.
 static int filter_device(const float *din, float *dout, int ntr, int nt, float dt, float flow, float fhig, float llow,
                          float lhig)
 {
@@ -1017,25 +1019,30 @@ static int filter_device(const float *din, float *dout, int ntr, int nt, float d
     const int nto = filter_design(nt, dt, flow, fhig, llow, lhig, w);
     if (nto < 0) return fail(DAW_ERR_ARG, "filmartin2d: nt=%d exceeds the 65536-sample FFT length list", nt);
     if (ntr <= 0) return 0;
-    if (nto > 8192) {
-        std::vector<float> h((size_t)ntr * nt), o((size_t)ntr * nt);
-        HIPCHK(hipStreamSynchronize(g.stream));
-        HIPCHK(hipMemcpy(h.data(), din, h.size() * 4, hipMemcpyDeviceToHost));
-        filter_apply_host(h.data(), ntr, nt, dt, flow, fhig, llow, lhig, o.data());
-        HIPCHK(hipMemcpy(dout, o.data(), o.size() * 4, hipMemcpyHostToDevice));
-        return 0;
-    }
-    std::vector<double> tw((size_t)nto);   // nto/2 complex twiddles
-    for (int m = 0; m < nto / 2; ++m) {
+    const bool lds = nto <= 8192;
+    // twiddles exp(-2 pi i m / nto): m < nto/2 for the LDS transform, m < nto for the four-step one
+    const int ntw = lds ? nto / 2 : nto;
+    std::vector<double> tw((size_t)2 * ntw);
+    for (int m = 0; m < ntw; ++m) {
         const double ang = 2.0 * M_PI * (double)m / (double)nto;
-        tw[2 * m] = std::cos(ang);
-        tw[2 * m + 1] = -std::sin(ang);
+        tw[2 * (size_t)m] = std::cos(ang);
+        tw[2 * (size_t)m + 1] = -std::sin(ang);
     }
     w.resize(nto / 2 + 1);
-    DevBuf d_w, d_tw;
+    DevBuf d_w, d_tw, work;
     if (int rc = upload(d_w, w)) return rc;
     if (int rc = upload(d_tw, tw)) return rc;
-    launch_filter(din, dout, ntr, nt, nto, d_w.as<double>(), d_tw.as<double2>(), g.stream);
+    if (lds) {
+        launch_filter(din, dout, ntr, nt, nto, d_w.as<double>(), d_tw.as<double2>(), g.stream);
+    } else {
+        // work area of up to ~256 MB: trace pairs in groups
+        const int npairs = (ntr + 1) / 2;
+        const int group = std::max(1, std::min(npairs, (int)((256u << 20) / ((size_t)nto * sizeof(double2)))));
+        if (int rc = work.alloc((size_t)group * nto * sizeof(double2))) return rc;
+        if (launch_filter_long(din, dout, ntr, nt, nto, d_w.as<double>(), d_tw.as<double2>(), work.as<double2>(), group,
+                               g.stream) != 0)
+            return fail(DAW_ERR_ARG, "filmartin2d: FFT length %d", nto);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(g.stream));
     return 0;
@@ -1045,6 +1052,7 @@ static int filter_device(const float *din, float *dout, int ntr, int nt, float d
 struct MinresWork {
     DevBuf vec;     // x, v, Y[3], Wv[3], hb: 9 vectors per shot
     DevBuf off, len, nts, part0, part1, scal, hist, trace;
+    DevBuf coff, q16, s16;   // 16-bit Krylov vectors: chunk offsets, values, chunk scales
 };
 
 // Per-shot MINRES log block handed to callers: kLogHead doubles (fac, Anorm,
@@ -1115,10 +1123,37 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
         launch_dot(hb, hb, doff, dlen, w.part1.as<double>(), ns, g.stream);
         launch_minres_init(w.part0.as<double>(), w.part1.as<double>(), dscal, mu, ns, g.stream);
     }
-    // y = r1 = r2 = b; x = w = w2 = 0
-    for (int k = 0; k < 3; ++k) HIPCHK(hipMemcpyAsync(Y[k], b, blk * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
+    // 16-bit Krylov vectors (FP64 path only): R16[0..1] hold r1/r2, W16[0..2] w/w1/w2
+    const int kf = ref32 ? (int)kSnapF32 : g.krylov_fmt;
+    const bool k16 = kf != kSnapF32;
+    unsigned short *Rq[2] = {nullptr, nullptr}, *Wq[3] = {nullptr, nullptr, nullptr};
+    float *Rs[2] = {nullptr, nullptr}, *Ws[3] = {nullptr, nullptr, nullptr};
+    const size_t *dcoff = nullptr;
+    if (k16) {
+        std::vector<size_t> coff(ns);
+        size_t nch = 0;
+        for (int s = 0; s < ns; ++s) {
+            coff[s] = nch;
+            nch += ((size_t)len[s] + kBlock - 1) / kBlock;
+        }
+        if (int rc = upload(w.coff, coff)) return rc;
+        dcoff = w.coff.as<size_t>();
+        if (int rc = w.q16.alloc(5 * blk * sizeof(unsigned short))) return rc;
+        if (int rc = w.s16.alloc(5 * std::max<size_t>(nch, 1) * sizeof(float))) return rc;
+        unsigned short *q = w.q16.as<unsigned short>();
+        float *sc = w.s16.as<float>();
+        for (int k = 0; k < 2; ++k) { Rq[k] = q + k * blk; Rs[k] = sc + k * nch; }
+        for (int k = 0; k < 3; ++k) { Wq[k] = q + (2 + k) * blk; Ws[k] = sc + (2 + k) * nch; }
+        launch_mk_init(kf, b, Rq[0], Rs[0], Rq[1], Rs[1], doff, dcoff, dlen, ns, g.stream);
+        launch_mk_zero(Wq[0], Ws[0], Wq[1], Ws[1], Wq[2], Ws[2], blk, nch, g.stream);
+    } else {
+        // y = r1 = r2 = b; w = w1 = w2 = 0
+        for (int k = 0; k < 3; ++k)
+            HIPCHK(hipMemcpyAsync(Y[k], b, blk * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
+        for (int k = 0; k < 3; ++k) launch_fill(Wv[k], blk, 0.0f, g.stream);
+    }
     launch_fill(x, blk, 0.0f, g.stream);
-    for (int k = 0; k < 3; ++k) launch_fill(Wv[k], blk, 0.0f, g.stream);
+    int ia = 0, ib = 1;                  // 16-bit roles: r1 = R16[ia], r2 = R16[ib]
     int iy = 0, ir1 = 1, ir2 = 2;        // roles in Y
     int iw = 0, iw1 = 1, iw2 = 2;        // roles in Wv
     std::vector<double> scal((size_t)ns * nsc);
@@ -1131,6 +1166,24 @@ static int minres_batch(Batch &B, const float *b, float *x, float mu, int itnlim
         if (!any) break;
         // y of the previous iteration lives in the r2 buffer (r2 = y, :193); the
         // free buffer Y[iy] receives the new operator output
+        if (k16) {
+            launch_mk_v(kf, Rq[ib], Rs[ib], v, doff, dcoff, dlen, dscal, ns, g.stream);
+            HessOut h;
+            h.sis = Y[0]; h.reverse = 1; h.variant = 0;
+            if (int rc = hessian(B, v, h)) return rc;
+            launch_mk_l1(kf, Y[0], v, Rq[ia], Rs[ia], doff, dcoff, dlen, dscal, w.part0.as<double>(), ns, g.stream);
+            // the new r2 (encode(y)) goes over r1, which this iteration no longer reads
+            launch_mk_l2(kf, Y[0], Rq[ib], Rs[ib], Rq[ia], Rs[ia], doff, dcoff, dlen, dscal, w.part1.as<double>(), ns,
+                         g.stream);
+            std::swap(ia, ib);
+            launch_minres_scalars(w.part1.as<double>(), dscal, w.hist.as<double>(), trace, dlen, itnlim, rtol, ns,
+                                  g.stream);
+            launch_mk_x(kf, x, Wq[iw1], Ws[iw1], Wq[iw], Ws[iw], Wq[iw2], Ws[iw2], v, doff, dcoff, dlen, dscal, trace,
+                        itnlim, ns, g.stream);
+            const int nw = iw1, nw2 = iw, nw1 = iw2;
+            iw = nw; iw2 = nw2; iw1 = nw1;
+            continue;
+        }
         if (ref32) launch_minres32_v(Y[ir2], v, doff, dlen, dscal, ns, g.stream);
         else launch_minres_v(Y[ir2], v, doff, dlen, dscal, ns, g.stream);
         HessOut h;
@@ -1668,9 +1721,19 @@ int daw_read_acqui(const char *path, int *nsrc, int *nrec, float *src_zx, int sr
 int daw_filmartin2d(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow, float lhig,
                     float *dout)
 {
+    std::lock_guard<std::mutex> lk(g.mu);
     if (nr < 0 || nt < 1 || (nr > 0 && (!din || !dout))) return fail(DAW_ERR_ARG, "filmartin2d: bad argument");
-    if (filter_apply_host(din, nr, nt, dt, flow, fhig, llow, lhig, dout) != 0)
+    std::vector<double> w;
+    if (filter_design(nt, dt, flow, fhig, llow, lhig, w) < 0)
         return fail(DAW_ERR_ARG, "filmartin2d: nt=%d exceeds the 65536-sample FFT length list", nt);
+    if (nr == 0) return 0;
+    if (int rc = ensure_stream()) return rc;
+    const size_t n = (size_t)nr * nt;
+    DevBuf d;
+    if (int rc = d.alloc(n * sizeof(float))) return rc;
+    HIPCHK(hipMemcpy(d.p, din, n * sizeof(float), hipMemcpyHostToDevice));
+    if (int rc = filter_device(d.as<float>(), d.as<float>(), nr, nt, dt, flow, fhig, llow, lhig)) return rc;
+    HIPCHK(hipMemcpy(dout, d.p, n * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1956,6 +2019,14 @@ int daw_set_hessian(int exact)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     g.hess_exact = exact != 0;
+    return 0;
+}
+
+int daw_set_krylov_format(int fmt)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (fmt < kSnapF32 || fmt > kSnapF16) return fail(DAW_ERR_ARG, "Krylov vector format %d: 0, 1 or 2", fmt);
+    g.krylov_fmt = fmt;
     return 0;
 }
 

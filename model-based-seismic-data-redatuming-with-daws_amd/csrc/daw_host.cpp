@@ -5,7 +5,6 @@
 #include "daw_host.h"
 
 #include <cmath>
-#include <complex>
 #include <cstdlib>
 #include <cstring>
 
@@ -239,56 +238,6 @@ int filter_design(int nt, float dt, float flow, float fhig, float llow, float lh
     w.assign(nto + 1, 0.0);
     for (int j = 1; j <= nto + 1; ++j) w[j - 1] = static_cast<double>(f[j] + 0.0001f);
     return nto;
-}
-
-namespace {
-// In-place iterative radix-2 complex FFT (sign -1 forward), FP64.
-void fft_inplace(std::vector<std::complex<double>> &a, int sign)
-{
-    const int n = static_cast<int>(a.size());
-    for (int i = 1, j = 0; i < n; ++i) {
-        int bit = n >> 1;
-        for (; j & bit; bit >>= 1) j ^= bit;
-        j ^= bit;
-        if (i < j) std::swap(a[i], a[j]);
-    }
-    for (int len = 2; len <= n; len <<= 1) {
-        const int half = len / 2;
-        for (int k = 0; k < half; ++k) {
-            const double ang = sign * 2.0 * M_PI * k / len;
-            const std::complex<double> tw(std::cos(ang), std::sin(ang));
-            for (int i = k; i < n; i += len) {
-                const std::complex<double> u = a[i], v = a[i + half] * tw;
-                a[i] = u + v;
-                a[i + half] = u - v;
-            }
-        }
-    }
-}
-}  // namespace
-
-int filter_apply_host(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow,
-                      float lhig, float *dout)
-{
-    std::vector<double> w;
-    const int nto = filter_design(nt, dt, flow, fhig, llow, lhig, w);
-    if (nto < 0) return -1;
-    const int half = nto / 2;
-    std::vector<std::complex<double>> a(nto);
-    for (int r = 0; r < nr; ++r) {
-        for (int j = 0; j < nto; ++j) a[j] = j < nt ? static_cast<double>(din[static_cast<size_t>(r) * nt + j]) : 0.0;
-        fft_inplace(a, -1);
-        // bin k < nto/2 scaled by dmartin(k+1); the Nyquist bin rides in the
-        // DC slot of the reference's packed spectrum and takes dmartin(1).
-        for (int k = 0; k <= half; ++k) {
-            const double g = (k == half) ? w[0] : w[k];
-            a[k] *= g;
-            if (k > 0 && k < half) a[nto - k] *= g;
-        }
-        fft_inplace(a, +1);
-        for (int j = 0; j < nt; ++j) dout[static_cast<size_t>(r) * nt + j] = static_cast<float>(a[j].real() / nto);
-    }
-    return 0;
 }
 
 }  // namespace daw

@@ -59,10 +59,4 @@ void sponge_b(int npml, int n, std::vector<float> &damp, std::vector<float> &dam
 // too large) and fills w[0..nto] with dmartin(1..nto+1).
 int filter_design(int nt, float dt, float flow, float fhig, float llow, float lhig, std::vector<double> &w);
 
-// Apply the band-pass to nr traces of length nt (trace-major, t fastest) on
-// the host with an FP64 real FFT.  Same transfer function as filmartin2d,
-// including its Nyquist bin being scaled by the DC weight.
-int filter_apply_host(const float *din, int nr, int nt, float dt, float flow, float fhig, float llow,
-                      float lhig, float *dout);
-
 }  // namespace daw

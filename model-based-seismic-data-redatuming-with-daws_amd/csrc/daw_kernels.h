@@ -26,6 +26,14 @@ void launch_resample(const float *wf, int n1, int n2, int nt, const int *ix1, co
 // nto <= 8192; wgt = dmartin(1..nto/2+1), tw[m] = exp(-2 pi i m / nto).
 int launch_filter(const float *din, float *dout, int ntr, int nt, int nto, const double *wgt, const double2 *tw,
                   hipStream_t st);
+// The same for FFT lengths nto = 16384, 32768, 65536: a four-step FFT in
+// three passes through work (pairs_per_group * nto complex values), trace
+// pairs pairs_per_group at a time; tw[m] = exp(-2 pi i m / nto), m < nto.
+// Returns -1 for other lengths.  din and dout may be the same traces.
+int launch_filter_long(const float *din, float *dout, int ntr, int nt, int nto, const double *wgt, const double2 *tw,
+                       double2 *work, int pairs_per_group, hipStream_t st);
+// nto = n1 * n2 of the long band-pass, or -1 when nto is not 16384..65536
+int filter_long_split(int nto, int &n1, int &n2);
 
 // MINRES over a batch: vectors of shot s start at off[s] and hold len[s]
 // floats; part holds minres_vec_blocks() FP64 partial sums per shot; scal
@@ -75,6 +83,25 @@ void launch_minres32_scalars(double *scal, double *hist, double *trace, const in
                              int nshots, hipStream_t st);
 void launch_minres32_x(float *x, float *wn, const float *w2, const float *w1, const float *v, const size_t *off,
                        const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st);
+
+// MINRES of the FP64 path with r1, r2, w, w1, w2 stored in 16 bits (fmt =
+// kSnapBF16 or kSnapF16): element i of shot s at q[off[s] + i], the scale of
+// its 256-element chunk at sc[coff[s] + i/256].  The scalar kernel
+// (launch_minres_scalars) is shared with the FP32-vector path.
+void launch_mk_init(int fmt, const float *b, unsigned short *qa, float *sa, unsigned short *qb, float *sb,
+                    const size_t *off, const size_t *coff, const int *len, int nshots, hipStream_t st);
+void launch_mk_zero(unsigned short *q0, float *s0, unsigned short *q1, float *s1, unsigned short *q2, float *s2,
+                    size_t nq, size_t nsc, hipStream_t st);
+void launch_mk_v(int fmt, const unsigned short *q, const float *sc, float *v, const size_t *off, const size_t *coff,
+                 const int *len, const double *scal, int nshots, hipStream_t st);
+void launch_mk_l1(int fmt, float *y, const float *v, const unsigned short *q1, const float *s1, const size_t *off,
+                  const size_t *coff, const int *len, const double *scal, double *part, int nshots, hipStream_t st);
+void launch_mk_l2(int fmt, float *y, const unsigned short *q2, const float *s2, unsigned short *qo, float *so,
+                  const size_t *off, const size_t *coff, const int *len, const double *scal, double *part, int nshots,
+                  hipStream_t st);
+void launch_mk_x(int fmt, float *x, unsigned short *qn, float *sn, const unsigned short *q2, const float *s2,
+                 const unsigned short *q1, const float *s1, const float *v, const size_t *off, const size_t *coff,
+                 const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st);
 
 // indices into a shot's scalar row (see k_minres_scalars)
 enum MinresScalIndex {

@@ -2134,6 +2134,224 @@ __global__ void k_minres32_x(float *x, float *w_new_over_w1, const float *w2_is_
     }
 }
 
+// ---------------------------------------------------------------------------
+// MINRES with its Krylov vectors r1, r2 and w, w1, w2 (minresModule.f90:29)
+// stored in 16 bits (BF16 or FP16): each 256-element chunk of a vector is
+// scaled by the power of two at or above its max |value| (an exact
+// rescaling, the snapshot codec's) and rounded to nearest even.  Arithmetic
+// is that of the FP64 path (FP64 dot products and scalars, FP32 results):
+// v, y and x stay FP32; a 16-bit vector is decoded where it is read and the
+// value that is encoded is the FP32 one just computed.  Chunk c of shot s has
+// its scale at sc[coff[s] + c].  The loops run over whole chunks so that
+// every thread of a workgroup reaches the chunk's barriers.
+// ---------------------------------------------------------------------------
+struct Vec16 {
+    unsigned short *q;
+    float *sc;
+};
+
+// max |v| over the workgroup (256 threads), the same value in every thread
+__device__ __forceinline__ float block_max_abs(float v)
+{
+    __shared__ float s_m[kBlock / 64];
+    const float w = wave_max_abs(v);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = w;
+    __syncthreads();
+    float m = s_m[0];
+    for (int i = 1; i < kBlock / 64; ++i) m = fmaxf(m, s_m[i]);
+    __syncthreads();
+    return m;
+}
+
+template <int F>
+__device__ __forceinline__ float k16_load(const Vec16 &V, size_t base, size_t cbase, size_t i)
+{
+    return snap_decode<F>(V.q[base + i]) * V.sc[cbase + (i >> 8)];
+}
+
+// encode chunk element i (all threads of the block call it for one chunk)
+template <int F>
+__device__ __forceinline__ void k16_store(const Vec16 &V, size_t base, size_t cbase, size_t i, bool ok, float v)
+{
+    const float sc = pow2_scale(block_max_abs(ok ? v : 0.0f));
+    if (ok) V.q[base + i] = snap_encode<F>(v * (1.0f / sc));
+    if (threadIdx.x == 0) V.sc[cbase + (i >> 8)] = sc;
+}
+
+// r1 = r2 = encode(b)
+template <int F>
+__global__ void k_mk_init(const float *b, Vec16 ra, Vec16 rb, const size_t *off, const size_t *coff, const int *len)
+{
+    const int s = blockIdx.y;
+    const size_t n = len[s];
+    for (size_t c = blockIdx.x; c * kBlock < n; c += kVecBlocks) {
+        const size_t i = c * kBlock + threadIdx.x;
+        const bool ok = i < n;
+        const float v = ok ? b[off[s] + i] : 0.0f;
+        k16_store<F>(ra, off[s], coff[s], i, ok, v);
+        k16_store<F>(rb, off[s], coff[s], i, ok, v);
+    }
+}
+
+// v = y / beta with y = r2 (:181-182)
+template <int F>
+__global__ void k_mk_v(Vec16 r2, float *v, const size_t *off, const size_t *coff, const int *len, const double *scal)
+{
+    const int s = blockIdx.y;
+    const double *q = scal + (size_t)s * kScal;
+    if (!active(q)) return;
+    const double sc = 1.0 / q[kBeta];
+    const size_t n = len[s];
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock)
+        v[off[s] + i] = (float)(sc * (double)k16_load<F>(r2, off[s], coff[s], i));
+}
+
+// y = Hv - shift*v - (beta/oldb)*r1; partial v.y (:185-190)
+template <int F>
+__global__ void k_mk_l1(float *y, const float *v, Vec16 r1, const size_t *off, const size_t *coff, const int *len,
+                        const double *scal, double *part)
+{
+    const int s = blockIdx.y;
+    const double *q = scal + (size_t)s * kScal;
+    double acc = 0.0;
+    if (active(q)) {
+        const double shift = q[kShift];
+        const double c = (q[kItn] >= 1.0) ? q[kBeta] / q[kOldb] : 0.0;
+        const size_t n = len[s];
+        for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)kVecBlocks * kBlock) {
+            const double vi = v[off[s] + i];
+            const float ri = c != 0.0 ? k16_load<F>(r1, off[s], coff[s], i) : 0.0f;
+            const float yi = (float)((double)y[off[s] + i] - shift * vi - c * (double)ri);
+            y[off[s] + i] = yi;
+            acc += vi * (double)yi;
+        }
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) part[(size_t)s * kVecBlocks + blockIdx.x] = acc;
+}
+
+// y = y - (alfa/beta)*r2; partial y.y; the new r2 = encode(y) goes over r1 (:191-197)
+template <int F>
+__global__ void k_mk_l2(float *y, Vec16 r2, Vec16 rout, const size_t *off, const size_t *coff, const int *len,
+                        const double *scal, double *part)
+{
+    const int s = blockIdx.y;
+    const double *q = scal + (size_t)s * kScal;
+    const bool act = active(q);
+    double acc = 0.0;
+    if (act) {
+        const double c = q[kAlfa] / q[kBeta];
+        const size_t n = len[s];
+        for (size_t ch = blockIdx.x; ch * kBlock < n; ch += kVecBlocks) {
+            const size_t i = ch * kBlock + threadIdx.x;
+            const bool ok = i < n;
+            float yi = 0.0f;
+            if (ok) {
+                yi = (float)((double)y[off[s] + i] - c * (double)k16_load<F>(r2, off[s], coff[s], i));
+                y[off[s] + i] = yi;
+                acc += (double)yi * (double)yi;
+            }
+            k16_store<F>(rout, off[s], coff[s], i, ok, yi);
+        }
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) part[(size_t)s * kVecBlocks + blockIdx.x] = acc;
+}
+
+// w_new = (v - oldeps*w1 - delta*w2)*denom stored over w1's slot, x += phi*w_new (:252-257)
+template <int F>
+__global__ void k_mk_x(float *x, Vec16 wn, Vec16 w2v, Vec16 w1v, const float *v, const size_t *off, const size_t *coff,
+                       const int *len, double *scal, double *trace, int itnlim)
+{
+    const int s = blockIdx.y;
+    double *q = scal + (size_t)s * kScal;
+    if (!(q[kIstop] == 0.0 && q[kItn] >= 1.0)) return;
+    const double oldeps = q[kOldeps], delta = q[kDelta], denom = q[kDenom], phi = q[kPhi];
+    const size_t n = len[s];
+    for (size_t ch = blockIdx.x; ch * kBlock < n; ch += kVecBlocks) {
+        const size_t i = ch * kBlock + threadIdx.x;
+        const bool ok = i < n;
+        float wi = 0.0f;
+        if (ok) {
+            const double w1 = k16_load<F>(w1v, off[s], coff[s], i), w2 = k16_load<F>(w2v, off[s], coff[s], i);
+            wi = (float)(((double)v[off[s] + i] - oldeps * w1 - delta * w2) * denom);
+            const float xi = (float)((double)x[off[s] + i] + phi * (double)wi);
+            x[off[s] + i] = xi;
+            if (trace && i == 0) trace[((size_t)s * itnlim + ((int)q[kItn] - 1)) * kTraceStride] = (double)xi;
+        }
+        k16_store<F>(wn, off[s], coff[s], i, ok, wi);
+    }
+}
+
+// w = w1 = w2 = 0
+__global__ void k_mk_zero(Vec16 a, Vec16 b, Vec16 c, size_t nq, size_t nsc)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (size_t)gridDim.x * blockDim.x) {
+        a.q[i] = 0; b.q[i] = 0; c.q[i] = 0;
+        if (i < nsc) { a.sc[i] = 1.0f; b.sc[i] = 1.0f; c.sc[i] = 1.0f; }
+    }
+}
+
+#define K16_DISPATCH(F, KERNEL, ...)                                                                  \
+    do {                                                                                             \
+        if ((F) == kSnapBF16) hipLaunchKernelGGL(KERNEL<kSnapBF16>, __VA_ARGS__);                    \
+        else hipLaunchKernelGGL(KERNEL<kSnapF16>, __VA_ARGS__);                                      \
+    } while (0)
+
+static Vec16 v16(unsigned short *q, float *sc) { return Vec16{q, sc}; }
+
+void launch_mk_init(int fmt, const float *b, unsigned short *qa, float *sa, unsigned short *qb, float *sb,
+                    const size_t *off, const size_t *coff, const int *len, int nshots, hipStream_t st)
+{
+    K16_DISPATCH(fmt, k_mk_init, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, b, v16(qa, sa), v16(qb, sb), off, coff,
+                 len);
+}
+
+void launch_mk_zero(unsigned short *q0, float *s0, unsigned short *q1, float *s1, unsigned short *q2, float *s2,
+                    size_t nq, size_t nsc, hipStream_t st)
+{
+    const size_t n = std::max(nq, nsc);
+    const unsigned blocks = (unsigned)std::min<size_t>(65536, (n + 255) / 256);
+    hipLaunchKernelGGL(k_mk_zero, dim3(std::max(1u, blocks)), dim3(256), 0, st, v16(q0, s0), v16(q1, s1), v16(q2, s2),
+                       nq, nsc);
+}
+
+void launch_mk_v(int fmt, const unsigned short *q, const float *sc, float *v, const size_t *off, const size_t *coff,
+                 const int *len, const double *scal, int nshots, hipStream_t st)
+{
+    K16_DISPATCH(fmt, k_mk_v, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st,
+                 v16(const_cast<unsigned short *>(q), const_cast<float *>(sc)), v, off, coff, len, scal);
+}
+
+void launch_mk_l1(int fmt, float *y, const float *v, const unsigned short *q1, const float *s1, const size_t *off,
+                  const size_t *coff, const int *len, const double *scal, double *part, int nshots, hipStream_t st)
+{
+    K16_DISPATCH(fmt, k_mk_l1, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, y, v,
+                 v16(const_cast<unsigned short *>(q1), const_cast<float *>(s1)), off, coff, len, scal, part);
+    hipLaunchKernelGGL(k_minres_alfa, dim3((nshots + 63) / 64), dim3(64), 0, st, part, const_cast<double *>(scal),
+                       nshots);
+}
+
+void launch_mk_l2(int fmt, float *y, const unsigned short *q2, const float *s2, unsigned short *qo, float *so,
+                  const size_t *off, const size_t *coff, const int *len, const double *scal, double *part, int nshots,
+                  hipStream_t st)
+{
+    K16_DISPATCH(fmt, k_mk_l2, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, y,
+                 v16(const_cast<unsigned short *>(q2), const_cast<float *>(s2)), v16(qo, so), off, coff, len, scal,
+                 part);
+}
+
+void launch_mk_x(int fmt, float *x, unsigned short *qn, float *sn, const unsigned short *q2, const float *s2,
+                 const unsigned short *q1, const float *s1, const float *v, const size_t *off, const size_t *coff,
+                 const int *len, double *scal, double *trace, int itnlim, int nshots, hipStream_t st)
+{
+    K16_DISPATCH(fmt, k_mk_x, dim3(kVecBlocks, nshots), dim3(kBlock), 0, st, x, v16(qn, sn),
+                 v16(const_cast<unsigned short *>(q2), const_cast<float *>(s2)),
+                 v16(const_cast<unsigned short *>(q1), const_cast<float *>(s1)), v, off, coff, len, scal, trace,
+                 itnlim);
+    hipLaunchKernelGGL(k_minres_close, dim3((nshots + 63) / 64), dim3(64), 0, st, scal, nshots);
+}
+
 void launch_seqdot32(const float *a, const float *b, const size_t *off, const int *len, const int *nts, double *scal,
                      int which, int order, bool skip_inactive, int nshots, hipStream_t st)
 {
@@ -2253,7 +2471,9 @@ __device__ void lds_fft(double2 *a, const double2 *__restrict__ tw, int nto, int
     }
 }
 
-__global__ __launch_bounds__(256) void k_filter(const float *__restrict__ din, float *__restrict__ dout, int ntr,
+// din and dout may be the same traces (in-place filtering): every load of a
+// pair precedes the barriers that precede its stores, so no __restrict__.
+__global__ __launch_bounds__(256) void k_filter(const float *din, float *dout, int ntr,
                                                 int nt, int nto, int bits, const double *__restrict__ wgt,
                                                 const double2 *__restrict__ tw)
 {
@@ -2297,6 +2517,161 @@ int launch_filter(const float *din, float *dout, int ntr, int nt, int nto, const
     const size_t lds = (size_t)nto * sizeof(double2);
     (void)hipFuncSetAttribute((const void *)k_filter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_filter, dim3((ntr + 1) / 2), dim3(256), lds, st, din, dout, ntr, nt, nto, bits, wgt, tw);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Band-pass for FFT lengths 16384..65536 (the rest of filmartin2d's length
+// list, minresModule.f90:501-510): a trace pair's transform no longer fits
+// LDS, so it runs as a four-step FFT, N = N1 * N2 (N1, N2 <= 256), in three
+// passes through an HBM work area Y[pair][k1][n2] (complex FP64):
+//   pass 1 (columns): x[N2*n1 + n2] -> FFT over n1 -> times W_N^(n2*k1) -> Y[k1][n2]
+//   pass 2 (rows):    FFT over n2 gives X[k1 + N1*k2]; times the filter;
+//                     inverse FFT over k2 -> Y[k1][n2]
+//   pass 3 (columns): times W_N^(-n2*k1) -> inverse FFT over k1 -> / N -> traces
+// Each workgroup stages kFftChunk complex values (64 KB) of one pair in LDS:
+// kFftChunk/N1 columns or kFftChunk/N2 rows.  tw[m] = exp(-2 pi i m / N),
+// m < N (the butterflies of a length-L stage use tw[k * N/L]).
+// ---------------------------------------------------------------------------
+constexpr int kFftChunk = 4096;
+
+// nb independent radix-2 FFTs of length L = 2^bits stored one after another
+// in a, each in bit-reversed order on entry and natural order on exit.
+__device__ void lds_fft_batch(double2 *a, int L, int nb, const double2 *tw, int tstride, bool inverse)
+{
+    const int hl = L >> 1;
+    for (int len = 2; len <= L; len <<= 1) {
+        const int half = len >> 1, stride = (L / len) * tstride;
+        for (int j = threadIdx.x; j < nb * hl; j += blockDim.x) {
+            const int b = j / hl, jj = j - b * hl;
+            const int grp = jj / half, k = jj - grp * half;
+            const int i0 = b * L + grp * len + k, i1 = i0 + half;
+            double2 w = tw[k * stride];
+            if (inverse) w.y = -w.y;
+            const double2 u = a[i0], v = a[i1];
+            const double2 vw = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+            a[i0] = make_double2(u.x + vw.x, u.y + vw.y);
+            a[i1] = make_double2(u.x - vw.x, u.y - vw.y);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b, bool conj_b)
+{
+    const double by = conj_b ? -b.y : b.y;
+    return make_double2(a.x * b.x - a.y * by, a.x * by + a.y * b.x);
+}
+
+// Pass 1 (INV = false) and pass 3 (INV = true).  grid (N2 / C, pairs in the group).
+template <bool INV>
+__global__ __launch_bounds__(256) void k_bp_cols(const float *din, float *dout, double2 *Y, int ntr, int nt, int N1,
+                                                 int bits1, int N2, int pair0, const double2 *tw)
+{
+    __shared__ double2 a[kFftChunk];
+    const int C = kFftChunk / N1;
+    const int p = blockIdx.y, pair = pair0 + p;
+    const int t1 = 2 * pair, t2 = t1 + 1;
+    const bool has2 = t2 < ntr;
+    const int N = N1 * N2, c0 = blockIdx.x * C;
+    double2 *Yp = Y + (size_t)p * N;
+    for (int j = threadIdx.x; j < kFftChunk; j += blockDim.x) {
+        const int c = j % C, i = j / C, n2 = c0 + c;          // i: n1 (pass 1) or k1 (pass 3)
+        double2 v;
+        if (!INV) {
+            const int n = N2 * i + n2;
+            v = make_double2(n < nt ? (double)din[(size_t)t1 * nt + n] : 0.0,
+                             (n < nt && has2) ? (double)din[(size_t)t2 * nt + n] : 0.0);
+        } else {
+            v = cmul(Yp[(size_t)i * N2 + n2], tw[(n2 * i) & (N - 1)], true);
+        }
+        a[c * N1 + bitrev(i, bits1)] = v;
+    }
+    __syncthreads();
+    lds_fft_batch(a, N1, C, tw, N / N1, INV);
+    const double sc = 1.0 / (double)N;
+    for (int j = threadIdx.x; j < kFftChunk; j += blockDim.x) {
+        const int c = j % C, i = j / C, n2 = c0 + c;          // i: k1 (pass 1) or n1 (pass 3)
+        const double2 v = a[c * N1 + i];
+        if (!INV) {
+            Yp[(size_t)i * N2 + n2] = cmul(v, tw[(n2 * i) & (N - 1)], false);
+        } else {
+            const int n = N2 * i + n2;
+            if (n < nt) {
+                dout[(size_t)t1 * nt + n] = (float)(v.x * sc);
+                if (has2) dout[(size_t)t2 * nt + n] = (float)(v.y * sc);
+            }
+        }
+    }
+}
+
+// Pass 2: rows k1 of the group's pairs.  grid (N1 / R, pairs in the group).
+__global__ __launch_bounds__(256) void k_bp_rows(double2 *Y, int N1, int N2, int bits2, const double *wgt,
+                                                 const double2 *tw)
+{
+    __shared__ double2 a[kFftChunk];
+    const int R = kFftChunk / N2;
+    const int p = blockIdx.y, r0 = blockIdx.x * R;
+    const int N = N1 * N2;
+    double2 *Yp = Y + (size_t)p * N;
+    for (int j = threadIdx.x; j < kFftChunk; j += blockDim.x) {
+        const int r = j / N2, n2 = j - r * N2;
+        a[r * N2 + bitrev(n2, bits2)] = Yp[(size_t)(r0 + r) * N2 + n2];
+    }
+    __syncthreads();
+    lds_fft_batch(a, N2, R, tw, N / N2, false);
+    for (int j = threadIdx.x; j < kFftChunk; j += blockDim.x) {
+        const int r = j / N2, k2 = j - r * N2;
+        const int k = (r0 + r) + N1 * k2;                     // frequency index of X
+        const int kk = k <= N / 2 ? k : N - k;
+        // bin kk < N/2 takes dmartin(kk+1); the Nyquist bin takes dmartin(1)
+        const double gain = kk == N / 2 ? wgt[0] : wgt[kk];
+        a[j] = make_double2(a[j].x * gain, a[j].y * gain);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kFftChunk; j += blockDim.x) {
+        const int r = j / N2, i = j - r * N2, rv = bitrev(i, bits2);
+        if (i < rv) {
+            const double2 tmp = a[r * N2 + i];
+            a[r * N2 + i] = a[r * N2 + rv];
+            a[r * N2 + rv] = tmp;
+        }
+    }
+    __syncthreads();
+    lds_fft_batch(a, N2, R, tw, N / N2, true);
+    for (int j = threadIdx.x; j < kFftChunk; j += blockDim.x) {
+        const int r = j / N2, n2 = j - r * N2;
+        Yp[(size_t)(r0 + r) * N2 + n2] = a[j];
+    }
+}
+
+int filter_long_split(int nto, int &n1, int &n2)
+{
+    int bits = 0;
+    while ((1 << bits) < nto) ++bits;
+    if ((1 << bits) != nto || bits < 14 || bits > 16) return -1;
+    n1 = 1 << ((bits + 1) / 2);      // 16384 = 128 x 128, 32768 = 256 x 128, 65536 = 256 x 256
+    n2 = nto / n1;
+    return 0;
+}
+
+int launch_filter_long(const float *din, float *dout, int ntr, int nt, int nto, const double *wgt, const double2 *tw,
+                       double2 *work, int pairs_per_group, hipStream_t st)
+{
+    int n1 = 0, n2 = 0;
+    if (filter_long_split(nto, n1, n2) != 0 || nt > nto || pairs_per_group < 1) return -1;
+    int b1 = 0, b2 = 0;
+    while ((1 << b1) < n1) ++b1;
+    while ((1 << b2) < n2) ++b2;
+    const int npairs = (ntr + 1) / 2;
+    for (int p0 = 0; p0 < npairs; p0 += pairs_per_group) {
+        const int np = std::min(pairs_per_group, npairs - p0);
+        hipLaunchKernelGGL(k_bp_cols<false>, dim3(n2 / (kFftChunk / n1), np), dim3(256), 0, st, din, dout, work, ntr,
+                           nt, n1, b1, n2, p0, tw);
+        hipLaunchKernelGGL(k_bp_rows, dim3(n1 / (kFftChunk / n2), np), dim3(256), 0, st, work, n1, n2, b2, wgt, tw);
+        hipLaunchKernelGGL(k_bp_cols<true>, dim3(n2 / (kFftChunk / n1), np), dim3(256), 0, st, din, dout, work, ntr,
+                           nt, n1, b1, n2, p0, tw);
+    }
     return 0;
 }
 

@@ -72,12 +72,16 @@ def test_redatum_batch_pipeline(gpu_daw):
             assert np.allclose(hist[s], h, rtol=1e-4)
 
 
-@pytest.mark.parametrize("nt,dt", [(300, 1e-3), (3000, 8e-4), (6000, 5e-4), (9000, 4e-4)])
-def test_filter_device(gpu_daw, nt, dt):
-    """filmartin2d on device traces: FFT lengths up to 8192 on the GPU, longer on the host."""
+@pytest.mark.parametrize("nt,dt,ntr", [(300, 1e-3, 7), (3000, 8e-4, 7), (6000, 5e-4, 7), (9000, 4e-4, 7),
+                                        (12000, 2.5e-4, 9), (20000, 2.5e-4, 5), (40000, 1e-4, 3), (65536, 1e-4, 2)])
+def test_filter_device(gpu_daw, nt, dt, ntr):
+    """filmartin2d on device traces against the oracle (the reference's NR
+    FFT restated), within 1e-6: FFT lengths up to 8192 in LDS, 16384-65536
+    as the four-step transform through HBM (odd trace counts leave the last
+    pair with one trace)."""
     import torch
     rng = np.random.default_rng(nt)
-    d = rng.standard_normal((7, nt)).astype(np.float32)
+    d = rng.standard_normal((ntr, nt)).astype(np.float32)
     band = (2.0, 25.0, 1.0, 5.0)
     x = torch.from_numpy(d).cuda()
     y = torch.empty_like(x)
@@ -86,3 +90,24 @@ def test_filter_device(gpu_daw, nt, dt):
     assert rel_l2(y.cpu().numpy(), ref) < 1e-6
     gpu_daw.filmartin2d_dev(x, x, dt, *band)          # in place
     assert rel_l2(x.cpu().numpy(), ref) < 1e-6
+
+
+def test_filmartin2d_dropin_golden(gpu_daw):
+    """The host-array drop-in against the reference's own filmartin2d output
+    (golden fixture written by the reference build), within 1e-6."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_small.npz"), allow_pickle=False)
+    d = gold["filt_in"]
+    assert rel_l2(gpu_daw.filmartin2d(d, 1e-3, 3.0, 30.0, 2.0, 10.0), gold["filt_out"]) < 1e-6
+    assert rel_l2(gpu_daw.filmartin2d(d, 8e-4, 2.0, 700.0, 1.0, 5.0), gold["filt_out2"]) < 1e-6
+
+
+@pytest.mark.parametrize("nt", [2500, 40000])
+def test_filmartin2d_dropin(gpu_daw, nt):
+    """The host-array drop-in filmartin2d runs on the GPU too, within 1e-6 of the oracle."""
+    rng = np.random.default_rng(nt + 1)
+    d = rng.standard_normal((5, nt)).astype(np.float32)
+    band = (3.0, 30.0, 2.0, 10.0)
+    got = gpu_daw.filmartin2d(d, 1e-3 if nt < 10000 else 2e-4, *band)
+    ref = O.filmartin2d(d, 1e-3 if nt < 10000 else 2e-4, *band)
+    assert rel_l2(got, ref) < 1e-6

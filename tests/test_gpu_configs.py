@@ -14,9 +14,9 @@ with one receiver per column and nt 4000-6000), sampled shots of its
 Configuration 4 (the 201 x 801, nt = 3000 workload with 16-bit snapshots):
   - the Hessian-type operator with BF16 and FP16 snapshots bit for bit
     against the oracle's operator with the same codec, on one shot's residual;
-  - one shot's redatuming (30 MINRES iterations) with BF16 and FP16 snapshots
-    against the FP64 oracle of the plain method (FP16 within 1e-4; BF16,
-    8 significant bits, within 2e-4; both errors printed).
+  - one shot's redatuming (30 MINRES iterations) with BF16 and FP16 snapshots,
+    and with 16-bit MINRES Krylov vectors as well, against the FP64 oracle of
+    the plain method (bounds at COMBOS; every error printed).
 Configuration 5 (801 x 3201, h = 3.125 m, nt = 12000, BF16 snapshots):
   - the admission of shots by the HBM budget: the largest shots of the line
     at full size run with the default budget and with a budget that admits
@@ -202,6 +202,7 @@ def _c2_shot():
 def snap16(gpu_daw):
     yield gpu_daw
     gpu_daw.set_snapshot_format("fp32")
+    gpu_daw.set_krylov_format("fp32")
 
 
 @pytest.mark.timeout(600)
@@ -217,6 +218,15 @@ def test_c4_hessian_16bit_bitexact(snap16):
         assert np.array_equal(a, b), (snap, rel_l2(a, b))
 
 
+# (snapshots, Krylov vectors) -> bound on the errors against the FP64 oracle.
+# FP16 snapshots with FP32 Krylov vectors meet the 1e-4 bar (measured 2.3e-5);
+# BF16 snapshots (8 significant bits) give 1.8e-4.  16-bit Krylov vectors
+# perturb the Lanczos vectors themselves, and the error grows with the
+# iterations: measured 1.5e-3 (gathers) and 1.0e-3 (history) for FP16 at 30
+# iterations; BF16 more.  They stay an option, not the default.
+COMBOS = {("fp16", "fp32"): 1e-4, ("bf16", "fp32"): 2.5e-4, ("fp16", "fp16"): 3e-3, ("bf16", "bf16"): 5e-2}
+
+
 @pytest.mark.timeout(900)
 def test_c4_pipeline_16bit_vs_f64(snap16):
     import torch
@@ -224,13 +234,15 @@ def test_c4_pipeline_16bit_vs_f64(snap16):
     dobs = O.forward(vp, w.npml, h, w.ofs, w.nt, dt, s, src[:, 0].copy(), rec)
     dobs = _syn().add_noise(dobs, 2002)
     got = {}
-    for snap in ("bf16", "fp16"):
-        print("C4 redatum_batch, one shot,", snap, "snapshots, 30 MINRES iterations", flush=True)
+    for snap, kry in COMBOS:
+        print("C4 redatum_batch, one shot,", snap, "snapshots,", kry, "Krylov vectors, 30 MINRES iterations",
+              flush=True)
         snap16.set_snapshot_format(snap)
+        snap16.set_krylov_format(kry)
         datum = torch.zeros((1, dat.shape[1], w.nt), dtype=torch.float32, device="cuda")
         hist = snap16.redatum_batch(c0, w.npml, h, w.ofs, w.nt, dt, torch.from_numpy(s).cuda(), src, rec, dat,
                                     torch.from_numpy(dobs[None]).cuda(), w.mu, w.maxiter, datum_dev=datum)
-        got[snap] = (datum.cpu().numpy()[0], hist[0])
+        got[(snap, kry)] = (datum.cpu().numpy()[0], hist[0])
     print("C4 FP64 oracle pipeline", flush=True)
     dr, wfr = O.forward(c0, w.npml, h, w.ofs, w.nt, dt, s, src[:, 0].copy(), rec, want_wavefield=True, prec="f64")
     res = np.ascontiguousarray((dobs.astype(np.float64) - dr)[:, ::-1])
@@ -239,11 +251,12 @@ def test_c4_pipeline_16bit_vs_f64(snap16):
     wfr += wfe
     del wfe
     ref = O.resample(wfr, w.npml, dat, w.ofs, prec="f64")
-    for snap, tol in (("fp16", 1e-4), ("bf16", 2e-4)):
-        d, hist = got[snap]
+    for combo in COMBOS:
+        d, hist = got[combo]
         e = (rel_l2(d, ref), rel_l2(hist, hh), float(np.max(np.abs(hist / hh - 1))))
-        print(f"C4 {snap}: datum rel L2, history rel L2, max rel history vs FP64", e, flush=True)
-        assert max(e) <= tol, (snap, e)
+        print(f"C4 {combo[0]} snapshots, {combo[1]} Krylov vectors: datum rel L2, history rel L2, "
+              f"max rel history vs FP64", e, flush=True)
+        assert max(e) <= COMBOS[combo], (combo, e)
 
 
 @pytest.mark.timeout(600)

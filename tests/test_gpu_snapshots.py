@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 def snap16(gpu_daw):
     yield gpu_daw
     gpu_daw.set_snapshot_format("fp32")
+    gpu_daw.set_krylov_format("fp32")
 
 
 @pytest.mark.parametrize("snap", ["bf16", "fp16"])
@@ -77,3 +78,40 @@ def test_redatum_batch_16bit_vs_f64(snap16, snap, tol):
         e = (rel_l2(dde[s], x), rel_l2(datum[s], ref_datum), float(np.max(np.abs(hist[s] / h - 1))))
         print(s, snap, "deltade / datum rel L2, max rel hist", e, flush=True)
         assert max(e) <= tol, (s, snap, e)
+
+
+@pytest.mark.parametrize("kry,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
+def test_redatum_batch_16bit_krylov_vs_f64(snap16, kry, tol):
+    """MINRES with r1, r2, w, w1, w2 stored in 16 bits (FP64 dot products and
+    scalars): gathers and history against the FP64 oracle; the two shots of a
+    batch equal the same shots redatumed one at a time (the chunk scales are
+    per shot)."""
+    import torch
+    c = Case(nt=250)
+    ns, its, mu = 2, 5, 1e-2
+    xmax = (c.n2 - 1) * float(c.h)
+    src = O.acqui_table(np.full(ns, 10.0), np.linspace(0.3, 0.7, ns) * xmax, c.h, c.npml, 1)
+    nd = c.dat.shape[1]
+    dobs_np = np.stack([O.forward(c.vp, c.npml, c.h, 1, c.nt, c.dt, c.s, src[:, s].copy(), c.rec) for s in range(ns)])
+    s_dev = torch.from_numpy(c.s).cuda()
+    dobs = torch.from_numpy(dobs_np).cuda()
+    snap16.set_krylov_format(kry)
+    datum = torch.zeros((ns, nd, c.nt), dtype=torch.float32, device="cuda")
+    print("redatum_batch, Krylov vectors", kry, flush=True)
+    hist = snap16.redatum_batch(c.vp0, c.npml, c.h, 1, c.nt, c.dt, s_dev, src, c.rec, c.dat, dobs, mu, its,
+                                datum_dev=datum)
+    one = torch.zeros((ns, nd, c.nt), dtype=torch.float32, device="cuda")
+    hist1 = snap16.redatum_batch(c.vp0, c.npml, c.h, 1, c.nt, c.dt, s_dev, src, c.rec, c.dat, dobs, mu, its,
+                                 datum_dev=one, max_batch=1)
+    assert torch.equal(datum, one) and np.array_equal(hist, hist1)
+    datum = datum.cpu().numpy()
+    for s in range(ns):
+        dr, wfr = O.forward(c.vp0, c.npml, c.h, 1, c.nt, c.dt, c.s, src[:, s].copy(), c.rec, want_wavefield=True,
+                            prec="f64")
+        res = np.ascontiguousarray((dobs_np[s].astype(np.float64) - dr)[:, ::-1])
+        x, h, _ = O.linearminres(c.vp0, c.npml, c.h, 1, c.nt, c.dt, res, c.rec, mu, its, prec="f64")
+        _, wfe = O.esource(c.vp0, c.npml, c.h, 1, c.nt, c.dt, x, c.rec, want_wavefield=True, prec="f64")
+        ref = O.resample(wfr + wfe, c.npml, c.dat, 1, prec="f64")
+        e = (rel_l2(datum[s], ref), float(np.max(np.abs(hist[s] / h - 1))))
+        print(s, kry, "datum rel L2, max rel history vs FP64", e, flush=True)
+        assert max(e) <= tol, (s, kry, e)
