@@ -7,14 +7,14 @@ reference's redatuming workflow (sub_redatum_DAW.f90 modes 1 and 2) for all
 64 shots.  Observed data are modelled beforehand in the true model with 1 %
 seeded noise (untimed input synthesis, the reference's mode 0).
 
-One step = one redatuming pass over one batch of 16 shots advanced together
-(the launch batch: the 64-shot line is four such batches, since the 64 FP32
-snapshot stores need 371 GB and a 16-shot launch keeps its wave state in the
-last-level cache): background forward, residual, fac, 30 MINRES iterations
+One step = one redatuming pass over one batch of up to 24 shots advanced
+together (the launch batch the library admits at this size: 3 shots per XCD,
+wave state within the last-level cache; the 64 FP32 snapshot stores would
+need 371 GB): the 64-shot line is batches of 24, 24 and 16 shots, which the
+steps walk in turn.  Per batch: background forward, residual, fac, 30 MINRES iterations
 (each one back-propagation + re-injection sweep pair), the source-extension
 sweep pair, the datum gathers and their band-pass (filmartin2d, as mode 2
-writes dataobs_redatum), with inputs resident in HBM.  Steps walk the
-batches of the line in turn.  The line also reports the plain
+writes dataobs_redatum), with inputs resident in HBM.  The line also reports the plain
 one-shot-at-a-time GPU path, the roofline of the dominant step kernel (HIP
 events on the library's stream) and the reference's own CPU path timed on
 this host's cores (one shot per core, truncated nt, scaled).
@@ -543,7 +543,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4, help="timed steps (batches of --batch shots)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16, help="shots per step (advanced together)")
+    ap.add_argument("--batch", type=int, default=24, help="shots per step (advanced together)")
     ap.add_argument("--time-budget", type=float, default=540.0,
                     help="seconds after which the plain-path and CPU legs are left out")
     ap.add_argument("--cpu-timeout", type=float, default=180.0)
@@ -619,33 +619,43 @@ def main():
                                  P["dat"], b["dobs"], w.mu, w.maxiter, 0.0, datum_dev=b["datum"], max_batch=bsz,
                                  hbm_budget_bytes=budget, band=w.band)
 
-    # first-use warm-up of the library (kernel loading, stream creation): one short shot
-    Q = setup(daw, syn, args.config, shots=1, nt=64, maxiter=1)
-    run_redatum(daw, Q)
+    # first-use warm-up of the library (kernel loading, stream creation) and
+    # of the launch autotuner for every batch size of the line: short runs
+    # (nt = 64, one MINRES iteration) of each size; the tuner's choice
+    # depends on the batch's shots and grid, not on nt
+    for ns in sorted({b["shots"][1] - b["shots"][0] for b in batches}):
+        Q = setup(daw, syn, args.config, shots=ns, nt=64, maxiter=1)
+        run_redatum(daw, Q, max_batch=ns)
     for i in range(args.warmup):
         _log(f"warmup step {i + 1}/{args.warmup}")
         step(i)
-    daw.reset_stats()
     daw.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hists = {}
+    per_step = []        # (shots, sweep stats, launch configuration) of every timed step
     for i in range(args.steps):
-        _log(f"timed step {i + 1}/{args.steps}: shots {batches[i % nbat]['shots'][0] + 1}.."
-             f"{batches[i % nbat]['shots'][1]}")
+        b = batches[i % nbat]
+        _log(f"timed step {i + 1}/{args.steps}: shots {b['shots'][0] + 1}..{b['shots'][1]}")
+        daw.reset_stats()
         hists[i % nbat] = step(i)
+        per_step.append((b["shots"][1] - b["shots"][0], daw.stats(), [daw.get_launch(m) for m in range(3)]))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     daw.set_timing(False)
-    st = daw.stats()
-    launch = {m: daw.get_launch(i) for i, m in enumerate(("forward", "back-propagation", "re-injection"))}
-    shots_timed = sum(batches[i % nbat]["shots"][1] - batches[i % nbat]["shots"][0] for i in range(args.steps))
+    shots_timed = sum(p[0] for p in per_step)
     shots_per_hour = shots_timed / elapsed * 3600.0
-    gpts = st["point_updates"] / elapsed / 1e9
+    gpts = sum(p[1]["point_updates"] for p in per_step) / elapsed / 1e9
     n1e, n2e = w.nz + 2 * w.npml, w.nx + 2 * w.npml
+    # roofline of the full batches (bsz shots per launch): their sweep times,
+    # from HIP events around the sweeps on the library's stream
+    full = [p for p in per_step if p[0] == bsz] or per_step
+    shots_per_launch = full[0][0]
+    st = {"sweep_ms": [sum(p[1]["sweep_ms"][m] for p in full) for m in range(3)],
+          "sweep_steps": [sum(p[1]["sweep_steps"][m] for p in full) for m in range(3)]}
+    launch = {m: tuple(full[-1][2][i]) for i, m in enumerate(("forward", "back-propagation", "re-injection"))}
     # dominant kernel: back-propagation or re-injection sweep; one launch
-    # advances all shots of a batch by launch_steps time steps; its average
-    # duration comes from HIP events around the sweeps on the library's stream
+    # advances all shots of a batch by launch_steps time steps
     dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
     launch_steps = launch[("forward", "back-propagation", "re-injection")[dom]][0]
     kname = "k_step2" if launch_steps == 2 else "k_step"
@@ -657,7 +667,6 @@ def main():
     # one 4 B scale per 64 columns)
     snap_b = 12.0 if args.snapshots == "fp32" else 6.0 + 12.0 / 64.0
     bytes_per_shot_launch = 36.0 * n1e * n2e + launch_steps * snap_b * w.nz * w.nx
-    shots_per_launch = st["max_batch_used"]
     alg_bytes = bytes_per_shot_launch * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
@@ -746,6 +755,7 @@ def main():
         "gpts_per_s": gpts,
         "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
                               for i, k in enumerate(("forward", "back-propagation", "re-injection"))},
+        "sweep_us_per_step_shots": shots_per_launch,
         "plain_shots_per_hour": plain,
         "batched_over_plain": (shots_per_hour / plain) if plain else None,
         "batched_vs_plain_max_rel_diff": diff,

@@ -212,7 +212,8 @@ typedef struct daw_redatum_args {
     float *deltade_dev;       /* device out or NULL: MINRES solutions [shot][nrec][nt] */
     double *hist_host;        /* host out or NULL: qrnorm/beta1 [shot][maxiter] */
     int max_batch;            /* shots advanced together; <= 0: admit by the HBM budget, at most
-                                 as many as keep a launch's wave state within ~128 MB */
+                                 as many as keep a launch's wave state within ~192 MB, in
+                                 multiples of 8 (the XCD count) where that fits */
     double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
     int filter;               /* 1: band-pass the gathers as mode 2 does (filmartin2d) */
     float flow, fhig, llow, lhig;
@@ -255,7 +256,7 @@ typedef struct daw_uneven_args {
     float *datae_dev;         /* device out or NULL: as dobs_dev */
     float *deltade_dev;       /* device out or NULL: as dobs_dev */
     double *hist_host;        /* host out or NULL: [shot][maxiter] */
-    int max_batch;            /* shots per sub-batch; <= 0: by the HBM budget and the ~128 MB state cap */
+    int max_batch;            /* shots per sub-batch; <= 0: by the HBM budget and the ~192 MB state cap */
     double hbm_budget_bytes;  /* <= 0: 85 % of free HBM */
     int filter;
     float flow, fhig, llow, lhig;
