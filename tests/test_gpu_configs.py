@@ -223,8 +223,8 @@ def test_c4_hessian_16bit_bitexact(snap16):
 # BF16 snapshots (8 significant bits) give 1.8e-4.  16-bit Krylov vectors
 # perturb the Lanczos vectors themselves, and the error grows with the
 # iterations: measured 1.5e-3 (gathers) and 1.0e-3 (history) for FP16 at 30
-# iterations; BF16 more.  They stay an option, not the default.
-COMBOS = {("fp16", "fp32"): 1e-4, ("bf16", "fp32"): 2.5e-4, ("fp16", "fp16"): 3e-3, ("bf16", "bf16"): 5e-2}
+# iterations; 1.2e-2 and 7.0e-2 for BF16.  They stay an option, not the default.
+COMBOS = {("fp16", "fp32"): 1e-4, ("bf16", "fp32"): 2.5e-4, ("fp16", "fp16"): 3e-3, ("bf16", "bf16"): 0.1}
 
 
 @pytest.mark.timeout(900)
