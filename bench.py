@@ -621,10 +621,10 @@ def main():
 
     # first-use warm-up of the library (kernel loading, stream creation) and
     # of the launch autotuner for every batch size of the line: short runs
-    # (nt = 64, one MINRES iteration) of each size; the tuner's choice
-    # depends on the batch's shots and grid, not on nt
+    # (nt = 256, one MINRES iteration) of each size; the tuner's choice
+    # depends on the batch's shots and grid, and its probes are 192 steps
     for ns in sorted({b["shots"][1] - b["shots"][0] for b in batches}):
-        Q = setup(daw, syn, args.config, shots=ns, nt=64, maxiter=1)
+        Q = setup(daw, syn, args.config, shots=ns, nt=256, maxiter=1)
         run_redatum(daw, Q, max_batch=ns)
     for i in range(args.warmup):
         _log(f"warmup step {i + 1}/{args.warmup}")

@@ -824,10 +824,12 @@ static int autotune_tiles(Batch &B)
         const bool timing = g.timing;
         const long long pts = g.pts, launches = g.launches;
         g.timing = false;
-        // Probes of 64 steps, three rounds over all candidates (round-robin, so
+        // Probes of 192 steps, three rounds over all candidates (round-robin, so
         // that clock drift during the tuning does not favour the candidates
-        // timed first or last), median of the three per candidate.
-        const int probe = std::min(64, B.nt);
+        // timed first or last), median of the three per candidate.  (64-step
+        // probes tied 16x64 and 20x64 back-propagation at 201x801, 24 shots,
+        // where full sweeps differ by 4 %.)
+        const int probe = std::min(192, B.nt);
         const bool log = getenv("DAW_TUNE_LOG") != nullptr;
         static const char *names[3] = {"forward", "back-propagation", "re-injection"};
         hipEvent_t e0, e1;
