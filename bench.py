@@ -638,8 +638,10 @@ def main():
         b = batches[i % nbat]
         _log(f"timed step {i + 1}/{args.steps}: shots {b['shots'][0] + 1}..{b['shots'][1]}")
         daw.reset_stats()
+        ts = time.perf_counter()
         hists[i % nbat] = step(i)
         per_step.append((b["shots"][1] - b["shots"][0], daw.stats(), [daw.get_launch(m) for m in range(3)]))
+        _log(f"  {time.perf_counter() - ts:.2f} s, of which sweeps {sum(per_step[-1][1]['sweep_ms']) / 1e3:.2f} s")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     daw.set_timing(False)
