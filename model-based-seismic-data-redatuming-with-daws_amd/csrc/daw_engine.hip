@@ -848,8 +848,21 @@ static int autotune_tiles(Batch &B)
             const RecItem *d_items = items ? W.d_items.as<RecItem>() : nullptr;
             const int n_items = items ? (int)items->size() : 0;
             std::vector<float> t(cand.size() * kRounds, 0.0f);
+            // rounds 2 and 3 time only the candidates within 15 % of round 1's
+            // best (the rest keep their round-1 time as their median)
+            std::vector<char> live(cand.size(), 1);
             for (int rep = 0; rep < kRounds; ++rep) {
+                if (rep == 1) {
+                    float b1 = 1e30f;
+                    for (size_t k = 0; k < cand.size(); ++k) b1 = std::min(b1, t[k * kRounds]);
+                    for (size_t k = 0; k < cand.size(); ++k)
+                        if (t[k * kRounds] > 1.15f * b1) {
+                            live[k] = 0;
+                            for (int r = 1; r < kRounds; ++r) t[k * kRounds + r] = t[k * kRounds];
+                        }
+                }
                 for (size_t k = 0; k < cand.size(); ++k) {
+                    if (!live[k]) continue;
                     W.cfg[mode] = cand[k];
                     if (int rc = finalize_sweepset(W)) return rc;
                     if (int rc = run_sweep(mode, W, d_items, n_items, 4)) return rc;       // warm
@@ -860,17 +873,27 @@ static int autotune_tiles(Batch &B)
                     HIPCHK(hipEventElapsedTime(&t[k * kRounds + rep], e0, e1));
                 }
             }
-            LaunchCfg pick = cand[0];
+            std::vector<float> med(cand.size());
             float best_ms = 1e30f;
             for (size_t k = 0; k < cand.size(); ++k) {
                 float *r = &t[k * kRounds];
                 std::sort(r, r + kRounds);
-                const float ms = r[kRounds / 2];
+                med[k] = r[kRounds / 2];
                 if (log)
                     fprintf(stderr, "[daw tune] %d shots %dx%d grid, %s: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
                             B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], cand[k].steps, cand[k].tz,
-                            64 * cand[k].txw, 1000.0f * ms / probe);
-                if (ms < best_ms) { best_ms = ms; pick = cand[k]; }
+                            64 * cand[k].txw, 1000.0f * med[k] / probe);
+                best_ms = std::min(best_ms, med[k]);
+            }
+            // Among the candidates within 1.5 % of the fastest probe, the
+            // smallest tile (most workgroups per launch) wins: at 201x801 with
+            // 24 shots the probes tie 16x64 and 20x64 back-propagation, whose
+            // full sweeps differ by 1-4 % in favour of 16x64.
+            LaunchCfg pick = cand[0];
+            int pick_area = 1 << 30;
+            for (size_t k = 0; k < cand.size(); ++k) {
+                const int area = cand[k].tz * 64 * cand[k].txw;
+                if (med[k] <= 1.015f * best_ms && area < pick_area) { pick = cand[k]; pick_area = area; }
             }
             W.cfg[mode] = pick;
             best[3 * mode] = pick.steps; best[3 * mode + 1] = pick.tz; best[3 * mode + 2] = pick.txw;
