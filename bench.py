@@ -563,6 +563,8 @@ def main():
     ap.add_argument("--snapshots", choices=["fp32", "bf16", "fp16"], default="fp32",
                     help="storage of the Hessian operator's wavefield snapshots")
     ap.add_argument("--graph", action="store_true", help="replay the sweeps as captured hipGraphs")
+    ap.add_argument("--rec-batch", type=int, default=1, choices=[0, 1],
+                    help="1: Hicks records as batched launches every few steps; 0: record blocks in the step launches")
     ap.add_argument("--f64-shots", type=int, default=0,
                     help="also run the FP64 oracle of the plain method on this many shots (slow, CPU)")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -593,6 +595,7 @@ def main():
     daw.set_tile(args.tile_rows, args.tile_cols)
     daw.set_snapshot_format(args.snapshots)
     daw.set_graph(args.graph)
+    daw.set_record_batch(args.rec_batch)
     if args.config in (3, 5):
         return main_uneven(args, daw, syn)
     t_start = T_PROC0
@@ -752,7 +755,7 @@ def main():
                                f"batch of {bsz} shots advanced together (steps walk the {nbat} batches of the line)",
                    "shots": nsh, "shots_per_step": bsz, "shots_timed": shots_timed, "nt": w.nt,
                    "grid": [w.nz, w.nx], "h": w.h, "npml": w.npml, "receivers": int(P["rec"].shape[1]),
-                   "shots_per_launch": shots_per_launch, "snapshots": args.snapshots, "graph": args.graph,
+                   "shots_per_launch": shots_per_launch, "snapshots": args.snapshots, "graph": args.graph, "record_batch": bool(args.rec_batch),
                    "launch": {m: {"steps": v[0], "tile": [v[1], v[2]]} for m, v in launch.items()}},
         "gpts_per_s": gpts,
         "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)

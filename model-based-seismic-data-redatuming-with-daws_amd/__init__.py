@@ -38,7 +38,7 @@ EXPORTS = [
     "daw_set_graph", "daw_set_steps", "daw_get_steps", "daw_redatum_uneven",
     "daw_get_launch", "daw_sublinearminres_log", "daw_write_minres_log", "daw_readpar", "daw_read_acqui",
     "daw_subresample", "daw_subresample_dev", "daw_set_minres_arith",
-    "daw_subresample_dev_noalloc", "daw_set_krylov_format",
+    "daw_subresample_dev_noalloc", "daw_set_krylov_format", "daw_set_record_batch",
 ]
 
 
@@ -660,8 +660,15 @@ def filmartin2d_dev(din_dev, dout_dev, dt, flow, fhig, llow, lhig):
 
 def set_tile(rows, cols=64):
     """Step-kernel tile rows x cols (8x128, 16x64, 16x128, 20x64, 24x64, 32x64,
-    32x128, 64x64), or rows = 0 for the autotuner (default)."""
+    32x128, 64x64; rows x 58 for the column march, rows = 8, 12, 16, 24, 32),
+    or rows = 0 for the autotuner (default)."""
     _check(lib().daw_set_tile(int(rows), int(cols)))
+
+
+def set_record_batch(on):
+    """True (default): Hicks records as batched launches every few steps;
+    False: record blocks inside the step launches.  Same results."""
+    _check(lib().daw_set_record_batch(1 if on else 0))
 
 
 def get_tile():

@@ -30,6 +30,9 @@ constexpr int TX = 64;
 constexpr int kRecPerBlock = 8;
 constexpr int kRecStride = 444;   // 441 window products padded to whole float4s
 constexpr int kBlock = 256;
+// Time slots of the record band (slot of step it: it & (kBandSlots-1)): a
+// step's band is recorded within kBandSlots steps of being written.
+constexpr int kBandSlots = 8;
 
 struct ShotDev {
     // grid
@@ -55,7 +58,8 @@ struct ShotDev {
     int vinj;                           // 1: the source is a vz source (type_src = 1), applied in the velocity phase
     // record band: band_row[z] = slot >= 0 for rows some record window touches
     const int *band_row;
-    float *pband[4];                    // [nslot][ldx], time slot it & 3
+    float *pband;                       // kBandSlots time slots of [nslot][ldx], it & (kBandSlots-1)
+    int band_stride;                    // floats per time slot
     // snapshots (kBack writes, kReinject reads) of the interior rows, [nt][n1][spitch]:
     // element (z, extended column gx) at z*spitch + gx - sxb, sxb = npml
     // rounded down to 64, so a wavefront's 64 columns are one aligned run.
@@ -76,9 +80,21 @@ struct RecItem {
     int ix1, ix2;       // 0-based extended-grid centre node
     int clip;           // 0: window clipped to the extended grid, 1: to the interior
     int reverse;        // write time index nt-1-it instead of it
-    int pad;
+    int cstride;        // coefficient k of the point at coefT[k * cstride]
     const float *coef;  // 441 coefficients, (i1+10)*21 + (i2+10) (record order)
+    const float *coefT; // the same, transposed: the point set's coefficient k of all points contiguous
     float *out;         // trace base (nt floats)
+};
+
+// Batched record groups: items first .. first+count-1 (count <= 64, one
+// shot, consecutive points of one point set) whose windows lie in grid rows
+// z0 .. z0+nz-1 and columns x0 .. x0+nx-1 (nz <= kRecRows, nx <= kRecCols).
+constexpr int kRecSteps = 4;
+constexpr int kRecRows = 24;
+constexpr int kRecCols = 140;
+struct RecGroup {
+    int first, count;
+    int z0, nz, x0, nx;
 };
 
 struct StepParams {

@@ -48,10 +48,16 @@ static int fail(int code, const char *fmt, ...)
     } while (0)
 
 // ------------------------------------------------------------------ state
-// Launch configuration of one sweep mode: time steps per launch and tile.
+// Launch configuration of one sweep mode: time steps per launch and tile;
+// march = 1: the column-march kernel, tz rows per wavefront (txw unused).
+// rec_batch = 1: the Hicks records of a sweep run as batched launches every
+// few steps (k_rec_group) instead of as record blocks inside the step launches.
 struct LaunchCfg {
-    int steps = 1, tz = 16, txw = 1;
+    int steps = 1, tz = 16, txw = 1, march = 0, rec_batch = 0;
 };
+// The tile's columns as the API reports them: 64 per wavefront, or the 58
+// owned columns of a marching wavefront.
+static int tile_cols(const LaunchCfg &c) { return c.march ? kMarchCols : 64 * c.txw; }
 
 struct Global {
     std::mutex mu;
@@ -66,6 +72,8 @@ struct Global {
     int max_batch_used = 0;
     int tz = 16;               // tile rows of k_step
     int txw = 1;               // tile width in wavefronts (64 columns each)
+    int march = 0;             // 1: the column-march kernel with tz rows per wavefront
+    int rec_batch = 1;         // 1: batched record launches (k_rec_group) instead of in-launch record blocks
     bool tz_auto = true;       // pick tz per batch shape with a timed probe
     int steps = 1;             // time steps per launch of the sweep kernels (1: k_step, 2: k_step2)
     bool steps_auto = true;    // let the probe pick the steps per launch too
@@ -74,7 +82,7 @@ struct Global {
     bool graph = false;        // replay full sweeps as captured hipGraphs
     int minres_arith = 0;      // 0: FP64 dot products and scalars; 1: the reference's FP32 arithmetic
     int krylov_fmt = kSnapF32; // storage of MINRES r1, r2, w, w1, w2 in the FP64 path (SnapFormat)
-    std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw} per mode
+    std::map<std::vector<int>, std::vector<int>> tz_cache;   // batch shape -> {steps, tz, txw, march} per mode
     LaunchCfg last_cfg[3];     // launch configuration of each mode in the last tuned batch
 };
 static Global g;
@@ -209,13 +217,20 @@ struct PointSet {
 
 // Coefficient tiles for the record blocks: each 21x21 tile in the order the
 // record sums it, (i1+10)*21 + (i2+10) (PointSet keeps (i1+10) + 21*(i2+10)).
-static int upload_record_coef(DevBuf &b, const std::vector<float> &coef)
+// bt gets the same tiles transposed, coefficient k of every point
+// contiguous (t[k*n + r]), for the batched record's lanes of neighbouring points.
+static int upload_record_coef(DevBuf &b, DevBuf &bt, const std::vector<float> &coef)
 {
-    std::vector<float> t(coef.size());
-    for (size_t r = 0; r < coef.size() / 441; ++r)
+    const size_t n = coef.size() / 441;
+    std::vector<float> t(coef.size()), tt(coef.size());
+    for (size_t r = 0; r < n; ++r)
         for (int a = 0; a < 21; ++a)
-            for (int c = 0; c < 21; ++c) t[r * 441 + a * 21 + c] = coef[r * 441 + a + 21 * c];
-    return upload(b, t);
+            for (int c = 0; c < 21; ++c) {
+                t[r * 441 + a * 21 + c] = coef[r * 441 + a + 21 * c];
+                tt[(size_t)(a * 21 + c) * n + r] = coef[r * 441 + a + 21 * c];
+            }
+    if (int rc = upload(b, t)) return rc;
+    return upload(bt, tt);
 }
 
 static int make_points(PointSet &P, const int *tab, int n, const Grid &G, bool interior_clip)
@@ -370,7 +385,8 @@ struct SweepSet {
     std::vector<ShotDev> host;
     DevBuf d_shots;
     LaunchCfg cfg[3];                                   // per SweepMode
-    std::map<std::vector<int>, TileList> tiles;         // per (tz, txw, TileOrder)
+    std::vector<int2> inj_band;                         // per shot: rows with receiver injection (z0, z1)
+    std::map<std::vector<int>, TileList> tiles;         // per tiles_key()
     int nt_max = 0;
     long long pts_per_step = 0;
     int snap = kSnapF32;       // SnapFormat of the shots' snapshot store
@@ -379,6 +395,8 @@ struct SweepSet {
     // asynchronous upload
     DevBuf d_items;
     std::vector<RecItem> items;
+    DevBuf d_groups;                                    // record groups of the items (batched record)
+    std::vector<RecGroup> groups;
     // captured sweeps, keyed by everything their launches bake in
     std::map<std::vector<long long>, hipGraphExec_t> graphs;
     SweepSet() = default;
@@ -393,7 +411,45 @@ struct SweepSet {
 };
 
 // Steps per launch of the sweeps: two only where the tile has a two-step kernel.
-static int steps_per_launch(const LaunchCfg &c) { return (c.steps == 2 && tile2_shape_ok(c.tz, c.txw)) ? 2 : 1; }
+static int steps_per_launch(const LaunchCfg &c)
+{
+    return (!c.march && c.steps == 2 && tile2_shape_ok(c.tz, c.txw)) ? 2 : 1;
+}
+
+static std::vector<int> tiles_key(const LaunchCfg &c, int mode)
+{
+    return {c.tz, c.txw, c.march ? 10 + (mode == kBack) : tile_order(mode), c.march};
+}
+
+// Row segments of the column march: h rows each, except that a segment which
+// would reach rows b0..b1 (those with receiver injection terms, about 14 per
+// point at 401 receivers) has kMarchBandRows rows, so that no wavefront walks
+// many of them.  Every segment has exactly h or kMarchBandRows rows (the
+// kernel unrolls both walks); the last may reach past the grid.
+static std::vector<int2> march_segments(int n1e, int h, int b0, int b1)
+{
+    std::vector<int2> segs;
+    for (int z = 0; z < n1e;) {
+        const int r = (b0 <= b1 && z <= b1 && z + h > b0) ? kMarchBandRows : h;
+        segs.push_back(make_int2(z, r));
+        z += r;
+    }
+    return segs;
+}
+
+// The exact adjoint (k_adj) runs on the back-propagation's tile, or on 16x64
+// where the back-propagation marches.
+static LaunchCfg adjoint_cfg(const LaunchCfg &c) { return c.march ? LaunchCfg{1, 16, 1, 0, c.rec_batch} : c; }
+
+// The column march runs the forward sweep, and the Hessian sweeps with FP32 snapshots.
+static bool march_ok(const LaunchCfg &c, int mode, int snap) { return !c.march || mode == kFwd || snap == kSnapF32; }
+
+// The configured launch of a mode (16x64 where a fixed march cannot run it).
+static LaunchCfg fixed_cfg(int mode, int snap)
+{
+    const LaunchCfg c{g.steps, g.tz, g.txw, g.march, g.rec_batch};
+    return march_ok(c, mode, snap) ? c : LaunchCfg{1, 16, 1, 0, g.rec_batch};
+}
 
 // Copies the record items of the next sweep to the batch's item buffer
 // (stream-ordered: the previous sweep has read the old items by then).
@@ -406,13 +462,47 @@ static int set_items(SweepSet &W, const std::vector<RecItem> &items)
         if (int rc = W.d_items.alloc(std::max(bytes, 2 * W.d_items.bytes))) return rc;
     }
     HIPCHK(hipStreamSynchronize(g.stream));   // the host copy of the last upload is free
-    // receiver-major: the items of one receiver (one coefficient tile) for
-    // all shots run in adjacent wavefronts and share the tile in cache
+    // by shot, then point: the batched record's lanes of a wavefront read
+    // neighbouring band columns and transposed coefficients
     W.items = items;
-    std::stable_sort(W.items.begin(), W.items.end(),
-                     [](const RecItem &a, const RecItem &b) { return a.coef < b.coef; });
+    std::stable_sort(W.items.begin(), W.items.end(), [](const RecItem &a, const RecItem &b) {
+        return a.shot != b.shot ? a.shot < b.shot : a.coef < b.coef;
+    });
     if (!items.empty())
         HIPCHK(hipMemcpyAsync(W.d_items.p, W.items.data(), items.size() * sizeof(RecItem), hipMemcpyHostToDevice,
+                              g.stream));
+    // record groups: runs of consecutive points of one point set of one shot
+    // whose windows (clipped to the grid) fit a kRecRows x kRecCols block
+    W.groups.clear();
+    for (size_t i = 0; i < W.items.size();) {
+        const RecItem &A = W.items[i];
+        const ShotDev &S = W.host[A.shot];
+        auto win = [&](const RecItem &R, int &z0, int &z1, int &x0, int &x1) {
+            z0 = std::max(R.ix1 - 10, 0); z1 = std::min(R.ix1 + 10, S.n1e - 1);
+            x0 = std::max(R.ix2 - 10, 0); x1 = std::min(R.ix2 + 10, S.n2e - 1);
+        };
+        int z0, z1, x0, x1;
+        win(A, z0, z1, x0, x1);
+        size_t k = i + 1;
+        for (; k < W.items.size() && k - i < 64; ++k) {
+            const RecItem &B = W.items[k], &P = W.items[k - 1];
+            if (B.shot != A.shot || B.cstride != P.cstride || B.coefT != P.coefT + 1) break;
+            int a0, a1, b0, b1;
+            win(B, a0, a1, b0, b1);
+            const int nz0 = std::min(z0, a0), nz1 = std::max(z1, a1), nx0 = std::min(x0, b0), nx1 = std::max(x1, b1);
+            if (nz1 - nz0 + 1 > kRecRows || nx1 - nx0 + 1 > kRecCols) break;
+            z0 = nz0; z1 = nz1; x0 = nx0; x1 = nx1;
+        }
+        W.groups.push_back(RecGroup{(int)i, (int)(k - i), z0, z1 - z0 + 1, x0, x1 - x0 + 1});
+        i = k;
+    }
+    const size_t gbytes = std::max<size_t>(W.groups.size(), 1) * sizeof(RecGroup);
+    if (gbytes > W.d_groups.bytes) {
+        W.clear_graphs();
+        if (int rc = W.d_groups.alloc(std::max(gbytes, 2 * W.d_groups.bytes))) return rc;
+    }
+    if (!W.groups.empty())
+        HIPCHK(hipMemcpyAsync(W.d_groups.p, W.groups.data(), W.groups.size() * sizeof(RecGroup), hipMemcpyHostToDevice,
                               g.stream));
     return 0;
 }
@@ -427,12 +517,35 @@ static int finalize_sweepset(SweepSet &W)
         W.nt_max = std::max(W.nt_max, W.host[s].nt);
         W.pts_per_step += (long long)W.host[s].n1e * W.host[s].n2e;
     }
-    for (int m = 0; m < 3; ++m) {
-        const LaunchCfg &c = W.cfg[m];
-        const std::vector<int> key{c.tz, c.txw, tile_order(m)};
+    for (int m = 0; m < 4; ++m) {
+        // modes 0-2, then the exact adjoint's tile (the back-propagation's)
+        const LaunchCfg c = m < 3 ? W.cfg[m] : adjoint_cfg(W.cfg[kBack]);
+        const int order = m < 3 ? m : kBack;
+        const std::vector<int> key = tiles_key(c, order);
         if (W.tiles.count(key)) continue;
         std::vector<int> tps(W.nshots);
         std::vector<std::vector<int>> cls(W.nshots);
+        if (c.march) {
+            // per shot: row segments (the back-propagation's receiver rows in
+            // short ones) times groups of four strips, strips fastest; entry
+            // (shot, z0 << 16 | rows << 8 | group)
+            std::vector<std::vector<int>> ys(W.nshots);
+            for (int s = 0; s < W.nshots; ++s) {
+                const ShotDev &S = W.host[s];
+                const int2 b = order == kBack && s < (int)W.inj_band.size() ? W.inj_band[s] : make_int2(0, -1);
+                const int groups = ((S.n2e + kMarchCols - 1) / kMarchCols + 3) / 4;
+                for (const int2 &sg : march_segments(S.n1e, c.tz, b.x, b.y))
+                    for (int q = 0; q < groups; ++q) ys[s].push_back((sg.x << 16) | (sg.y << 8) | q);
+                tps[s] = (int)ys[s].size();
+                cls[s].assign(tps[s], 0);
+            }
+            std::vector<int2> tl = make_tiles(tps, cls, kTileRaster);
+            for (int2 &e : tl) e.y = ys[e.x][e.y];
+            TileList &T = W.tiles[key];
+            T.n = (int)tl.size();
+            if (int rc = upload(T.d, tl)) return rc;
+            continue;
+        }
         const int txt = TX * c.txw;
         for (int s = 0; s < W.nshots; ++s) {
             const ShotDev &S = W.host[s];
@@ -447,7 +560,7 @@ static int finalize_sweepset(SweepSet &W)
                 cls[s][t] = (2 * rows < c.tz || 2 * cols < txt) ? 0 : inner ? 1 : 2;
             }
         }
-        std::vector<int2> tl = make_tiles(tps, cls, tile_order(m));
+        std::vector<int2> tl = make_tiles(tps, cls, tile_order(order));
         TileList &T = W.tiles[key];
         T.n = (int)tl.size();
         if (int rc = upload(T.d, tl)) return rc;
@@ -466,7 +579,7 @@ static int refresh_shots(SweepSet &W)
 static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int nsteps)
 {
     const LaunchCfg &c = W.cfg[mode];
-    const TileList &T = W.tiles.at({c.tz, c.txw, tile_order(mode)});
+    const TileList &T = W.tiles.at(tiles_key(c, mode));
     for (int s = 0; s < W.nshots; ++s) {
         const ShotDev &S = W.host[s];
         const size_t pe = (size_t)S.n1e * S.ldx;
@@ -480,12 +593,41 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
     P.tiles = T.d.as<int2>();
     P.rec = d_rec;
     P.nrec = nrec;
-    // launch k advances steps it .. it+nsub-1 and records the steps of launch k-1
     const int per = steps_per_launch(c);
-    int cur = 0, prev_it = 0, prev_n = 0;
+    int cur = 0;
+    if (c.rec_batch) {
+        // launch k advances steps it .. it+nsub-1; the record of the steps
+        // since the last record runs as one batched launch before the band's
+        // time slots would be reused, and after the last step
+        P.rec_nsub = 0;
+        P.nrec_blocks = 0;
+        int rec_from = 0;
+        for (int it = 0; it < nsteps; it += per) {
+            const int n = std::min(per, nsteps - it);
+            P.ntiles = T.n;
+            P.it = it;
+            P.nsub = n;
+            P.cur = cur;
+            if (c.march) launch_march(mode, P, c.tz, g.stream);
+            else if (per == 2) launch_step2(mode, W.snap, P, c.tz, c.txw, g.stream);
+            else launch_step(mode, W.snap, P, c.tz, c.txw, g.stream);
+            cur ^= 1;
+            const int done = it + n;
+            if (nrec > 0 && (done == nsteps || done - rec_from + per > kBandSlots)) {
+                launch_rec_batch(P.shots, d_rec, W.d_groups.as<RecGroup>(), (int)W.groups.size(), rec_from,
+                                 done - rec_from, g.stream);
+                rec_from = done;
+            }
+        }
+        return;
+    }
+    // record blocks inside the step launches: launch k advances steps it ..
+    // it+nsub-1 and records the steps of launch k-1 (its first workgroups)
+    int prev_it = 0, prev_n = 0;
     for (int it = 0;; it += per) {
         const bool more = it < nsteps;
         const int n = more ? std::min(per, nsteps - it) : 0;
+        if (!more && (nrec == 0 || prev_n == 0)) break;
         P.ntiles = more ? T.n : 0;
         P.it = it;
         P.nsub = n;
@@ -493,7 +635,8 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
         P.rec_it = prev_it;
         P.rec_nsub = nrec > 0 ? prev_n : 0;
         P.nrec_blocks = (nrec * P.rec_nsub + kRecPerBlock - 1) / kRecPerBlock;
-        if (per == 2) launch_step2(mode, W.snap, P, c.tz, c.txw, g.stream);
+        if (c.march) launch_march(mode, P, c.tz, g.stream);
+        else if (per == 2) launch_step2(mode, W.snap, P, c.tz, c.txw, g.stream);
         else launch_step(mode, W.snap, P, c.tz, c.txw, g.stream);
         cur ^= 1;
         prev_it = it;
@@ -517,8 +660,8 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
     const int nsteps = nsteps_limit > 0 ? std::min(nsteps_limit, W.nt_max) : W.nt_max;
     if (g.graph && nsteps_limit == 0) {
         const LaunchCfg &c = W.cfg[mode];
-        const std::vector<long long> key{mode, nsteps, nrec, steps_per_launch(c), c.tz, c.txw, W.snap,
-                                         (long long)(intptr_t)d_rec, (long long)(intptr_t)W.tiles.at({c.tz, c.txw, tile_order(mode)}).d.p,
+        const std::vector<long long> key{mode, nsteps, nrec, steps_per_launch(c), c.tz, c.txw, c.march, c.rec_batch, W.snap,
+                                         (long long)(intptr_t)d_rec, (long long)(intptr_t)W.tiles.at(tiles_key(c, mode)).d.p,
                                          (long long)(intptr_t)W.d_shots.p};
         auto hit = W.graphs.find(key);
         hipGraphExec_t ex = nullptr;
@@ -556,7 +699,7 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
     }
     g.pts += W.pts_per_step * nsteps;
     const int per = steps_per_launch(W.cfg[mode]);
-    g.launches += (nsteps + per - 1) / per + 1;
+    g.launches += (nsteps + per - 1) / per;   // step launches (the batched records not counted)
     return 0;
 }
 
@@ -572,8 +715,8 @@ static int run_adjoint_sweep(bool hess, SweepSet &W, const RecItem *d_rec, int n
         launch_fill(S.vx[0], pe, 0.0f, g.stream);
         launch_fill(S.vz[0], pe, 0.0f, g.stream);
     }
-    const LaunchCfg &c = W.cfg[kBack];                  // the one-step tile of the back-propagation
-    const TileList &T = W.tiles.at({c.tz, c.txw, tile_order(kBack)});
+    const LaunchCfg c = adjoint_cfg(W.cfg[kBack]);      // the one-step tile of the back-propagation
+    const TileList &T = W.tiles.at(tiles_key(c, kBack));
     StepParams P;
     P.shots = W.d_shots.as<ShotDev>();
     P.tiles = T.d.as<int2>();
@@ -627,7 +770,7 @@ static int alloc_store(ShotStore &st, int nshots, const Grid &G, int nslot, int 
 {
     const size_t pe = G.plane_e();
     if (int rc = st.fields.alloc((size_t)nshots * 8 * pe * sizeof(float))) return rc;
-    if (int rc = st.band.alloc((size_t)nshots * 4 * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
+    if (int rc = st.band.alloc((size_t)nshots * kBandSlots * std::max(nslot, 1) * G.ldx * sizeof(float))) return rc;
     st.fmt = fmt;
     if (fmt == kSnapF32) {
         if (int rc = st.snap.alloc((size_t)nshots * 3 * G.plane_s() * nt * sizeof(float))) return rc;
@@ -654,8 +797,8 @@ static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &s
     S.px[1] = f + 4 * pe; S.pz[1] = f + 5 * pe; S.vx[1] = f + 6 * pe; S.vz[1] = f + 7 * pe;
     if (band) {
         S.band_row = band->d_row.as<int>();
-        float *b = st.band.as<float>() + (size_t)s * 4 * std::max(nslot, 1) * G.ldx;
-        for (int k = 0; k < 4; ++k) S.pband[k] = b + (size_t)k * std::max(nslot, 1) * G.ldx;
+        S.band_stride = std::max(nslot, 1) * G.ldx;
+        S.pband = st.band.as<float>() + (size_t)s * kBandSlots * S.band_stride;
     }
     const size_t pn = G.plane_s() * nt;
     S.spitch = G.spitch();
@@ -687,12 +830,14 @@ static void set_inject(ShotDev &S, const Inject &I, const float *amp, int amp_st
 }
 
 static void add_records(std::vector<RecItem> &v, const PointSet &P, int shot, int clip, int reverse, const DevBuf &coef,
-                        float *out, int nt)
+                        const DevBuf &coefT, float *out, int nt)
 {
     for (int r = 0; r < P.n; ++r) {
         RecItem R;
-        R.shot = shot; R.ix1 = P.ix1[r]; R.ix2 = P.ix2[r]; R.clip = clip; R.reverse = reverse; R.pad = 0;
+        R.shot = shot; R.ix1 = P.ix1[r]; R.ix2 = P.ix2[r]; R.clip = clip; R.reverse = reverse;
+        R.cstride = P.n;
         R.coef = coef.as<float>() + (size_t)r * 441;
+        R.coefT = coefT.as<float>() + r;
         R.out = out + (size_t)r * nt;
         v.push_back(R);
     }
@@ -747,6 +892,7 @@ struct Geo {
     Inject inj_rec_c;               // receivers with coefficient c (adjoint data spreading, R^T)
     Band band;
     DevBuf d_rec_coef, d_dat_coef;
+    DevBuf d_rec_coefT, d_dat_coefT;   // transposed (batched record)
 };
 
 struct Batch {
@@ -783,18 +929,20 @@ static int autotune_tiles(Batch &B)
 {
     SweepSet &W = B.W;
     if (!g.tz_auto && !g.steps_auto) return 0;
-    // candidates: {steps, rows, wavefronts across}; fixed settings are kept
-    static const int shapes[][3] = {{1, 8, 2}, {1, 16, 1}, {1, 16, 2}, {1, 20, 1}, {1, 24, 1}, {1, 32, 1}, {1, 32, 2},
-                                    {1, 64, 1}, {2, 16, 1}, {2, 32, 1}, {2, 16, 2}};
+    // candidates: {steps, rows, wavefronts across, march}; fixed settings are kept
+    static const int shapes[][4] = {{1, 8, 2, 0},  {1, 16, 1, 0}, {1, 16, 2, 0}, {1, 20, 1, 0}, {1, 24, 1, 0},
+                                    {1, 32, 1, 0}, {1, 32, 2, 0}, {1, 64, 1, 0}, {2, 16, 1, 0}, {2, 32, 1, 0},
+                                    {2, 16, 2, 0}, {1, 8, 0, 1},  {1, 12, 0, 1}, {1, 16, 0, 1}, {1, 24, 0, 1},
+                                    {1, 32, 0, 1}};
     std::vector<LaunchCfg> cand;
     for (const auto &sh : shapes) {
         if (!g.steps_auto && sh[0] != g.steps) continue;
-        if (!g.tz_auto && (sh[1] != g.tz || sh[2] != g.txw)) continue;
-        cand.push_back({sh[0], sh[1], sh[2]});
+        if (!g.tz_auto && (sh[1] != g.tz || sh[2] != g.txw || sh[3] != g.march)) continue;
+        cand.push_back({sh[0], sh[1], sh[2], sh[3], g.rec_batch});
     }
     if (cand.empty() || B.store.empty() || !B.store[0].has_snap()) return 0;
     const std::vector<int> key{(int)(W.pts_per_step >> 10), B.geo_of(0).G.n2e, B.nshots, g.steps_auto ? 0 : g.steps,
-                               g.tz_auto ? 0 : g.tz, g.tz_auto ? 0 : g.txw};
+                               g.tz_auto ? 0 : g.tz, g.tz_auto ? 0 : g.txw, g.tz_auto ? 0 : g.march, W.snap, g.rec_batch};
     auto hit = g.tz_cache.find(key);
     if (hit == g.tz_cache.end()) {
         DevBuf zeros;
@@ -816,10 +964,10 @@ static int autotune_tiles(Batch &B)
         std::vector<RecItem> it_reinj, it_fwd;
         for (int s = 0; s < ns; ++s) {
             const Geo &Q = B.geo_of(s);
-            add_records(it_reinj, Q.rec, s, 0, 1, Q.d_rec_coef, rec_out.as<float>() + B.roff[s], Q.nt);
-            add_records(it_fwd, Q.rec, s, 0, 0, Q.d_rec_coef, rec_out.as<float>() + B.roff[s], Q.nt);
+            add_records(it_reinj, Q.rec, s, 0, 1, Q.d_rec_coef, Q.d_rec_coefT, rec_out.as<float>() + B.roff[s], Q.nt);
+            add_records(it_fwd, Q.rec, s, 0, 0, Q.d_rec_coef, Q.d_rec_coefT, rec_out.as<float>() + B.roff[s], Q.nt);
             if (Q.dat.n > 0)
-                add_records(it_fwd, Q.dat, s, 1, 0, Q.d_dat_coef, dat_out.as<float>() + B.doff[s], Q.nt);
+                add_records(it_fwd, Q.dat, s, 1, 0, Q.d_dat_coef, Q.d_dat_coefT, dat_out.as<float>() + B.doff[s], Q.nt);
         }
         const bool timing = g.timing;
         const long long pts = g.pts, launches = g.launches;
@@ -835,7 +983,7 @@ static int autotune_tiles(Batch &B)
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        std::vector<int> best(9);
+        std::vector<int> best(15);
         constexpr int kRounds = 3;
         for (int mode : {kBack, kReinject, kFwd}) {
             if (mode == kFwd && B.inj_src.size() == (size_t)ns) {   // the forward sweep injects the sources
@@ -847,10 +995,19 @@ static int autotune_tiles(Batch &B)
                 if (int rc = set_items(W, *items)) return rc;
             const RecItem *d_items = items ? W.d_items.as<RecItem>() : nullptr;
             const int n_items = items ? (int)items->size() : 0;
-            std::vector<float> t(cand.size() * kRounds, 0.0f);
+            std::vector<float> t(cand.size() * kRounds, 1e30f);
             // rounds 2 and 3 time only the candidates within 15 % of round 1's
-            // best (the rest keep their round-1 time as their median)
+            // best (the rest keep their round-1 time as their median); the
+            // column march has no 16-bit snapshot codec
             std::vector<char> live(cand.size(), 1);
+            for (size_t k = 0; k < cand.size(); ++k)
+                if (!march_ok(cand[k], mode, W.snap)) live[k] = 0;
+            if (std::find(live.begin(), live.end(), 1) == live.end()) {
+                const LaunchCfg d = fixed_cfg(mode, W.snap);
+                best[5 * mode] = d.steps; best[5 * mode + 1] = d.tz; best[5 * mode + 2] = d.txw; best[5 * mode + 3] = d.march;
+                best[5 * mode + 4] = d.rec_batch;
+                continue;
+            }
             for (int rep = 0; rep < kRounds; ++rep) {
                 if (rep == 1) {
                     float b1 = 1e30f;
@@ -879,10 +1036,10 @@ static int autotune_tiles(Batch &B)
                 float *r = &t[k * kRounds];
                 std::sort(r, r + kRounds);
                 med[k] = r[kRounds / 2];
-                if (log)
-                    fprintf(stderr, "[daw tune] %d shots %dx%d grid, %s: %d step(s)/launch, tile %dx%d  %.1f us/step\n",
-                            B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], cand[k].steps, cand[k].tz,
-                            64 * cand[k].txw, 1000.0f * med[k] / probe);
+                if (log && med[k] < 1e29f)
+                    fprintf(stderr, "[daw tune] %d shots %dx%d grid, %s: %d step(s)/launch, %s %dx%d  %.1f us/step\n",
+                            B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], cand[k].steps,
+                            cand[k].march ? "march" : "tile", cand[k].tz, tile_cols(cand[k]), 1000.0f * med[k] / probe);
                 best_ms = std::min(best_ms, med[k]);
             }
             // Among the candidates within 1.5 % of the fastest probe, the
@@ -892,11 +1049,12 @@ static int autotune_tiles(Batch &B)
             LaunchCfg pick = cand[0];
             int pick_area = 1 << 30;
             for (size_t k = 0; k < cand.size(); ++k) {
-                const int area = cand[k].tz * 64 * cand[k].txw;
+                const int area = cand[k].tz * tile_cols(cand[k]);
                 if (med[k] <= 1.015f * best_ms && area < pick_area) { pick = cand[k]; pick_area = area; }
             }
             W.cfg[mode] = pick;
-            best[3 * mode] = pick.steps; best[3 * mode + 1] = pick.tz; best[3 * mode + 2] = pick.txw;
+            best[5 * mode] = pick.steps; best[5 * mode + 1] = pick.tz; best[5 * mode + 2] = pick.txw;
+            best[5 * mode + 3] = pick.march; best[5 * mode + 4] = pick.rec_batch;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -906,7 +1064,7 @@ static int autotune_tiles(Batch &B)
         hit = g.tz_cache.emplace(key, best).first;
     }
     const std::vector<int> &b = hit->second;
-    for (int m = 0; m < 3; ++m) W.cfg[m] = {b[3 * m], b[3 * m + 1], b[3 * m + 2]};
+    for (int m = 0; m < 3; ++m) W.cfg[m] = {b[5 * m], b[5 * m + 1], b[5 * m + 2], b[5 * m + 3], b[5 * m + 4]};
     for (int m = 0; m < 3; ++m) g.last_cfg[m] = W.cfg[m];
     return finalize_sweepset(W);
 }
@@ -953,8 +1111,8 @@ static int hessian(Batch &B, const float *amp, const HessOut &o)
         S.amp_rev = 0;
         S.wf = o.wf;
         set_range(S, Q.G, o.variant);
-        if (o.record && o.sis) add_records(items, Q.rec, s, 0, o.reverse, Q.d_rec_coef, o.sis + B.roff[s], Q.nt);
-        if (o.datum) add_records(items, Q.dat, s, 1, 0, Q.d_dat_coef, o.datum + B.doff[s], Q.nt);
+        if (o.record && o.sis) add_records(items, Q.rec, s, 0, o.reverse, Q.d_rec_coef, Q.d_rec_coefT, o.sis + B.roff[s], Q.nt);
+        if (o.datum) add_records(items, Q.dat, s, 1, 0, Q.d_dat_coef, Q.d_dat_coefT, o.datum + B.doff[s], Q.nt);
     }
     if (!o.record) for (auto &S : W.host) S.wf = nullptr;
     if (int rc = set_items(W, items)) return rc;
@@ -975,8 +1133,8 @@ static int forward(Batch &B, const float *s_wavelet, int type_src, int record, f
         set_inject(S, B.inj_src[s], s_wavelet, 0, type_src == 1 ? 1 : 0);
         if (type_src != 0 && type_src != 1) S.inj_row = nullptr;
         S.wf = record ? wf : nullptr;
-        if (record && sis) add_records(items, Q.rec, s, 0, 0, Q.d_rec_coef, sis + B.roff[s], Q.nt);
-        if (datum) add_records(items, Q.dat, s, 1, 0, Q.d_dat_coef, datum + B.doff[s], Q.nt);
+        if (record && sis) add_records(items, Q.rec, s, 0, 0, Q.d_rec_coef, Q.d_rec_coefT, sis + B.roff[s], Q.nt);
+        if (datum) add_records(items, Q.dat, s, 1, 0, Q.d_dat_coef, Q.d_dat_coefT, datum + B.doff[s], Q.nt);
     }
     if (int rc = set_items(W, items)) return rc;
     if (int rc = refresh_shots(W)) return rc;
@@ -1001,9 +1159,9 @@ static int make_geo(Geo &Q, const int *rec_tab, int nrec, const int *dat_tab, in
     std::vector<const PointSet *> sets{&Q.rec};
     if (Q.dat.n) sets.push_back(&Q.dat);
     if (int rc = make_band(Q.band, sets, G)) return rc;
-    if (int rc = upload_record_coef(Q.d_rec_coef, Q.rec.coef)) return rc;
+    if (int rc = upload_record_coef(Q.d_rec_coef, Q.d_rec_coefT, Q.rec.coef)) return rc;
     if (Q.dat.n)
-        if (int rc = upload_record_coef(Q.d_dat_coef, Q.dat.coef)) return rc;
+        if (int rc = upload_record_coef(Q.d_dat_coef, Q.d_dat_coefT, Q.dat.coef)) return rc;
     return 0;
 }
 
@@ -1025,9 +1183,10 @@ static int finish_batch(Batch &B, bool want_snap)
     B.store.clear();
     B.store.resize(B.nshots);
     B.W.nshots = B.nshots;
-    for (auto &c : B.W.cfg) c = {g.steps, g.tz, g.txw};
     B.W.snap = fmt < 0 ? (int)kSnapF32 : fmt;
+    for (int m = 0; m < 3; ++m) B.W.cfg[m] = fixed_cfg(m, B.W.snap);
     B.W.host.resize(B.nshots);
+    B.W.inj_band.assign(B.nshots, make_int2(0, -1));
     B.roff.assign(B.nshots + 1, 0);
     B.doff.assign(B.nshots + 1, 0);
     B.nt = 0;
@@ -1035,6 +1194,7 @@ static int finish_batch(Batch &B, bool want_snap)
         const Geo &Q = B.geo_of(s);
         if (int rc = alloc_store(B.store[s], 1, Q.G, Q.band.nslot, Q.nt, fmt)) return rc;
         fill_shot(B.W.host[s], Q.G, *Q.M, B.store[s], 0, Q.nt, Q.band.nslot, &Q.band);
+        B.W.inj_band[s] = make_int2(Q.inj_rec.z0, Q.inj_rec.z1);
         B.roff[s + 1] = B.roff[s] + (size_t)Q.rec.n * Q.nt;
         B.doff[s + 1] = B.doff[s] + (size_t)Q.dat.n * Q.nt;
         B.nt = std::max(B.nt, Q.nt);
@@ -2042,19 +2202,20 @@ int daw_submodeling_transpose(const float *vp, int n1, int n2, int npml, float h
     if (int rc = make_points(src, is5, 1, G, false)) return rc;
     Band sband;
     if (int rc = make_band(sband, {&src}, G)) return rc;
-    DevBuf d_scoef, d_d, d_g, d_band, d_items;
-    if (int rc = upload_record_coef(d_scoef, src.coef)) return rc;
+    DevBuf d_scoef, d_scoefT, d_d, d_g, d_band, d_items;
+    if (int rc = upload_record_coef(d_scoef, d_scoefT, src.coef)) return rc;
     if (int rc = d_d.alloc((size_t)nr * nt * 4)) return rc;
     HIPCHK(hipMemcpy(d_d.p, d, (size_t)nr * nt * 4, hipMemcpyHostToDevice));
     if (int rc = d_g.alloc((size_t)nt * 4)) return rc;
-    if (int rc = d_band.alloc((size_t)4 * std::max(sband.nslot, 1) * G.ldx * 4)) return rc;
+    if (int rc = d_band.alloc((size_t)kBandSlots * std::max(sband.nslot, 1) * G.ldx * 4)) return rc;
     ShotDev &S = B.W.host[0];
     set_inject(S, B.geo[0]->inj_rec_c, d_d.as<float>(), nt, 0);
     S.amp_rev = 0;
     S.band_row = sband.d_row.as<int>();
-    for (int k = 0; k < 4; ++k) S.pband[k] = d_band.as<float>() + (size_t)k * std::max(sband.nslot, 1) * G.ldx;
+    S.band_stride = std::max(sband.nslot, 1) * G.ldx;
+    S.pband = d_band.as<float>();
     std::vector<RecItem> items;
-    add_records(items, src, 0, 0, 0, d_scoef, d_g.as<float>(), nt);
+    add_records(items, src, 0, 0, 0, d_scoef, d_scoefT, d_g.as<float>(), nt);
     if (int rc = upload(d_items, items)) return rc;
     if (int rc = refresh_shots(B.W)) return rc;
     if (int rc = run_adjoint_sweep(false, B.W, d_items.as<RecItem>(), (int)items.size())) return rc;
@@ -2094,6 +2255,13 @@ int daw_set_krylov_format(int fmt)
     return 0;
 }
 
+int daw_set_record_batch(int on)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.rec_batch = on != 0;
+    return 0;
+}
+
 int daw_set_minres_arith(int arith)
 {
     std::lock_guard<std::mutex> lk(g.mu);
@@ -2106,13 +2274,24 @@ int daw_set_tile(int rows, int cols)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     if (rows == 0) { g.tz_auto = true; return 0; }
+    if (cols == kMarchCols) {
+        if (!march_rows_ok(rows))
+            return fail(DAW_ERR_ARG, "column march of %d rows has no compiled kernel (8, 12, 16, 24, 32)", rows);
+        if (!g.steps_auto && g.steps == 2) return fail(DAW_ERR_ARG, "the column march advances one step per launch");
+        g.tz_auto = false;
+        g.tz = rows;
+        g.txw = 0;
+        g.march = 1;
+        return 0;
+    }
     if (cols % 64 != 0 || !tile_shape_ok(rows, cols / 64))
-        return fail(DAW_ERR_ARG, "tile %dx%d has no compiled kernel (8x128, 16x64, 16x128, 20x64, 24x64, 32x64, 32x128, 64x64)", rows, cols);
+        return fail(DAW_ERR_ARG, "tile %dx%d has no compiled kernel (8x128, 16x64, 16x128, 20x64, 24x64, 32x64, 32x128, 64x64; column march Rx58)", rows, cols);
     if (!g.steps_auto && g.steps == 2 && !tile2_shape_ok(rows, cols / 64))
         return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel (16x64, 32x64, 16x128)", rows, cols);
     g.tz_auto = false;
     g.tz = rows;
     g.txw = cols / 64;
+    g.march = 0;
     return 0;
 }
 
@@ -2121,8 +2300,8 @@ int daw_set_steps(int steps)
     std::lock_guard<std::mutex> lk(g.mu);
     if (steps == 0) { g.steps_auto = true; return 0; }
     if (steps != 1 && steps != 2) return fail(DAW_ERR_ARG, "steps per launch %d: 0 (autotuned), 1 or 2", steps);
-    if (steps == 2 && !g.tz_auto && !tile2_shape_ok(g.tz, g.txw))
-        return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel", g.tz, 64 * g.txw);
+    if (steps == 2 && !g.tz_auto && (g.march || !tile2_shape_ok(g.tz, g.txw)))
+        return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel", g.tz, g.march ? kMarchCols : 64 * g.txw);
     g.steps_auto = false;
     g.steps = steps;
     return 0;
@@ -2135,7 +2314,7 @@ int daw_get_launch(int mode, int *steps, int *rows, int *cols)
     const LaunchCfg &c = g.last_cfg[mode];
     if (steps) *steps = c.steps;
     if (rows) *rows = c.tz;
-    if (cols) *cols = 64 * c.txw;
+    if (cols) *cols = tile_cols(c);
     return 0;
 }
 
@@ -2149,7 +2328,7 @@ int daw_get_tile(int *rows, int *cols)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     if (rows) *rows = g.tz;
-    if (cols) *cols = 64 * g.txw;
+    if (cols) *cols = g.march ? kMarchCols : 64 * g.txw;
     return 0;
 }
 

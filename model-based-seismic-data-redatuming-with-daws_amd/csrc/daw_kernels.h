@@ -13,6 +13,18 @@ bool tile_shape_ok(int tz, int txw);
 // Two steps (it, it+1) per launch: tile of tz rows and 64*nc columns.
 void launch_step2(int mode, int snap, const StepParams &P, int tz, int nc, hipStream_t st);
 bool tile2_shape_ok(int tz, int nc);
+// Column march (one step per launch): wavefronts of 58 owned columns walking
+// h rows each; FP32 snapshots only.  march_rows_ok: h has a compiled kernel.
+void launch_march(int mode, const StepParams &P, int h, hipStream_t st);
+bool march_rows_ok(int h);
+constexpr int kMarchCols = 58;
+// Rows of the column march's short segments (rows with many Hicks injection terms).
+constexpr int kMarchBandRows = 2;
+// Hicks record of the items of ngroups record groups at steps it0 ..
+// it0+nsteps-1 (nsteps <= kBandSlots) from the band's time slots, after
+// those steps ran.
+void launch_rec_batch(const ShotDev *shots, const RecItem *items, const RecGroup *groups, int ngroups, int it0,
+                      int nsteps, hipStream_t st);
 // Exact adjoint of one step (reverse time); hess selects the transpose of
 // the re-injection pass (snapshots out) over that of the forward sweep.
 void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st);

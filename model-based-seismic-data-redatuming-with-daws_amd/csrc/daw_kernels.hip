@@ -112,7 +112,7 @@ __device__ __forceinline__ void record_block(const StepParams &P, int blk, float
                     q.lo1 = S->npml - R.ix1; q.hi1 = S->npml + S->n1 - 1 - R.ix1;
                     q.lo2 = S->npml - R.ix2; q.hi2 = S->npml + S->n2 - 1 - R.ix2;
                 }
-                q.band = glb(S->pband[rit & 3]);
+                q.band = glb(S->pband + (size_t)(rit & (kBandSlots - 1)) * S->band_stride);
                 q.brow = glb(S->band_row);
                 q.coef = glb(R.coef);
                 q.out = glb(R.out) + (R.reverse ? (nt - 1 - rit) : rit);
@@ -181,7 +181,7 @@ __device__ __forceinline__ View make_view(const ShotDev *__restrict__ S, int cur
 {
     View v;
     v.n1 = S->n1; v.n2 = S->n2; v.npml = S->npml; v.n1e = S->n1e; v.n2e = S->n2e; v.ldx = S->ldx;
-    v.nt = S->nt; v.ofs = S->ofs; v.par = it & 3; v.sd = S;
+    v.nt = S->nt; v.ofs = S->ofs; v.par = it & (kBandSlots - 1); v.sd = S;
     v.dt = S->dt; v.dtdx = S->dtdx;
     v.kappa = glb(S->kappa); v.d1a = glb(S->d1a); v.d1a1 = glb(S->d1a1); v.d1b = glb(S->d1b); v.bw1 = glb(S->bw1);
     v.d2a = glb(S->d2a); v.d2a1 = glb(S->d2a1); v.d2b = glb(S->d2b); v.bw2 = glb(S->bw2);
@@ -375,7 +375,7 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
         if (MODE != kBack) {
             if (S.sd->band_row != nullptr) {
                 const int slot = glb(S.sd->band_row)[gz];
-                if (slot >= 0) glb(S.sd->pband[S.par])[slot * S.ldx + gx] = p;
+                if (slot >= 0) glb(S.sd->pband + (size_t)S.par * S.sd->band_stride)[slot * S.ldx + gx] = p;
             }
             if (S.sd->wf != nullptr && interior)
                 plane_at(glb(S.sd->wf), S, it)[(gx - npml) * S.n1 + (gz - npml)] = p;
@@ -516,7 +516,7 @@ __device__ __forceinline__ void tile_interior(const View &S, float *s_vx, float 
             st(S.fpzn, g, pzv);
             if (MODE != kBack && S.sd->band_row != nullptr) {
                 const int slot = glb(S.sd->band_row)[gz];
-                if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = p;
+                if (slot >= 0) glb(S.sd->pband + (size_t)S.par * S.sd->band_stride)[slot * ldx + gx] = p;
             }
             if (MODE == kBack) snap_store_wave<SNAP>(S, S.sp, S.sp16, 0, it, gz, gx, p, true, true);
         }
@@ -998,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
                         st_nt(splane_at(S.sp, S, ic), snap_off(S, gz, gx), 0.0f);
                 } else if (S.sd->band_row != nullptr) {
                     const int slot = glb(S.sd->band_row)[gz];
-                    if (slot >= 0) glb(S.sd->pband[S.par])[slot * ldx + gx] = S.dt * kap * qpx;
+                    if (slot >= 0) glb(S.sd->pband + (size_t)S.par * S.sd->band_stride)[slot * ldx + gx] = S.dt * kap * qpx;
                 }
             }
         }
@@ -1139,7 +1139,7 @@ __device__ __forceinline__ void p_out(const View &S, int t, int gz, int gx, floa
     const int npml = S.npml;
     if (S.sd->band_row != nullptr) {
         const int slot = glb(S.sd->band_row)[gz];
-        if (slot >= 0) glb(S.sd->pband[t & 3])[slot * S.ldx + gx] = p;
+        if (slot >= 0) glb(S.sd->pband + (size_t)(t & (kBandSlots - 1)) * S.sd->band_stride)[slot * S.ldx + gx] = p;
     }
     if (S.sd->wf != nullptr && gz >= npml && gz < npml + S.n1 && gx >= npml && gx < npml + S.n2)
         plane_at(glb(S.sd->wf), S, t)[(gx - npml) * S.n1 + (gz - npml)] = p;
@@ -1387,6 +1387,508 @@ __global__ __launch_bounds__(kBlock) void k_step2(StepParams P)
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Column march: one time step of one sweep, the arithmetic of k_step (the
+// general body, operation for operation), organised for latency instead of
+// LDS.  A wavefront owns kMarchCols columns x0 .. x0+57 of H rows z0 ..
+// z0+H-1; its 64 lanes hold columns x0-3 .. x0+60, so the x-neighbours the
+// two derivative stencils need (x-2 .. x+1 for the pressure, x-1 .. x+2 for
+// the velocity) come from DPP lane shifts: the pressure is valid on lanes
+// 2..62 and the velocity on lanes 3..60.  The wavefront walks down its rows
+// once, keeping the last four old-vz rows, the last three old-vx rows and
+// the last four new-p rows in registers: iteration k loads vz of row
+// zr = z0-3+k, forms p of row zr-1 (the tile rows plus the one ring row
+// above and two below, as k_step recomputes them) and the velocities of
+// row zr-3.  No LDS, no barrier; the only cross-lane traffic is the DPP
+// shifts, which need every lane of the wavefront active.  Per-row profile
+// values (d1a, d1a1, d1b, bw1) and the injection / record-band row maps are
+// read once per wavefront into lanes and taken out with readlane per row.
+// A workgroup's tile entry holds its shot and (z0 << 16 | rows << 8 | g):
+// its four wavefronts take strips 4g .. 4g+3 of the row segment z0 ..
+// z0+rows-1 (rows <= H; the host splits rows with many Hicks injection
+// terms into short segments).  The record blocks come first as in k_step.
+// FP32 snapshots only (the 16-bit codec scales 64-aligned segments).
+// ---------------------------------------------------------------------------
+
+// value of the lane on the left (column x-1) or right (x+1); lanes at the
+// wavefront's edges receive 0
+__device__ __forceinline__ float from_left(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));   // wave_shr:1
+}
+__device__ __forceinline__ float from_right(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));   // wave_shl:1
+}
+
+// lane j of the wavefront holds v[z0 - 1 + j] (rows outside 0..n-1: fill)
+__device__ __forceinline__ float row_lane_f(const GLB float *v, int z0, int n, int lane, float fill)
+{
+    const int z = z0 - 1 + lane;
+    const float a = ld(v, (unsigned)clampi(z, 0, n - 1));
+    return (z >= 0 && z < n) ? a : fill;
+}
+__device__ __forceinline__ int row_lane_i(const GLB int *v, int z0, int n, int lane)
+{
+    const int z = z0 - 1 + lane;
+    const int a = ld(v, (unsigned)clampi(z, 0, n - 1));
+    return (z >= 0 && z < n) ? a : -1;
+}
+// row z's value out of the lanes (z - z0 + 1 in 0..63): a scalar
+__device__ __forceinline__ float lane_row_f(float v, int z, int z0)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), z - z0 + 1));
+}
+__device__ __forceinline__ int lane_row_i(int v, int z, int z0)
+{
+    return __builtin_amdgcn_readlane(v, z - z0 + 1);
+}
+// Raw buffer accesses: a resource (base address) in scalar registers, this
+// lane's byte offset (its column, computed once) and a scalar byte offset
+// (plane and row, formed on the scalar unit), so a row's load or store costs
+// no vector instruction for its address.  No range limit (0xffffffff
+// records); dword 3 is the gfx9 raw 32-bit buffer format.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc make_rsrc(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ float bld(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bst(Rsrc r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+// non-temporal (the snapshot stream; see st_nt)
+__device__ __forceinline__ void bst_nt(Rsrc r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 2);
+}
+// Non-temporal load, for the re-injection's snapshot reads: each is read
+// once, and as a streaming access it displaces less of the wave state from
+// the caches (201x801, 24 shots, 16-row march without records: 50.9 -> 47.8
+// us per step).
+__device__ __forceinline__ float bld_nt(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 2));
+}
+
+// Rows of loads in flight ahead of the row being computed.
+constexpr int kMarchAhead = 3;
+
+// Waves per SIMD the register budget must allow: short segments make many
+// wavefronts (201x801 with 24 shots: 7560 at 12 rows), tall ones few.
+// Hicks injection into one point (the reference's receiver loop, one term at a
+// time in receiver order): entries kb .. ke-1 of the row's CSR, term =
+// (c*dt * amp(r, it)) * kappa added (SIGN > 0) or subtracted.  Entries go four
+// at a time, all their loads issued before the first dependent one, so a row
+// costs two load round trips per four entries instead of two per entry; the
+// additions stay one rounded operation each, in order.
+template <int SIGN>
+__device__ __forceinline__ float inject_terms(float acc, float kap, int kb, int ke, const GLB int *rid,
+                                              const GLB float *cdt, const GLB float *amp, int stride, int it)
+{
+    int q = kb;
+    for (; q + 4 <= ke; q += 4) {
+        const int r0 = rid[q], r1 = rid[q + 1], r2 = rid[q + 2], r3 = rid[q + 3];
+        const float c0 = cdt[q], c1 = cdt[q + 1], c2 = cdt[q + 2], c3 = cdt[q + 3];
+        const float a0 = amp[r0 * stride + it], a1 = amp[r1 * stride + it];
+        const float a2 = amp[r2 * stride + it], a3 = amp[r3 * stride + it];
+        const float t0 = (c0 * a0) * kap, t1 = (c1 * a1) * kap, t2 = (c2 * a2) * kap, t3 = (c3 * a3) * kap;
+        if (SIGN > 0) {
+            acc = acc + t0; acc = acc + t1; acc = acc + t2; acc = acc + t3;
+        } else {
+            acc = acc - t0; acc = acc - t1; acc = acc - t2; acc = acc - t3;
+        }
+    }
+    for (; q < ke; ++q) {
+        const float t = (cdt[q] * amp[rid[q] * stride + it]) * kap;
+        acc = SIGN > 0 ? acc + t : acc - t;
+    }
+    return acc;
+}
+
+// Waves per SIMD the register budget must allow: short segments make many
+// wavefronts (201x801 with 24 shots: 7560 at 12 rows), tall ones few; the
+// sweeps with Hicks injection terms hold four more entries in flight.
+template <int MODE, int H>
+constexpr int march_min_waves()
+{
+    return H <= 12 ? (MODE == kReinject ? 8 : 6) : H <= 16 ? 6 : 4;
+}
+
+template <int MODE, int H>
+__global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(StepParams P)
+{
+    static_assert(H + 3 <= 64, "row profiles of a segment must fit the lanes");
+    constexpr int D = kMarchAhead;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    __shared__ __align__(16) float s_lds[kRecPerBlock * kRecStride];
+    if ((int)blockIdx.x < P.nrec_blocks) {
+        record_block(P, blockIdx.x, s_lds);
+        return;
+    }
+    const int2 t = P.tiles[blockIdx.x - P.nrec_blocks];
+    const ShotDev *__restrict__ S = P.shots + t.x;
+    const int it = P.it;
+    const int nt = S->nt;
+    if (it >= nt) return;
+    const int n1e = S->n1e, n2e = S->n2e, ldx = S->ldx, npml = S->npml, n1 = S->n1, n2 = S->n2;
+    const int nstrip = (n2e + kMarchCols - 1) / kMarchCols;
+    const int z0 = t.y >> 16, nrows = (t.y >> 8) & 255, strip = 4 * (t.y & 255) + wv;
+    if (strip >= nstrip) return;                 // whole wavefront: no lane of it takes part in a shift
+    const int x0 = strip * kMarchCols;
+    const int gx = x0 - 3 + lane;
+    const bool colok = gx >= 0 && gx < n2e;
+    const bool own = lane >= 3 && lane < 3 + kMarchCols && gx < n2e;
+    const unsigned gxc = (unsigned)clampi(gx, 0, n2e - 1);
+    const int lo = 2;
+    const int hiz = (MODE == kFwd) ? n1e - 3 : n1e - 4;
+    const int hix = (MODE == kFwd) ? n2e - 3 : n2e - 4;
+    const bool colbox = gx >= lo && gx <= hix;
+    const bool colint = gx >= npml && gx < npml + n2;
+    const float dt = S->dt, dtdx = S->dtdx, mdt = -dt;
+    const int ofs = S->ofs;
+    const int ic = nt - 1 - it;
+
+    // Buffers and byte offsets: the state (8 planes, read half px, pz, vx,
+    // vz at +0, pe, 2pe, 3pe floats, write half likewise), kappa, and the
+    // snapshot planes of the step (written at it by the back-propagation,
+    // read at nt-1-it by the re-injection; row z of the interior at
+    // (z-npml)*spitch, column gx at gx-sxb).  Lane offsets are columns.
+    const unsigned pe = (unsigned)(n1e * ldx);
+    const Rsrc rs_st = make_rsrc(S->px[0]), rs_ka = make_rsrc(S->kappa);
+    const unsigned rd = P.cur ? 16 * pe : 0u, wr = P.cur ? 0u : 16 * pe;   // bytes
+    const unsigned ldx4 = 4u * (unsigned)ldx, pe4 = 4u * pe;
+    const unsigned vcol = 4u * gxc;
+    const int xlo1 = S->xlo1, xhi1 = S->xhi1, xlo2 = S->xlo2, xhi2 = S->xhi2;
+    const int spitch = S->spitch, sxb = S->sxb;
+    const unsigned sp4 = 4u * (unsigned)spitch;
+    const bool xcol = gx >= xlo2 && gx <= xhi2;
+    const unsigned sxs = 4u * (unsigned)(clampi(gx, xlo2, xhi2) - sxb);            // extended-source read column
+    const unsigned sxi = 4u * (unsigned)(clampi(gx, npml, npml + n2 - 1) - sxb);   // interior read column
+    const unsigned sxo = 4u * (unsigned)(gx - sxb);                                // snapshot store column (interior lanes)
+    const size_t plane = (size_t)n1 * spitch;
+    const size_t tsn = (size_t)(MODE == kBack ? it : ic) * plane;
+    const Rsrc rs_sp = make_rsrc(S->sp + tsn), rs_sx = make_rsrc(S->svx + tsn), rs_sz = make_rsrc(S->svz + tsn);
+
+    // sponge profiles: one block d1a, d1a1, d1b, bw1 [n1e] then d2a, d2a1, d2b, bw2 [n2e]
+    const GLB float *const prof = glb(S->d1a);
+    const unsigned a1e = (unsigned)n1e;
+    const float c_d2a = ld(prof + 4 * a1e, gxc);
+    const float c_d2a1 = ld(prof + (4 * a1e + n2e), gxc);
+    const float c_d2b = ld(prof + (4 * a1e + 2 * n2e), gxc);
+    const float c_bw2 = ld(prof + (4 * a1e + 3 * n2e), gxc);
+    // row profiles of rows z0-1 .. z0+62 in lanes
+    const float l_d1a = row_lane_f(prof, z0, n1e, lane, 1.0f);
+    const float l_d1a1 = row_lane_f(prof + a1e, z0, n1e, lane, 1.0f);
+    const float l_d1b = row_lane_f(prof + 2 * a1e, z0, n1e, lane, 1.0f);
+    const float l_bw1 = row_lane_f(prof + 3 * a1e, z0, n1e, lane, 0.0f);
+    const GLB int *const inj_row = glb(S->inj_row);
+    const bool inj_p = MODE != kReinject && inj_row != nullptr && !S->vinj;   // Hicks injection into px
+    const bool inj_v = MODE == kFwd && inj_row != nullptr && S->vinj;         // vz source
+    const int l_inj = (inj_p || inj_v) ? row_lane_i(inj_row, z0, n1e, lane) : -1;
+    const GLB int *const band_row = glb(S->band_row);
+    const int l_band = (MODE != kBack && band_row != nullptr) ? row_lane_i(band_row, z0, n1e, lane) : -1;
+    const Rsrc rs_band = make_rsrc(S->pband + (size_t)(it & (kBandSlots - 1)) * S->band_stride);
+    GLB float *const wf = glb(S->wf);
+
+    // the rows: a full segment of H, or one of the short segments of the
+    // injection band; each a fully unrolled walk (queue slots are registers)
+    auto body = [&](auto hc) {
+        constexpr int HH = decltype(hc)::value;
+        constexpr int NK = HH + 6;                   // iterations: vz rows z0-3 .. z0+HH+2
+        // ---- the loads of iteration k (vz of row z0-3+k; vx, px, pz, kappa and
+        // the extended source of row z0-4+k; the velocity snapshots of row
+        // z0-6+k) into slot k % D, issued D iterations before they are used.
+        // No branch around a load (the wait counts stay exact), and no mask on
+        // a loaded value: rows and columns outside the grid read a clamped
+        // in-grid address, and nothing stored depends on what they hold (a point
+        // in the stencil box reads neighbours inside the grid; rows and lanes
+        // outside it are never stored).
+        float Lvz[D] = {}, Lvx[D] = {}, Lpx[D] = {}, Lpz[D] = {}, Lka[D] = {}, Lsp[D] = {}, Lsx[D] = {}, Lsz[D] = {};
+        auto issue = [&](int k) {
+            const int s = k % D;
+            const int zr = z0 - 3 + k, zp = zr - 1, zv = zr - 3;
+            Lvz[s] = bld(rs_st, vcol, rd + 3 * pe4 + (unsigned)clampi(zr, 0, n1e - 1) * ldx4);
+            if (k >= 3) {
+                const unsigned r = (unsigned)clampi(zp, 0, n1e - 1) * ldx4;
+                Lvx[s] = bld(rs_st, vcol, rd + 2 * pe4 + r);
+                Lpx[s] = bld(rs_st, vcol, rd + r);
+                Lpz[s] = bld(rs_st, vcol, rd + pe4 + r);
+                Lka[s] = bld(rs_ka, vcol, r);
+                if (MODE == kReinject) Lsp[s] = bld_nt(rs_sp, sxs, (unsigned)(clampi(zp, xlo1, xhi1) - npml) * sp4);
+            }
+            if (MODE == kReinject && k >= 6) {
+                const unsigned r = (unsigned)(clampi(zv, npml, npml + n1 - 1) - npml) * sp4;
+                Lsx[s] = bld_nt(rs_sx, sxi, r);
+                Lsz[s] = bld_nt(rs_sz, sxi, r);
+            }
+        };
+    #pragma unroll
+        for (int k = 0; k < D; ++k) issue(k);
+
+
+        const float a0 = 1.125f;
+        const float a1 = -(1.0f / 24.0f);
+        float qvz[4], qvx[3], qp[4];
+    #pragma unroll
+        for (int j = 0; j < 4; ++j) { qvz[j] = 0.0f; qp[j] = 0.0f; }
+    #pragma unroll
+        for (int j = 0; j < 3; ++j) qvx[j] = 0.0f;
+
+    #pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const int s = k % D;
+            const int zr = z0 - 3 + k;
+            // this iteration's values out of slot s, then the slot's next loads
+            const float r_vz = Lvz[s], r_vx = Lvx[s], r_px = Lpx[s], r_pz = Lpz[s], r_ka = Lka[s], r_sp = Lsp[s];
+            const float r_sx = Lsx[s], r_sz = Lsz[s];
+            if (k + D < NK) issue(k + D);
+            qvz[k & 3] = r_vz;                               // old vz of row zr
+            // ---- pressure of row zp = zr - 1 (tile rows and ring), k >= 3
+            if (k >= 3) {
+                const int zp = zr - 1;
+                float p = 0.0f;
+                if (zp < n1e) {                               // zp >= z0 - 1 >= -1; row -1 is outside
+                    const float vx = r_vx, kap = r_ka;
+                    float pxv = r_px, pzv = r_pz;
+                    // every lane shifts (inactive lanes would feed the bound value)
+                    const float vxm1 = from_left(vx), vxm2 = from_left(vxm1), vxp1 = from_right(vx);
+                    if (zp >= lo && zp <= hiz) {
+                        const float der2 = a0 * (vx - vxm1) + a1 * (vxp1 - vxm2);
+                        const float der1 = a0 * (qvz[(k - 1) & 3] - qvz[(k - 2) & 3]) + a1 * (qvz[k & 3] - qvz[(k - 3) & 3]);
+                        const float tk = dtdx * kap;
+                        const float kw2 = tk * c_d2a1;
+                        const float kw1 = tk * lane_row_f(l_d1a1, zp, z0);
+                        const float d1a = lane_row_f(l_d1a, zp, z0);
+                        float nx, nz;
+                        if (MODE == kBack) {
+                            nx = c_d2a * pxv - kw2 * der2;
+                            nz = d1a * pzv - kw1 * der1;
+                        } else {
+                            nx = c_d2a * pxv + kw2 * der2;
+                            nz = d1a * pzv + kw1 * der1;
+                        }
+                        pxv = colbox ? nx : pxv;
+                        pzv = colbox ? nz : pzv;
+                    }
+                    if (inj_p && zp >= 0) {
+                        const int injq = lane_row_i(l_inj, zp, z0);
+                        if (injq >= 0 && colok) {                     // :1042 (subtracted), :175 (added)
+                            const GLB int *ptr = glb(S->inj_ptr) + injq * (n2e + 1);
+                            pxv = inject_terms<MODE == kBack ? -1 : 1>(pxv, kap, ptr[gx], ptr[gx + 1], glb(S->inj_r),
+                                                                       glb(S->inj_cdt), glb(S->inj_amp), S->amp_stride, it);
+                        }
+                    }
+                    if (MODE == kReinject && zp >= xlo1 && zp <= xhi1)
+                        pxv = xcol ? pxv + (mdt * r_sp) * kap : pxv;   // ss*dt*sp*kappa, ss = -1 (:1123)
+                    p = pxv + pzv;
+                    if (ofs == 1 && zp <= npml) p = 0.0f;        // p(1:npml+1,:) = 0
+                    if (zp >= z0 && zp < z0 + HH && own) {
+                        const unsigned r = (unsigned)zp * ldx4;
+                        bst(rs_st, vcol, wr + r, pxv);
+                        bst(rs_st, vcol, wr + pe4 + r, pzv);
+                        if (MODE != kBack) {
+                            if (band_row != nullptr) {
+                                const int slot = lane_row_i(l_band, zp, z0);
+                                if (slot >= 0) bst(rs_band, vcol, (unsigned)slot * ldx4, p);
+                            }
+                            if (wf != nullptr && zp >= npml && zp < npml + n1 && colint)
+                                (wf + (size_t)it * ((size_t)n1 * n2))[(gx - npml) * n1 + (zp - npml)] = p;
+                        } else if (zp >= npml && zp < npml + n1 && colint) {
+                            bst_nt(rs_sp, sxo, (unsigned)(zp - npml) * sp4, p);   // sp(:,:,it) (:1060-1064)
+                        }
+                    }
+                }
+                qp[k & 3] = p;
+                qvx[k % 3] = r_vx;
+            }
+            // ---- velocities of row zv = zr - 3 (the tile rows), k >= 6
+            if (k >= 6) {
+                const int zv = zr - 3;
+                if (zv < n1e) {
+                    const bool rowi = zv >= npml && zv < npml + n1;
+                    const float p0 = qp[(k - 2) & 3];
+                    const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
+                    float vxv = qvx[(k - 2) % 3], vzv = qvz[(k - 3) & 3];
+                    if (zv >= lo && zv <= hiz) {
+                        const float der2 = a0 * (pr1 - p0) + a1 * (pr2 - pl1);
+                        const float der1 = a0 * (qp[(k - 1) & 3] - p0) + a1 * (qp[k & 3] - qp[(k - 3) & 3]);
+                        const float d1b = lane_row_f(l_d1b, zv, z0), bw1 = lane_row_f(l_bw1, zv, z0);
+                        float nx, nz;
+                        if (MODE == kBack) {
+                            nx = c_d2b * vxv - c_bw2 * der2;
+                            nz = d1b * vzv - bw1 * der1;
+                        } else {
+                            nx = c_d2b * vxv + c_bw2 * der2;
+                            nz = d1b * vzv + bw1 * der1;
+                        }
+                        vxv = colbox ? nx : vxv;
+                        vzv = colbox ? nz : vzv;
+                    }
+                    if (inj_v) {
+                        const int q = lane_row_i(l_inj, zv, z0);
+                        if (q >= 0 && colok) {
+                            const GLB int *ptr = glb(S->inj_ptr) + q * (n2e + 1);
+                            for (int m = ptr[gx]; m < ptr[gx + 1]; ++m)
+                                vzv = vzv + glb(S->inj_cdt)[m] * glb(S->inj_amp)[glb(S->inj_r)[m] * S->amp_stride + it];   // * bue (= 1), :252
+                        }
+                    }
+                    if (MODE == kReinject && rowi) {
+                        vxv = colint ? vxv + mdt * r_sx : vxv;         // * bue (= 1), :1177
+                        vzv = colint ? vzv + mdt * r_sz : vzv;         // :1184
+                    }
+                    if (own) {
+                        const unsigned r = (unsigned)zv * ldx4;
+                        bst(rs_st, vcol, wr + 2 * pe4 + r, vxv);
+                        if (!(ofs == 1 && zv == npml - 1)) {           // vz(npml,:) is written by the mirror below
+                            bst(rs_st, vcol, wr + 3 * pe4 + r, vzv);
+                            if (ofs == 1 && zv == npml) bst(rs_st, vcol, wr + 3 * pe4 + r - ldx4, vzv);   // vz(npml,:) = vz(npml+1,:)
+                        }
+                        if (MODE == kBack && rowi && colint) {         // svx, svz(:,:,it) (:1082-1087)
+                            const unsigned rs = (unsigned)(zv - npml) * sp4;
+                            bst_nt(rs_sx, sxo, rs, vxv);
+                            bst_nt(rs_sz, sxo, rs, vzv);
+                        }
+                    }
+                }
+            }
+        }
+    };
+    if (MODE != kBack || nrows == H) body(std::integral_constant<int, H>{});   // only the back-propagation injects
+    else body(std::integral_constant<int, kMarchBandRows>{});                 // receiver rows
+}
+
+// Row-segment heights of the column march the autotuner may pick.
+#define DAW_MARCH_ROWS(X) X(8) X(12) X(16) X(24) X(32)
+
+template <int MODE>
+static void launch_march_mode(const StepParams &P, int h, hipStream_t st)
+{
+    const int grid = P.ntiles + P.nrec_blocks;
+#define DAW_MARCH(R)                                                                                           \
+    if (h == R) {                                                                                              \
+        hipLaunchKernelGGL((k_march<MODE, R>), dim3(grid), dim3(kBlock), 0, st, P);                            \
+        return;                                                                                                \
+    }
+    DAW_MARCH_ROWS(DAW_MARCH)
+#undef DAW_MARCH
+}
+
+bool march_rows_ok(int h)
+{
+#define DAW_OK_MARCH(R) if (h == R) return true;
+    DAW_MARCH_ROWS(DAW_OK_MARCH)
+#undef DAW_OK_MARCH
+    return false;
+}
+
+void launch_march(int mode, const StepParams &P, int h, hipStream_t st)
+{
+    if (P.ntiles + P.nrec_blocks == 0) return;
+    if (mode == kFwd) launch_march_mode<kFwd>(P, h, st);
+    else if (mode == kBack) launch_march_mode<kBack>(P, h, st);
+    else launch_march_mode<kReinject>(P, h, st);
+}
+
+// ---------------------------------------------------------------------------
+// Batched record: the Hicks record of groups of (shot, point) items at
+// steps it0 .. it0+nsteps-1, read from the band's time slots after the steps
+// ran (a step's slot is not overwritten for kBandSlots steps).  A group is
+// up to 64 consecutive points of one point set of one shot whose windows
+// fit a kRecRows x kRecCols block of the grid; a workgroup stages that block
+// of the band for kRecSteps steps in LDS with coalesced loads, then
+// wavefront w runs step w: one lane per item, the item's 441-term sum in the
+// order of submodeling2.f90:214-219 and subwavefieldresample.f90:32-37 (i1
+// outer, i2 inner; a term outside the item's clip window contributes
+// nothing, as the record blocks' +0 does), coefficients read from the
+// transposed table (neighbouring points' coefficient k side by side).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rec_group(const ShotDev *__restrict__ shots, const RecItem *__restrict__ items,
+                                                  const RecGroup *__restrict__ groups, int it0, int nsteps)
+{
+    // rows of kRecCols + 20 floats, the block's columns from offset 10: a
+    // window column ix2-10 .. ix2+10 of any item stays inside its row
+    constexpr int RW = kRecCols + 20;
+    __shared__ float sb[kRecSteps][kRecRows][RW];
+    __shared__ int srow[kRecRows];
+    const RecGroup G = groups[blockIdx.x];
+    const int j0 = (int)blockIdx.y * kRecSteps;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+    const ShotDev *__restrict__ S = shots + items[G.first].shot;
+    const int nt = S->nt, ldx = S->ldx;
+    if (tid < G.nz) srow[tid] = glb(S->band_row)[G.z0 + tid];
+    __syncthreads();
+    // ---- stage: wavefront w takes rows (step j, row r) = w, w+4, ... of the
+    // block, its lanes the columns; a row's loads are in flight together
+    const GLB float *band = glb(S->pband);
+    for (int jr = w; jr < kRecSteps * G.nz; jr += 4) {
+        const int j = jr / G.nz, r = jr - j * G.nz;             // uniform
+        const int it = it0 + j0 + j, slot = srow[r];
+        const bool live = j0 + j < nsteps && it < nt && slot >= 0;
+        const GLB float *src = band + (live ? (size_t)(it & (kBandSlots - 1)) * S->band_stride + (size_t)slot * ldx + G.x0 : 0);
+        float v[(kRecCols + 63) / 64];
+#pragma unroll
+        for (int m = 0; m < (kRecCols + 63) / 64; ++m) {
+            const int c = l + 64 * m;
+            v[m] = src[c < G.nx && live ? c : 0];
+        }
+#pragma unroll
+        for (int m = 0; m < (kRecCols + 63) / 64; ++m) {
+            const int c = l + 64 * m;
+            if (c < G.nx) sb[j][r][10 + c] = live ? v[m] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int it = it0 + j0 + w;
+    if (l >= G.count || j0 + w >= nsteps || it >= nt) return;
+    const RecItem R = items[G.first + l];
+    int lo1, hi1, lo2, hi2;
+    if (R.clip == 0) {
+        lo1 = -R.ix1; hi1 = S->n1e - 1 - R.ix1;
+        lo2 = -R.ix2; hi2 = S->n2e - 1 - R.ix2;
+    } else {
+        lo1 = S->npml - R.ix1; hi1 = S->npml + S->n1 - 1 - R.ix1;
+        lo2 = S->npml - R.ix2; hi2 = S->npml + S->n2 - 1 - R.ix2;
+    }
+    // columns of the window inside the clip range: bit q for i2 = q - 10
+    unsigned m2 = 0;
+#pragma unroll
+    for (int q = 0; q < 21; ++q) m2 |= (q - 10 >= lo2 && q - 10 <= hi2) ? (1u << q) : 0u;
+    const GLB float *ct = glb(R.coefT);
+    const int cs = R.cstride;
+    const int cbase = R.ix2 - 10 - G.x0 + 10;                 // column of i2 = -10 in a staged row
+    float acc = 0.0f;
+#pragma unroll 3
+    for (int a = 0; a < 21; ++a) {
+        const int i1 = a - 10;
+        const unsigned mr = (i1 >= lo1 && i1 <= hi1) ? m2 : 0u;
+        const float *row = &sb[w][clampi(R.ix1 + i1 - G.z0, 0, kRecRows - 1)][cbase];
+        float c[21];
+#pragma unroll
+        for (int q = 0; q < 21; ++q) c[q] = ct[(a * 21 + q) * cs];
+        // a term outside the window becomes +0 (bits cleared), which adds
+        // nothing: acc is never -0; no branch, so every read is in flight
+#pragma unroll
+        for (int q = 0; q < 21; ++q) {
+            const unsigned keep = 0u - ((mr >> q) & 1u);
+            acc = acc + __uint_as_float(__float_as_uint(c[q] * row[q]) & keep);
+        }
+    }
+    R.out[R.reverse ? nt - 1 - it : it] = acc;
+}
+
+void launch_rec_batch(const ShotDev *shots, const RecItem *items, const RecGroup *groups, int ngroups, int it0,
+                      int nsteps, hipStream_t st)
+{
+    if (ngroups <= 0 || nsteps <= 0) return;
+    hipLaunchKernelGGL(k_rec_group, dim3(ngroups, (nsteps + kRecSteps - 1) / kRecSteps), dim3(256), 0, st, shots,
+                       items, groups, it0, nsteps);
 }
 
 // Tile shapes the autotuner may pick: (rows, wavefronts across x).
