@@ -52,8 +52,9 @@ struct ShotDev {
     const int *inj_r;
     const float *inj_cdt;
     int inj_z0, inj_z1;                 // first and last row with inj_row[z] >= 0 (z1 < z0: none)
-    const float *inj_amp;               // amplitude of receiver r at step it: inj_amp[r*amp_stride + it]
+    const float *inj_amp;               // amplitude of receiver r at step it: inj_amp[r*amp_stride + it*amp_tstride]
     int amp_stride;
+    int amp_tstride;                    // 1 for [r][t] traces; the receiver count for a [t][r] (transposed) copy
     int amp_rev;                        // amplitude time index nt-1-it (adjoint of a time-reversed input)
     int vinj;                           // 1: the source is a vz source (type_src = 1), applied in the velocity phase
     // record band: band_row[z] = slot >= 0 for rows some record window touches

@@ -529,17 +529,22 @@ static int finalize_sweepset(SweepSet &W)
             // per shot: row segments (the back-propagation's receiver rows in
             // short ones) times groups of four strips, strips fastest; entry
             // (shot, z0 << 16 | rows << 8 | group)
+            // The short segments of the injection band launch first within
+            // each XCD's run: their wavefronts carry the receiver terms'
+            // dependent loads, the longest chains of the sweep.
             std::vector<std::vector<int>> ys(W.nshots);
             for (int s = 0; s < W.nshots; ++s) {
                 const ShotDev &S = W.host[s];
                 const int2 b = order == kBack && s < (int)W.inj_band.size() ? W.inj_band[s] : make_int2(0, -1);
                 const int groups = ((S.n2e + kMarchCols - 1) / kMarchCols + 3) / 4;
                 for (const int2 &sg : march_segments(S.n1e, c.tz, b.x, b.y))
-                    for (int q = 0; q < groups; ++q) ys[s].push_back((sg.x << 16) | (sg.y << 8) | q);
+                    for (int q = 0; q < groups; ++q) {
+                        ys[s].push_back((sg.x << 16) | (sg.y << 8) | q);
+                        cls[s].push_back(sg.y == c.tz ? 0 : 1);
+                    }
                 tps[s] = (int)ys[s].size();
-                cls[s].assign(tps[s], 0);
             }
-            std::vector<int2> tl = make_tiles(tps, cls, kTileRaster);
+            std::vector<int2> tl = make_tiles(tps, cls, kTileCostFirst);
             for (int2 &e : tl) e.y = ys[e.x][e.y];
             TileList &T = W.tiles[key];
             T.n = (int)tl.size();
@@ -816,7 +821,9 @@ static void fill_shot(ShotDev &S, const Grid &G, const DevModel &M, ShotStore &s
     S.xlo2 = G.npml; S.xhi2 = G.npml + G.n2 - 1;
 }
 
-static void set_inject(ShotDev &S, const Inject &I, const float *amp, int amp_stride, int vinj)
+// amp[r*amp_stride + it*amp_tstride]: amp_tstride 1 for [r][t] traces, the
+// receiver count for a transposed [t][r] copy
+static void set_inject(ShotDev &S, const Inject &I, const float *amp, int amp_stride, int vinj, int amp_tstride = 1)
 {
     S.inj_row = I.d_row.as<int>();
     S.inj_z0 = I.z0;
@@ -826,6 +833,7 @@ static void set_inject(ShotDev &S, const Inject &I, const float *amp, int amp_st
     S.inj_cdt = I.d_cdt.as<float>();
     S.inj_amp = amp;
     S.amp_stride = amp_stride;
+    S.amp_tstride = amp_tstride;
     S.vinj = vinj;
 }
 
@@ -897,6 +905,8 @@ struct Geo {
 
 struct Batch {
     int nshots = 0, nt = 0;         // nt: the longest shot
+    DevBuf ampT;                    // [t][r] copies of the injected traces (roff layout)
+    int max_nrec = 1;
     std::vector<std::unique_ptr<Geo>> geo;
     std::vector<int> gi;            // geometry of each shot
     std::vector<Inject> inj_src;    // one per shot
@@ -945,11 +955,14 @@ static int autotune_tiles(Batch &B)
                                g.tz_auto ? 0 : g.tz, g.tz_auto ? 0 : g.txw, g.tz_auto ? 0 : g.march, W.snap, g.rec_batch};
     auto hit = g.tz_cache.find(key);
     if (hit == g.tz_cache.end()) {
+        // zero amplitudes laid out as the real sweeps read them (the
+        // back-propagation's [t][r] copy): the probes carry the gathers' cost
         DevBuf zeros;
-        if (int rc = zeros.alloc((size_t)B.nt * sizeof(float))) return rc;
-        launch_fill(zeros.as<float>(), B.nt, 0.0f, g.stream);
+        const size_t nz = (size_t)B.nt * B.max_nrec;
+        if (int rc = zeros.alloc(nz * sizeof(float))) return rc;
+        launch_fill(zeros.as<float>(), nz, 0.0f, g.stream);
         for (int s = 0; s < B.nshots; ++s) {
-            set_inject(W.host[s], B.geo_of(s).inj_rec, zeros.as<float>(), 0, 0);
+            set_inject(W.host[s], B.geo_of(s).inj_rec, zeros.as<float>(), 1, 0, B.geo_of(s).rec.n);
             W.host[s].amp_rev = 0;
         }
         if (int rc = refresh_shots(W)) return rc;
@@ -1082,6 +1095,8 @@ static int hessian(Batch &B, const float *amp, const HessOut &o)
 {
     SweepSet &W = B.W;
     const bool exact = g.hess_exact;
+    if (!exact && B.ampT.bytes < B.roff[B.nshots] * sizeof(float))
+        if (int rc = B.ampT.alloc(B.roff[B.nshots] * sizeof(float))) return rc;
     for (int s = 0; s < B.nshots; ++s) {
         ShotDev &S = W.host[s];
         const Geo &Q = B.geo_of(s);
@@ -1093,7 +1108,10 @@ static int hessian(Batch &B, const float *amp, const HessOut &o)
             S.amp_rev = 1;
             set_range(S, Q.G, o.variant);
         } else {
-            set_inject(S, Q.inj_rec, amp + B.roff[s], Q.nt, 0);
+            // a [t][r] copy of the shot's traces: the receivers' amplitudes
+            // of one step side by side for the injection's gathers
+            launch_transpose(amp + B.roff[s], B.ampT.as<float>() + B.roff[s], Q.rec.n, Q.nt, g.stream);
+            set_inject(S, Q.inj_rec, B.ampT.as<float>() + B.roff[s], 1, 0, Q.rec.n);
             S.amp_rev = 0;
         }
     }
@@ -1190,15 +1208,20 @@ static int finish_batch(Batch &B, bool want_snap)
     B.roff.assign(B.nshots + 1, 0);
     B.doff.assign(B.nshots + 1, 0);
     B.nt = 0;
+    int max_nrec = 1;
     for (int s = 0; s < B.nshots; ++s) {
         const Geo &Q = B.geo_of(s);
         if (int rc = alloc_store(B.store[s], 1, Q.G, Q.band.nslot, Q.nt, fmt)) return rc;
         fill_shot(B.W.host[s], Q.G, *Q.M, B.store[s], 0, Q.nt, Q.band.nslot, &Q.band);
         B.W.inj_band[s] = make_int2(Q.inj_rec.z0, Q.inj_rec.z1);
         B.roff[s + 1] = B.roff[s] + (size_t)Q.rec.n * Q.nt;
+        max_nrec = std::max(max_nrec, Q.rec.n);
         B.doff[s + 1] = B.doff[s] + (size_t)Q.dat.n * Q.nt;
         B.nt = std::max(B.nt, Q.nt);
     }
+    if (want_snap && B.roff[B.nshots] > 0)
+        if (int rc = B.ampT.alloc(B.roff[B.nshots] * sizeof(float))) return rc;
+    B.max_nrec = max_nrec;
     return finalize_sweepset(B.W);
 }
 

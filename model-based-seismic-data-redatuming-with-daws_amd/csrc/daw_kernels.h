@@ -25,6 +25,8 @@ constexpr int kMarchBandRows = 2;
 // those steps ran.
 void launch_rec_batch(const ShotDev *shots, const RecItem *items, const RecGroup *groups, int ngroups, int it0,
                       int nsteps, hipStream_t st);
+// out[t][r] = in[r][t], r < nr, t < nt
+void launch_transpose(const float *in, float *out, int nr, int nt, hipStream_t st);
 // Exact adjoint of one step (reverse time); hess selects the transpose of
 // the re-injection pass (snapshots out) over that of the forward sweep.
 void launch_adjoint(bool hess, const StepParams &P, int tz, int txw, hipStream_t st);

@@ -355,7 +355,7 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
         const GLB int *ptr = glb(S.sd->inj_ptr) + injq * (S.n2e + 1);
         const int kb = ptr[gx], ke = ptr[gx + 1];
         for (int k = kb; k < ke; ++k) {
-            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it]) * kap;
+            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it * S.sd->amp_tstride]) * kap;
             if (MODE == kBack) pxv = pxv - term;   // :1042
             else pxv = pxv + term;                 // :175
         }
@@ -848,7 +848,7 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<MODE, TZ, TXW, SNAP>())) vo
                 if (q >= 0) {
                     const GLB int *ptr = glb(S.sd->inj_ptr) + q * (n2e + 1);
                     for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
-                        vzv = vzv + glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it];   // * bue (= 1), :252
+                        vzv = vzv + glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it * S.sd->amp_tstride];   // * bue (= 1), :252
                 }
             }
             if (MODE == kReinject && interior) {
@@ -978,7 +978,7 @@ __global__ __launch_bounds__(kBlock) void k_adj(StepParams P)
                     const GLB int *ptr = glb(S.sd->inj_ptr) + q * (n2e + 1);
                     const int ia = S.sd->amp_rev ? S.nt - 1 - it : it;
                     for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
-                        qp += glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + ia];
+                        qp += glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + ia * S.sd->amp_tstride];
                 }
             }
             if (fs && gz <= npml) qp = 0.0f;          // C^T: M
@@ -1088,7 +1088,7 @@ __device__ __forceinline__ float p_half(const View &S, int t, int gz, int gx, fl
         const GLB int *ptr = glb(S.sd->inj_ptr) + injq * (S.n2e + 1);
         const int kb = ptr[gx], ke = ptr[gx + 1];
         for (int k = kb; k < ke; ++k) {
-            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + t]) * kap;
+            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + t * S.sd->amp_tstride]) * kap;
             if (MODE == kBack) pxv = pxv - term;   // :1042
             else pxv = pxv + term;                 // :175
         }
@@ -1122,7 +1122,7 @@ __device__ __forceinline__ void v_half(const View &S, int t, int gz, int gx, flo
         if (q >= 0) {
             const GLB int *ptr = glb(S.sd->inj_ptr) + q * (S.n2e + 1);
             for (int k = ptr[gx]; k < ptr[gx + 1]; ++k)
-                vzv = vzv + glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + t];   // :252
+                vzv = vzv + glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + t * S.sd->amp_tstride];   // :252
         }
     }
     if (MODE == kReinject && interior) {
@@ -1490,14 +1490,14 @@ constexpr int kMarchAhead = 3;
 // additions stay one rounded operation each, in order.
 template <int SIGN>
 __device__ __forceinline__ float inject_terms(float acc, float kap, int kb, int ke, const GLB int *rid,
-                                              const GLB float *cdt, const GLB float *amp, int stride, int it)
+                                              const GLB float *cdt, const GLB float *amp, int stride, int toff)
 {
     int q = kb;
     for (; q + 4 <= ke; q += 4) {
         const int r0 = rid[q], r1 = rid[q + 1], r2 = rid[q + 2], r3 = rid[q + 3];
         const float c0 = cdt[q], c1 = cdt[q + 1], c2 = cdt[q + 2], c3 = cdt[q + 3];
-        const float a0 = amp[r0 * stride + it], a1 = amp[r1 * stride + it];
-        const float a2 = amp[r2 * stride + it], a3 = amp[r3 * stride + it];
+        const float a0 = amp[r0 * stride + toff], a1 = amp[r1 * stride + toff];
+        const float a2 = amp[r2 * stride + toff], a3 = amp[r3 * stride + toff];
         const float t0 = (c0 * a0) * kap, t1 = (c1 * a1) * kap, t2 = (c2 * a2) * kap, t3 = (c3 * a3) * kap;
         if (SIGN > 0) {
             acc = acc + t0; acc = acc + t1; acc = acc + t2; acc = acc + t3;
@@ -1506,7 +1506,7 @@ __device__ __forceinline__ float inject_terms(float acc, float kap, int kb, int 
         }
     }
     for (; q < ke; ++q) {
-        const float t = (cdt[q] * amp[rid[q] * stride + it]) * kap;
+        const float t = (cdt[q] * amp[rid[q] * stride + toff]) * kap;
         acc = SIGN > 0 ? acc + t : acc - t;
     }
     return acc;
@@ -1682,7 +1682,8 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         if (injq >= 0 && colok) {                     // :1042 (subtracted), :175 (added)
                             const GLB int *ptr = glb(S->inj_ptr) + injq * (n2e + 1);
                             pxv = inject_terms<MODE == kBack ? -1 : 1>(pxv, kap, ptr[gx], ptr[gx + 1], glb(S->inj_r),
-                                                                       glb(S->inj_cdt), glb(S->inj_amp), S->amp_stride, it);
+                                                                       glb(S->inj_cdt), glb(S->inj_amp), S->amp_stride,
+                                                                       it * S->amp_tstride);
                         }
                     }
                     if (MODE == kReinject && zp >= xlo1 && zp <= xhi1)
@@ -1736,7 +1737,7 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         if (q >= 0 && colok) {
                             const GLB int *ptr = glb(S->inj_ptr) + q * (n2e + 1);
                             for (int m = ptr[gx]; m < ptr[gx + 1]; ++m)
-                                vzv = vzv + glb(S->inj_cdt)[m] * glb(S->inj_amp)[glb(S->inj_r)[m] * S->amp_stride + it];   // * bue (= 1), :252
+                                vzv = vzv + glb(S->inj_cdt)[m] * glb(S->inj_amp)[glb(S->inj_r)[m] * S->amp_stride + it * S->amp_tstride];   // * bue (= 1), :252
                         }
                     }
                     if (MODE == kReinject && rowi) {
@@ -1889,6 +1890,33 @@ void launch_rec_batch(const ShotDev *shots, const RecItem *items, const RecGroup
     if (ngroups <= 0 || nsteps <= 0) return;
     hipLaunchKernelGGL(k_rec_group, dim3(ngroups, (nsteps + kRecSteps - 1) / kRecSteps), dim3(256), 0, st, shots,
                        items, groups, it0, nsteps);
+}
+
+// ---------------------------------------------------------------------------
+// Transposed copy of traces: out[t][r] = in[r][t] for r < nr, t < nt (32x32
+// tiles through LDS, both sides coalesced).  The back-propagation injects
+// from the copy, so the receivers' amplitudes of one step are contiguous.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ in, float *__restrict__ out, int nr, int nt)
+{
+    __shared__ float tile[32][33];
+    const int t0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, t = t0 + tx;
+        if (r < nr && t < nt) tile[k][tx] = in[(size_t)r * nt + t];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int t = t0 + k, r = r0 + tx;
+        if (r < nr && t < nt) out[(size_t)t * nr + r] = tile[tx][k];
+    }
+}
+
+void launch_transpose(const float *in, float *out, int nr, int nt, hipStream_t st)
+{
+    if (nr <= 0 || nt <= 0) return;
+    hipLaunchKernelGGL(k_transpose, dim3((nt + 31) / 32, (nr + 31) / 32), dim3(256), 0, st, in, out, nr, nt);
 }
 
 // Tile shapes the autotuner may pick: (rows, wavefronts across x).
