@@ -657,15 +657,21 @@ def main():
     full = [p for p in per_step if p[0] == bsz] or per_step
     shots_per_launch = full[0][0]
     st = {"sweep_ms": [sum(p[1]["sweep_ms"][m] for p in full) for m in range(3)],
-          "sweep_steps": [sum(p[1]["sweep_steps"][m] for p in full) for m in range(3)]}
+          "sweep_steps": [sum(p[1]["sweep_steps"][m] for p in full) for m in range(3)],
+          "record_ms": [sum(p[1]["record_ms"][m] for p in full) for m in range(3)]}
+    # the step kernels' own time: the sweeps' events less the batched record
+    # launches they contain (their own events)
+    step_ms = [st["sweep_ms"][m] - st["record_ms"][m] for m in range(3)]
     launch = {m: tuple(full[-1][2][i]) for i, m in enumerate(("forward", "back-propagation", "re-injection"))}
-    # dominant kernel: back-propagation or re-injection sweep; one launch
-    # advances all shots of a batch by launch_steps time steps
-    dom = max((1, 2), key=lambda m: st["sweep_ms"][m])
-    launch_steps = launch[("forward", "back-propagation", "re-injection")[dom]][0]
-    kname = "k_step2" if launch_steps == 2 else "k_step"
+    # dominant kernel: that of the back-propagation or the re-injection sweep;
+    # one launch advances all shots of a batch by launch_steps time steps
+    dom = max((1, 2), key=lambda m: step_ms[m])
+    dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
+    launch_steps = dcfg[0]
+    march = dcfg[2] == 58
+    kname = "k_march" if march else "k_step2" if launch_steps == 2 else "k_step"
     modes = {1: f"{kname}<1> (back-propagation)", 2: f"{kname}<2> (re-injection)"}
-    avg_ms = st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1) * launch_steps   # per launch
+    avg_ms = step_ms[dom] / max(st["sweep_steps"][dom], 1) * launch_steps   # per launch
     # algorithmic bytes of one launch per shot: the wave state read and written
     # once (px, pz, vx, vz: 32 B) and kappa read (4 B) per extended point, plus
     # each step's snapshots of the interior (3 fields, 4 B each, or 2 B each +
@@ -674,9 +680,9 @@ def main():
     bytes_per_shot_launch = 36.0 * n1e * n2e + launch_steps * snap_b * w.nz * w.nx
     alg_bytes = bytes_per_shot_launch * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
     snap_code = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snapshots]
-    targs = (f"{dom},{snap_code},{dcfg[1]},{dcfg[2] // 64}" if launch_steps == 2     # k_step2<MODE, SNAP, TZ, NC>
+    targs = (f"{dom},{dcfg[1]}" if march                                                # k_march<MODE, ROWS>
+             else f"{dom},{snap_code},{dcfg[1]},{dcfg[2] // 64}" if launch_steps == 2     # k_step2<MODE, SNAP, TZ, NC>
              else f"{dom},{dcfg[1]},{dcfg[2] // 64},{snap_code}")                     # k_step<MODE, TZ, TXW, SNAP>
     traffic_key = f"{kname}<{targs}> {w.nz}x{w.nx} {int(shots_per_launch)} shots per launch"
     traffic = measured_traffic(traffic_key)
@@ -760,6 +766,8 @@ def main():
         "gpts_per_s": gpts,
         "sweep_us_per_step": {k: st["sweep_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
                               for i, k in enumerate(("forward", "back-propagation", "re-injection"))},
+        "record_us_per_step": {k: st["record_ms"][i] * 1e3 / max(st["sweep_steps"][i], 1)
+                               for i, k in enumerate(("forward", "back-propagation", "re-injection"))},
         "sweep_us_per_step_shots": shots_per_launch,
         "plain_shots_per_hour": plain,
         "batched_over_plain": (shots_per_hour / plain) if plain else None,
@@ -770,7 +778,9 @@ def main():
                      "frac": achieved / 8000.0, "traffic": traffic, "traffic_key": traffic_key,
                      "kernel": modes[dom],
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes,
-                     "steps_per_launch": launch_steps},
+                     "steps_per_launch": launch_steps,
+                     "sweep_frac": alg_bytes / (st["sweep_ms"][dom] / max(st["sweep_steps"][dom], 1) * launch_steps
+                                                * 1e-3) / 1e9 / 8000.0},
         "cpu_baseline": cpu,
         "wall_s": round(time.perf_counter() - t_start, 1),
     }

@@ -325,6 +325,7 @@ typedef struct daw_stats {
     long long point_updates;      /* sum over executed sweeps of n1e*n2e per shot per step */
     long long step_launches;
     int max_batch_used;
+    double record_ms[3];          /* of sweep_ms: the batched record launches (daw_set_record_batch(1)) */
 } daw_stats;
 
 int daw_set_timing(int on);

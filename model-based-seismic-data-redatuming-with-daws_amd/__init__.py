@@ -103,6 +103,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("sweep_ms", C.c_double * 3), ("sweep_steps", C.c_longlong * 3), ("batch_ms", C.c_double),
         ("point_updates", C.c_longlong), ("step_launches", C.c_longlong), ("max_batch_used", C.c_int),
+        ("record_ms", C.c_double * 3),
     ]
 
 
@@ -685,4 +686,5 @@ def stats():
     st = Stats()
     _check(lib().daw_get_stats(C.byref(st)))
     return dict(sweep_ms=list(st.sweep_ms), sweep_steps=list(st.sweep_steps), batch_ms=st.batch_ms,
-                point_updates=st.point_updates, step_launches=st.step_launches, max_batch_used=st.max_batch_used)
+                point_updates=st.point_updates, step_launches=st.step_launches, max_batch_used=st.max_batch_used,
+                record_ms=list(st.record_ms))

@@ -32,7 +32,7 @@ constexpr int kRecStride = 444;   // 441 window products padded to whole float4s
 constexpr int kBlock = 256;
 // Time slots of the record band (slot of step it: it & (kBandSlots-1)): a
 // step's band is recorded within kBandSlots steps of being written.
-constexpr int kBandSlots = 8;
+constexpr int kBandSlots = 16;
 
 struct ShotDev {
     // grid

@@ -66,6 +66,8 @@ struct Global {
     bool timing = false;
     double t_ms[3] = {0, 0, 0};
     long long t_cnt[3] = {0, 0, 0};
+    double t_rec_ms[3] = {0, 0, 0};   // the batched record launches within those sweeps
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> rec_ev;   // events around them (timing on)
     double sweep_ms = 0.0;
     long long pts = 0;         // point-updates executed (sum n1e*n2e per shot per step)
     long long launches = 0;
@@ -619,8 +621,18 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
             cur ^= 1;
             const int done = it + n;
             if (nrec > 0 && (done == nsteps || done - rec_from + per > kBandSlots)) {
+                std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+                if (g.timing && !g.graph && hipEventCreate(&ev.first) == hipSuccess &&
+                    hipEventCreate(&ev.second) == hipSuccess)
+                    (void)hipEventRecord(ev.first, g.stream);
                 launch_rec_batch(P.shots, d_rec, W.d_groups.as<RecGroup>(), (int)W.groups.size(), rec_from,
                                  done - rec_from, g.stream);
+                if (ev.second) {
+                    (void)hipEventRecord(ev.second, g.stream);
+                    g.rec_ev.push_back(ev);
+                } else if (ev.first) {
+                    (void)hipEventDestroy(ev.first);
+                }
                 rec_from = done;
             }
         }
@@ -699,6 +711,13 @@ static int run_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, int 
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         g.t_ms[mode] += ms;
         g.t_cnt[mode] += nsteps;
+        for (auto &ev : g.rec_ev) {
+            float r = 0;
+            if (hipEventElapsedTime(&r, ev.first, ev.second) == hipSuccess) g.t_rec_ms[mode] += r;
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+        g.rec_ev.clear();
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
@@ -2365,7 +2384,7 @@ int daw_set_timing(int on)
 int daw_reset_stats(void)
 {
     std::lock_guard<std::mutex> lk(g.mu);
-    for (int i = 0; i < 3; ++i) { g.t_ms[i] = 0; g.t_cnt[i] = 0; }
+    for (int i = 0; i < 3; ++i) { g.t_ms[i] = 0; g.t_cnt[i] = 0; g.t_rec_ms[i] = 0; }
     g.sweep_ms = 0;
     g.pts = 0;
     g.launches = 0;
@@ -2376,7 +2395,11 @@ int daw_get_stats(daw_stats *st)
 {
     if (!st) return fail(DAW_ERR_ARG, "null stats");
     std::lock_guard<std::mutex> lk(g.mu);
-    for (int i = 0; i < 3; ++i) { st->sweep_ms[i] = g.t_ms[i]; st->sweep_steps[i] = g.t_cnt[i]; }
+    for (int i = 0; i < 3; ++i) {
+        st->sweep_ms[i] = g.t_ms[i];
+        st->sweep_steps[i] = g.t_cnt[i];
+        st->record_ms[i] = g.t_rec_ms[i];
+    }
     st->batch_ms = g.sweep_ms;
     st->point_updates = g.pts;
     st->step_launches = g.launches;

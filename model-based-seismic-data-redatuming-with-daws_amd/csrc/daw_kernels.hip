@@ -329,6 +329,39 @@ __device__ __forceinline__ void snap_store_wave(const View &S, GLB float *f32, G
     if (row_interior && (threadIdx.x & 63) == 0 && (gx >> 6) < S.nseg) st(scale_at(S, t, field), scale_off(S, gz, gx), sc);
 }
 
+// Hicks injection into one point (the reference's receiver loop, one term at a
+// time in receiver order): entries kb .. ke-1 of the row's CSR, term =
+// (c*dt * amp(r, it)) * kappa added (SIGN > 0) or subtracted.  Entries go four
+// at a time, all their loads issued before the first dependent one, so a row
+// costs two load round trips per four entries instead of two per entry; the
+// additions stay one rounded operation each, in order.
+template <int SIGN>
+__device__ __forceinline__ float inject_terms(float acc, float kap, int kb, int ke, const GLB int *rid,
+                                              const GLB float *cdt, const GLB float *amp, int stride, int toff)
+{
+    int q = kb;
+    for (; q + 4 <= ke; q += 4) {
+        const int r0 = rid[q], r1 = rid[q + 1], r2 = rid[q + 2], r3 = rid[q + 3];
+        const float c0 = cdt[q], c1 = cdt[q + 1], c2 = cdt[q + 2], c3 = cdt[q + 3];
+        const float a0 = amp[r0 * stride + toff], a1 = amp[r1 * stride + toff];
+        const float a2 = amp[r2 * stride + toff], a3 = amp[r3 * stride + toff];
+        const float t0 = (c0 * a0) * kap, t1 = (c1 * a1) * kap, t2 = (c2 * a2) * kap, t3 = (c3 * a3) * kap;
+        if (SIGN > 0) {
+            acc = acc + t0; acc = acc + t1; acc = acc + t2; acc = acc + t3;
+        } else {
+            acc = acc - t0; acc = acc - t1; acc = acc - t2; acc = acc - t3;
+        }
+    }
+    for (; q < ke; ++q) {
+        const float t = (cdt[q] * amp[rid[q] * stride + toff]) * kap;
+        acc = SIGN > 0 ? acc + t : acc - t;
+    }
+    return acc;
+}
+
+// Waves per SIMD the register budget must allow: short segments make many
+// wavefronts (201x801 with 24 shots: 7560 at 12 rows), tall ones few; the
+// sweeps with Hicks injection terms hold four more entries in flight.
 template <int MODE, int VZW_>
 __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, int gx, float pxv, float pzv,
                                          float kap, const float *vr, const float *vc, float d1a, float d1a1,
@@ -351,14 +384,10 @@ __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, 
             pzv = d1a * pzv + kw1 * der1;
         }
     }
-    if (MODE != kReinject && injq >= 0) {
+    if (MODE != kReinject && injq >= 0) {                // :1042 (subtracted), :175 (added)
         const GLB int *ptr = glb(S.sd->inj_ptr) + injq * (S.n2e + 1);
-        const int kb = ptr[gx], ke = ptr[gx + 1];
-        for (int k = kb; k < ke; ++k) {
-            const float term = (glb(S.sd->inj_cdt)[k] * glb(S.sd->inj_amp)[glb(S.sd->inj_r)[k] * S.sd->amp_stride + it * S.sd->amp_tstride]) * kap;
-            if (MODE == kBack) pxv = pxv - term;   // :1042
-            else pxv = pxv + term;                 // :175
-        }
+        pxv = inject_terms<MODE == kBack ? -1 : 1>(pxv, kap, ptr[gx], ptr[gx + 1], glb(S.sd->inj_r), glb(S.sd->inj_cdt),
+                                                  glb(S.sd->inj_amp), S.sd->amp_stride, it * S.sd->amp_tstride);
     }
     if (MODE == kReinject) {
         if (gz >= S.sd->xlo1 && gz <= S.sd->xhi1 && gx >= S.sd->xlo2 && gx <= S.sd->xhi2)
@@ -1477,44 +1506,12 @@ __device__ __forceinline__ float bld_nt(Rsrc r, unsigned voff, unsigned soff)
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 2));
 }
 
-// Rows of loads in flight ahead of the row being computed.
+// Rows of loads in flight ahead of the row being computed (5 measured no
+// faster than 3 for 24- and 32-row segments at 201x801, 24 shots).
 constexpr int kMarchAhead = 3;
 
 // Waves per SIMD the register budget must allow: short segments make many
 // wavefronts (201x801 with 24 shots: 7560 at 12 rows), tall ones few.
-// Hicks injection into one point (the reference's receiver loop, one term at a
-// time in receiver order): entries kb .. ke-1 of the row's CSR, term =
-// (c*dt * amp(r, it)) * kappa added (SIGN > 0) or subtracted.  Entries go four
-// at a time, all their loads issued before the first dependent one, so a row
-// costs two load round trips per four entries instead of two per entry; the
-// additions stay one rounded operation each, in order.
-template <int SIGN>
-__device__ __forceinline__ float inject_terms(float acc, float kap, int kb, int ke, const GLB int *rid,
-                                              const GLB float *cdt, const GLB float *amp, int stride, int toff)
-{
-    int q = kb;
-    for (; q + 4 <= ke; q += 4) {
-        const int r0 = rid[q], r1 = rid[q + 1], r2 = rid[q + 2], r3 = rid[q + 3];
-        const float c0 = cdt[q], c1 = cdt[q + 1], c2 = cdt[q + 2], c3 = cdt[q + 3];
-        const float a0 = amp[r0 * stride + toff], a1 = amp[r1 * stride + toff];
-        const float a2 = amp[r2 * stride + toff], a3 = amp[r3 * stride + toff];
-        const float t0 = (c0 * a0) * kap, t1 = (c1 * a1) * kap, t2 = (c2 * a2) * kap, t3 = (c3 * a3) * kap;
-        if (SIGN > 0) {
-            acc = acc + t0; acc = acc + t1; acc = acc + t2; acc = acc + t3;
-        } else {
-            acc = acc - t0; acc = acc - t1; acc = acc - t2; acc = acc - t3;
-        }
-    }
-    for (; q < ke; ++q) {
-        const float t = (cdt[q] * amp[rid[q] * stride + toff]) * kap;
-        acc = SIGN > 0 ? acc + t : acc - t;
-    }
-    return acc;
-}
-
-// Waves per SIMD the register budget must allow: short segments make many
-// wavefronts (201x801 with 24 shots: 7560 at 12 rows), tall ones few; the
-// sweeps with Hicks injection terms hold four more entries in flight.
 template <int MODE, int H>
 constexpr int march_min_waves()
 {
