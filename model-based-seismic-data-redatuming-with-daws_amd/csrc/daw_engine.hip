@@ -530,7 +530,10 @@ static int finalize_sweepset(SweepSet &W)
         if (c.march) {
             // per shot: row segments (the back-propagation's receiver rows in
             // short ones) times groups of four strips, strips fastest; entry
-            // (shot, z0 << 16 | rows << 8 | group)
+            // (shot, z0 << 16 | rows << 8 | up << 7 | group).  Odd segments
+            // walk bottom to top, so two neighbouring segments read the rows
+            // of their common ring at the same point of their walks (both
+            // first or both last) and the second read is an L2 hit.
             // The short segments of the injection band launch first within
             // each XCD's run: their wavefronts carry the receiver terms'
             // dependent loads, the longest chains of the sweep.
@@ -539,10 +542,12 @@ static int finalize_sweepset(SweepSet &W)
                 const ShotDev &S = W.host[s];
                 const int2 b = order == kBack && s < (int)W.inj_band.size() ? W.inj_band[s] : make_int2(0, -1);
                 const int groups = ((S.n2e + kMarchCols - 1) / kMarchCols + 3) / 4;
-                for (const int2 &sg : march_segments(S.n1e, c.tz, b.x, b.y))
+                if (groups > 128 || S.n1e >= 32768) return fail(DAW_ERR_ARG, "grid %d x %d too large for the column march", S.n1e, S.n2e);
+                const std::vector<int2> segs = march_segments(S.n1e, c.tz, b.x, b.y);
+                for (size_t i = 0; i < segs.size(); ++i)
                     for (int q = 0; q < groups; ++q) {
-                        ys[s].push_back((sg.x << 16) | (sg.y << 8) | q);
-                        cls[s].push_back(sg.y == c.tz ? 0 : 1);
+                        ys[s].push_back((segs[i].x << 16) | (segs[i].y << 8) | (int)(i & 1) << 7 | q);
+                        cls[s].push_back(segs[i].y == c.tz ? 0 : 1);
                     }
                 tps[s] = (int)ys[s].size();
             }

@@ -1434,10 +1434,15 @@ __global__ __launch_bounds__(kBlock) void k_step2(StepParams P)
 // shifts, which need every lane of the wavefront active.  Per-row profile
 // values (d1a, d1a1, d1b, bw1) and the injection / record-band row maps are
 // read once per wavefront into lanes and taken out with readlane per row.
-// A workgroup's tile entry holds its shot and (z0 << 16 | rows << 8 | g):
-// its four wavefronts take strips 4g .. 4g+3 of the row segment z0 ..
+// A workgroup's tile entry holds its shot and (z0 << 16 | rows << 8 | up << 7
+// | g): its four wavefronts take strips 4g .. 4g+3 of the row segment z0 ..
 // z0+rows-1 (rows <= H; the host splits rows with many Hicks injection
-// terms into short segments).  The record blocks come first as in k_step.
+// terms into short segments).  With up set the walk runs bottom to top
+// (iteration k loads vz of row z0+rows+2-k, forms p of row zr+2 and the
+// velocities of row zr+3), the same operations on the same operands in the
+// other row order; alternate segments walk up, so two neighbouring segments
+// read the ring rows they share at the same time.  The record blocks come
+// first as in k_step.
 // FP32 snapshots only (the 16-bit codec scales 64-aligned segments).
 // ---------------------------------------------------------------------------
 
@@ -1536,7 +1541,8 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     if (it >= nt) return;
     const int n1e = S->n1e, n2e = S->n2e, ldx = S->ldx, npml = S->npml, n1 = S->n1, n2 = S->n2;
     const int nstrip = (n2e + kMarchCols - 1) / kMarchCols;
-    const int z0 = t.y >> 16, nrows = (t.y >> 8) & 255, strip = 4 * (t.y & 255) + wv;
+    const int z0 = t.y >> 16, nrows = (t.y >> 8) & 255, strip = 4 * (t.y & 127) + wv;
+    const bool up = (t.y >> 7) & 1;              // walk the segment bottom to top
     if (strip >= nstrip) return;                 // whole wavefront: no lane of it takes part in a shift
     const int x0 = strip * kMarchCols;
     const int gx = x0 - 3 + lane;
@@ -1599,9 +1605,9 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     auto body = [&](auto hc) {
         constexpr int HH = decltype(hc)::value;
         constexpr int NK = HH + 6;                   // iterations: vz rows z0-3 .. z0+HH+2
-        // ---- the loads of iteration k (vz of row z0-3+k; vx, px, pz, kappa and
-        // the extended source of row z0-4+k; the velocity snapshots of row
-        // z0-6+k) into slot k % D, issued D iterations before they are used.
+        // ---- the loads of iteration k (vz of row zr; vx, px, pz, kappa and
+        // the extended source of row zp; the velocity snapshots of row zv)
+        // into slot k % D, issued D iterations before they are used.
         // No branch around a load (the wait counts stay exact), and no mask on
         // a loaded value: rows and columns outside the grid read a clamped
         // in-grid address, and nothing stored depends on what they hold (a point
@@ -1610,7 +1616,7 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
         float Lvz[D] = {}, Lvx[D] = {}, Lpx[D] = {}, Lpz[D] = {}, Lka[D] = {}, Lsp[D] = {}, Lsx[D] = {}, Lsz[D] = {};
         auto issue = [&](int k) {
             const int s = k % D;
-            const int zr = z0 - 3 + k, zp = zr - 1, zv = zr - 3;
+            const int zr = up ? z0 + HH + 2 - k : z0 - 3 + k, zp = up ? zr + 2 : zr - 1, zv = up ? zr + 3 : zr - 3;
             Lvz[s] = bld(rs_st, vcol, rd + 3 * pe4 + (unsigned)clampi(zr, 0, n1e - 1) * ldx4);
             if (k >= 3) {
                 const unsigned r = (unsigned)clampi(zp, 0, n1e - 1) * ldx4;
@@ -1641,15 +1647,16 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     #pragma unroll
         for (int k = 0; k < NK; ++k) {
             const int s = k % D;
-            const int zr = z0 - 3 + k;
+            const int zr = up ? z0 + HH + 2 - k : z0 - 3 + k;
             // this iteration's values out of slot s, then the slot's next loads
             const float r_vz = Lvz[s], r_vx = Lvx[s], r_px = Lpx[s], r_pz = Lpz[s], r_ka = Lka[s], r_sp = Lsp[s];
             const float r_sx = Lsx[s], r_sz = Lsz[s];
             if (k + D < NK) issue(k + D);
             qvz[k & 3] = r_vz;                               // old vz of row zr
-            // ---- pressure of row zp = zr - 1 (tile rows and ring), k >= 3
+            // ---- pressure of row zp = zr - 1 (zr + 2 walking up: the tile rows and
+            // the ring), k >= 3; the queue holds vz of rows zr .. zr -/+ 3
             if (k >= 3) {
-                const int zp = zr - 1;
+                const int zp = up ? zr + 2 : zr - 1;
                 float p = 0.0f;
                 if (zp < n1e) {                               // zp >= z0 - 1 >= -1; row -1 is outside
                     const float vx = r_vx, kap = r_ka;
@@ -1658,7 +1665,11 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                     const float vxm1 = from_left(vx), vxm2 = from_left(vxm1), vxp1 = from_right(vx);
                     if (zp >= lo && zp <= hiz) {
                         const float der2 = a0 * (vx - vxm1) + a1 * (vxp1 - vxm2);
-                        const float der1 = a0 * (qvz[(k - 1) & 3] - qvz[(k - 2) & 3]) + a1 * (qvz[k & 3] - qvz[(k - 3) & 3]);
+                        const float vz0 = up ? qvz[(k - 2) & 3] : qvz[(k - 1) & 3];    // vz(zp)
+                        const float vzm1 = up ? qvz[(k - 1) & 3] : qvz[(k - 2) & 3];   // vz(zp-1)
+                        const float vzp1 = up ? qvz[(k - 3) & 3] : qvz[k & 3];         // vz(zp+1)
+                        const float vzm2 = up ? qvz[k & 3] : qvz[(k - 3) & 3];         // vz(zp-2)
+                        const float der1 = a0 * (vz0 - vzm1) + a1 * (vzp1 - vzm2);
                         const float tk = dtdx * kap;
                         const float kw2 = tk * c_d2a1;
                         const float kw1 = tk * lane_row_f(l_d1a1, zp, z0);
@@ -1706,17 +1717,21 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                 qp[k & 3] = p;
                 qvx[k % 3] = r_vx;
             }
-            // ---- velocities of row zv = zr - 3 (the tile rows), k >= 6
+            // ---- velocities of row zv = zr - 3 (zr + 3 walking up: the tile
+            // rows), k >= 6; the queue holds p of rows zp .. zp -/+ 3
             if (k >= 6) {
-                const int zv = zr - 3;
+                const int zv = up ? zr + 3 : zr - 3;
                 if (zv < n1e) {
                     const bool rowi = zv >= npml && zv < npml + n1;
-                    const float p0 = qp[(k - 2) & 3];
+                    const float p0 = up ? qp[(k - 1) & 3] : qp[(k - 2) & 3];    // p(zv)
                     const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
-                    float vxv = qvx[(k - 2) % 3], vzv = qvz[(k - 3) & 3];
+                    float vxv = up ? qvx[(k - 1) % 3] : qvx[(k - 2) % 3], vzv = qvz[(k - 3) & 3];
                     if (zv >= lo && zv <= hiz) {
                         const float der2 = a0 * (pr1 - p0) + a1 * (pr2 - pl1);
-                        const float der1 = a0 * (qp[(k - 1) & 3] - p0) + a1 * (qp[k & 3] - qp[(k - 3) & 3]);
+                        const float pm1 = up ? qp[k & 3] : qp[(k - 3) & 3];         // p(zv-1)
+                        const float pp1 = up ? qp[(k - 2) & 3] : qp[(k - 1) & 3];   // p(zv+1)
+                        const float pp2 = up ? qp[(k - 3) & 3] : qp[k & 3];         // p(zv+2)
+                        const float der1 = a0 * (pp1 - p0) + a1 * (pp2 - pm1);
                         const float d1b = lane_row_f(l_d1b, zv, z0), bw1 = lane_row_f(l_bw1, zv, z0);
                         float nx, nz;
                         if (MODE == kBack) {
