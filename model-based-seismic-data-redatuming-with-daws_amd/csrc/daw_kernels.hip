@@ -1638,11 +1638,13 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
 
         const float a0 = 1.125f;
         const float a1 = -(1.0f / 24.0f);
-        float qvz[4], qvx[3], qp[4];
-    #pragma unroll
-        for (int j = 0; j < 4; ++j) { qvz[j] = 0.0f; qp[j] = 0.0f; }
-    #pragma unroll
-        for (int j = 0; j < 3; ++j) qvx[j] = 0.0f;
+        // queues as named registers, newest first (z0: vz loaded at k, z1 at
+        // k-1, ...; x0.. vx and p0q.. p likewise), shifted each iteration (in
+        // the unrolled walk a shift is a renaming).  Walking down the rows in
+        // a queue descend from the newest; walking up they ascend, and each
+        // operand is selected by direction between two named registers.
+        float z0q = 0.0f, z1q = 0.0f, z2q = 0.0f, z3q = 0.0f, x0q = 0.0f, x1q = 0.0f, x2q = 0.0f;
+        float p0q = 0.0f, p1q = 0.0f, p2q = 0.0f, p3q = 0.0f;
 
     #pragma unroll
         for (int k = 0; k < NK; ++k) {
@@ -1652,7 +1654,7 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
             const float r_vz = Lvz[s], r_vx = Lvx[s], r_px = Lpx[s], r_pz = Lpz[s], r_ka = Lka[s], r_sp = Lsp[s];
             const float r_sx = Lsx[s], r_sz = Lsz[s];
             if (k + D < NK) issue(k + D);
-            qvz[k & 3] = r_vz;                               // old vz of row zr
+            z3q = z2q; z2q = z1q; z1q = z0q; z0q = r_vz;     // old vz of row zr
             // ---- pressure of row zp = zr - 1 (zr + 2 walking up: the tile rows and
             // the ring), k >= 3; the queue holds vz of rows zr .. zr -/+ 3
             if (k >= 3) {
@@ -1665,10 +1667,10 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                     const float vxm1 = from_left(vx), vxm2 = from_left(vxm1), vxp1 = from_right(vx);
                     if (zp >= lo && zp <= hiz) {
                         const float der2 = a0 * (vx - vxm1) + a1 * (vxp1 - vxm2);
-                        const float vz0 = up ? qvz[(k - 2) & 3] : qvz[(k - 1) & 3];    // vz(zp)
-                        const float vzm1 = up ? qvz[(k - 1) & 3] : qvz[(k - 2) & 3];   // vz(zp-1)
-                        const float vzp1 = up ? qvz[(k - 3) & 3] : qvz[k & 3];         // vz(zp+1)
-                        const float vzm2 = up ? qvz[k & 3] : qvz[(k - 3) & 3];         // vz(zp-2)
+                        const float vz0 = up ? z2q : z1q;     // vz(zp)
+                        const float vzm1 = up ? z1q : z2q;    // vz(zp-1)
+                        const float vzp1 = up ? z3q : z0q;    // vz(zp+1)
+                        const float vzm2 = up ? z0q : z3q;    // vz(zp-2)
                         const float der1 = a0 * (vz0 - vzm1) + a1 * (vzp1 - vzm2);
                         const float tk = dtdx * kap;
                         const float kw2 = tk * c_d2a1;
@@ -1714,8 +1716,8 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         }
                     }
                 }
-                qp[k & 3] = p;
-                qvx[k % 3] = r_vx;
+                p3q = p2q; p2q = p1q; p1q = p0q; p0q = p;
+                x2q = x1q; x1q = x0q; x0q = r_vx;
             }
             // ---- velocities of row zv = zr - 3 (zr + 3 walking up: the tile
             // rows), k >= 6; the queue holds p of rows zp .. zp -/+ 3
@@ -1723,14 +1725,14 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                 const int zv = up ? zr + 3 : zr - 3;
                 if (zv < n1e) {
                     const bool rowi = zv >= npml && zv < npml + n1;
-                    const float p0 = up ? qp[(k - 1) & 3] : qp[(k - 2) & 3];    // p(zv)
+                    const float p0 = up ? p1q : p2q;     // p(zv)
                     const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
-                    float vxv = up ? qvx[(k - 1) % 3] : qvx[(k - 2) % 3], vzv = qvz[(k - 3) & 3];
+                    float vxv = up ? x1q : x2q, vzv = z3q;
                     if (zv >= lo && zv <= hiz) {
                         const float der2 = a0 * (pr1 - p0) + a1 * (pr2 - pl1);
-                        const float pm1 = up ? qp[k & 3] : qp[(k - 3) & 3];         // p(zv-1)
-                        const float pp1 = up ? qp[(k - 2) & 3] : qp[(k - 1) & 3];   // p(zv+1)
-                        const float pp2 = up ? qp[(k - 3) & 3] : qp[k & 3];         // p(zv+2)
+                        const float pm1 = up ? p0q : p3q;    // p(zv-1)
+                        const float pp1 = up ? p2q : p1q;    // p(zv+1)
+                        const float pp2 = up ? p3q : p0q;    // p(zv+2)
                         const float der1 = a0 * (pp1 - p0) + a1 * (pp2 - pm1);
                         const float d1b = lane_row_f(l_d1b, zv, z0), bw1 = lane_row_f(l_bw1, zv, z0);
                         float nx, nz;
