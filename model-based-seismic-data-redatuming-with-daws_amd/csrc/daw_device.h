@@ -90,9 +90,10 @@ struct RecItem {
 // Batched record groups: items first .. first+count-1 (count <= 64, one
 // shot, consecutive points of one point set) whose windows lie in grid rows
 // z0 .. z0+nz-1 and columns x0 .. x0+nx-1 (nz <= kRecRows, nx <= kRecCols).
+// (4 x 21 staged rows of 158 floats: 53 KB of LDS, three workgroups per CU)
 constexpr int kRecSteps = 4;
-constexpr int kRecRows = 24;
-constexpr int kRecCols = 140;
+constexpr int kRecRows = 21;
+constexpr int kRecCols = 138;
 struct RecGroup {
     int first, count;
     int z0, nz, x0, nx;

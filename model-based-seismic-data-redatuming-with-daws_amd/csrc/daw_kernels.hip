@@ -1457,28 +1457,16 @@ __device__ __forceinline__ float from_right(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));   // wave_shl:1
 }
 
-// lane j of the wavefront holds v[z0 - 1 + j] (rows outside 0..n-1: fill)
-__device__ __forceinline__ float row_lane_f(const GLB float *v, int z0, int n, int lane, float fill)
+// lane k's value (k a constant: the walk's iteration) as a scalar
+__device__ __forceinline__ float lane_f(float v, int k)
 {
-    const int z = z0 - 1 + lane;
-    const float a = ld(v, (unsigned)clampi(z, 0, n - 1));
-    return (z >= 0 && z < n) ? a : fill;
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
-__device__ __forceinline__ int row_lane_i(const GLB int *v, int z0, int n, int lane)
-{
-    const int z = z0 - 1 + lane;
-    const int a = ld(v, (unsigned)clampi(z, 0, n - 1));
-    return (z >= 0 && z < n) ? a : -1;
-}
-// row z's value out of the lanes (z - z0 + 1 in 0..63): a scalar
-__device__ __forceinline__ float lane_row_f(float v, int z, int z0)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), z - z0 + 1));
-}
-__device__ __forceinline__ int lane_row_i(int v, int z, int z0)
-{
-    return __builtin_amdgcn_readlane(v, z - z0 + 1);
-}
+__device__ __forceinline__ int lane_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ unsigned lane_u(unsigned v, int k) { return (unsigned)__builtin_amdgcn_readlane((int)v, k); }
+// one bit per lane of the wavefront; bit k of it
+__device__ __forceinline__ unsigned long long lanes_where(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+__device__ __forceinline__ bool bit(unsigned long long m, int k) { return (m >> k) & 1ull; }
 // Raw buffer accesses: a resource (base address) in scalar registers, this
 // lane's byte offset (its column, computed once) and a scalar byte offset
 // (plane and row, formed on the scalar unit), so a row's load or store costs
@@ -1520,13 +1508,13 @@ constexpr int kMarchAhead = 3;
 template <int MODE, int H>
 constexpr int march_min_waves()
 {
-    return H <= 12 ? (MODE == kReinject ? 8 : 6) : H <= 16 ? 6 : 4;
+    return H <= 12 ? (MODE == kReinject ? 7 : 6) : H <= 16 ? 6 : 4;
 }
 
 template <int MODE, int H>
 __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(StepParams P)
 {
-    static_assert(H + 3 <= 64, "row profiles of a segment must fit the lanes");
+    static_assert(H + 6 <= 64, "the iterations of a walk must fit the lanes");
     constexpr int D = kMarchAhead;
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     __shared__ __align__(16) float s_lds[kRecPerBlock * kRecStride];
@@ -1586,17 +1574,11 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     const float c_d2a1 = ld(prof + (4 * a1e + n2e), gxc);
     const float c_d2b = ld(prof + (4 * a1e + 2 * n2e), gxc);
     const float c_bw2 = ld(prof + (4 * a1e + 3 * n2e), gxc);
-    // row profiles of rows z0-1 .. z0+62 in lanes
-    const float l_d1a = row_lane_f(prof, z0, n1e, lane, 1.0f);
-    const float l_d1a1 = row_lane_f(prof + a1e, z0, n1e, lane, 1.0f);
-    const float l_d1b = row_lane_f(prof + 2 * a1e, z0, n1e, lane, 1.0f);
-    const float l_bw1 = row_lane_f(prof + 3 * a1e, z0, n1e, lane, 0.0f);
     const GLB int *const inj_row = glb(S->inj_row);
     const bool inj_p = MODE != kReinject && inj_row != nullptr && !S->vinj;   // Hicks injection into px
     const bool inj_v = MODE == kFwd && inj_row != nullptr && S->vinj;         // vz source
-    const int l_inj = (inj_p || inj_v) ? row_lane_i(inj_row, z0, n1e, lane) : -1;
     const GLB int *const band_row = glb(S->band_row);
-    const int l_band = (MODE != kBack && band_row != nullptr) ? row_lane_i(band_row, z0, n1e, lane) : -1;
+    const bool rec_band = MODE != kBack && band_row != nullptr;
     const Rsrc rs_band = make_rsrc(S->pband + (size_t)(it & (kBandSlots - 1)) * S->band_stride);
     GLB float *const wf = glb(S->wf);
 
@@ -1605,6 +1587,38 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     auto body = [&](auto hc) {
         constexpr int HH = decltype(hc)::value;
         constexpr int NK = HH + 6;                   // iterations: vz rows z0-3 .. z0+HH+2
+        // What iteration k needs of its rows sits in lane k: the rows (vz of
+        // zr, p of zp, velocities of zv), their clamped byte offsets, profile
+        // values and row-map entries; each per-row condition is bit k of a
+        // wavefront mask.  The walk takes them with constant lane and bit
+        // indices, so a row costs few scalar instructions.
+        const int zr_l = up ? z0 + HH + 2 - lane : z0 - 3 + lane;
+        const int zp_l = up ? zr_l + 2 : zr_l - 1, zv_l = up ? zr_l + 3 : zr_l - 3;
+        const bool pin = zp_l >= 0 && zp_l < n1e, vin = zv_l >= 0 && zv_l < n1e;
+        const unsigned zpc = (unsigned)clampi(zp_l, 0, n1e - 1), zvc = (unsigned)clampi(zv_l, 0, n1e - 1);
+        const unsigned o_vz = rd + 3 * pe4 + (unsigned)clampi(zr_l, 0, n1e - 1) * ldx4;   // vz load
+        const unsigned o_p = zpc * ldx4, o_v = zvc * ldx4;                                // state rows zp, zv
+        const unsigned o_sp = (unsigned)(clampi(zp_l, xlo1, xhi1) - npml) * sp4;           // extended source row zp
+        const unsigned o_sv = (unsigned)(clampi(zv_l, npml, npml + n1 - 1) - npml) * sp4;  // velocity snapshot row zv
+        const float l_d1a = pin ? ld(prof, zpc) : 1.0f, l_d1a1 = pin ? ld(prof + a1e, zpc) : 1.0f;
+        const float l_d1b = vin ? ld(prof + 2 * a1e, zvc) : 1.0f, l_bw1 = vin ? ld(prof + 3 * a1e, zvc) : 0.0f;
+        const int l_injp = inj_p && pin ? ld(inj_row, zpc) : -1;
+        const int l_injv = inj_v && vin ? ld(inj_row, zvc) : -1;
+        const int l_band = rec_band && pin ? ld(band_row, zpc) : -1;
+        const unsigned long long mPv = lanes_where(zp_l < n1e);                      // p row exists (zp >= -1)
+        const unsigned long long mPbox = lanes_where(zp_l >= lo && zp_l <= hiz);     // p row in the stencil box
+        const unsigned long long mPinj = lanes_where(l_injp >= 0);                   // p row with injection terms
+        const unsigned long long mPsrc = lanes_where(zp_l >= xlo1 && zp_l <= xhi1);  // p row of the extended source
+        const unsigned long long mPfs = lanes_where(ofs == 1 && zp_l <= npml);       // p row zeroed (free surface)
+        const unsigned long long mPst = lanes_where(zp_l >= z0 && zp_l < z0 + HH);   // p row of the tile (stored)
+        const unsigned long long mPint = lanes_where(zp_l >= npml && zp_l < npml + n1);
+        const unsigned long long mPband = lanes_where(l_band >= 0);
+        const unsigned long long mVv = lanes_where(zv_l < n1e);
+        const unsigned long long mVbox = lanes_where(zv_l >= lo && zv_l <= hiz);
+        const unsigned long long mVint = lanes_where(zv_l >= npml && zv_l < npml + n1);
+        const unsigned long long mVinj = lanes_where(l_injv >= 0);
+        const unsigned long long mVfs0 = lanes_where(ofs == 1 && zv_l == npml - 1);  // vz row written by the mirror
+        const unsigned long long mVfs1 = lanes_where(ofs == 1 && zv_l == npml);      // vz row mirrored upwards
         // ---- the loads of iteration k (vz of row zr; vx, px, pz, kappa and
         // the extended source of row zp; the velocity snapshots of row zv)
         // into slot k % D, issued D iterations before they are used.
@@ -1616,18 +1630,17 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
         float Lvz[D] = {}, Lvx[D] = {}, Lpx[D] = {}, Lpz[D] = {}, Lka[D] = {}, Lsp[D] = {}, Lsx[D] = {}, Lsz[D] = {};
         auto issue = [&](int k) {
             const int s = k % D;
-            const int zr = up ? z0 + HH + 2 - k : z0 - 3 + k, zp = up ? zr + 2 : zr - 1, zv = up ? zr + 3 : zr - 3;
-            Lvz[s] = bld(rs_st, vcol, rd + 3 * pe4 + (unsigned)clampi(zr, 0, n1e - 1) * ldx4);
+            Lvz[s] = bld(rs_st, vcol, lane_u(o_vz, k));
             if (k >= 3) {
-                const unsigned r = (unsigned)clampi(zp, 0, n1e - 1) * ldx4;
+                const unsigned r = lane_u(o_p, k);
                 Lvx[s] = bld(rs_st, vcol, rd + 2 * pe4 + r);
                 Lpx[s] = bld(rs_st, vcol, rd + r);
                 Lpz[s] = bld(rs_st, vcol, rd + pe4 + r);
                 Lka[s] = bld(rs_ka, vcol, r);
-                if (MODE == kReinject) Lsp[s] = bld_nt(rs_sp, sxs, (unsigned)(clampi(zp, xlo1, xhi1) - npml) * sp4);
+                if (MODE == kReinject) Lsp[s] = bld_nt(rs_sp, sxs, lane_u(o_sp, k));
             }
             if (MODE == kReinject && k >= 6) {
-                const unsigned r = (unsigned)(clampi(zv, npml, npml + n1 - 1) - npml) * sp4;
+                const unsigned r = lane_u(o_sv, k);
                 Lsx[s] = bld_nt(rs_sx, sxi, r);
                 Lsz[s] = bld_nt(rs_sz, sxi, r);
             }
@@ -1649,7 +1662,6 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     #pragma unroll
         for (int k = 0; k < NK; ++k) {
             const int s = k % D;
-            const int zr = up ? z0 + HH + 2 - k : z0 - 3 + k;
             // this iteration's values out of slot s, then the slot's next loads
             const float r_vz = Lvz[s], r_vx = Lvx[s], r_px = Lpx[s], r_pz = Lpz[s], r_ka = Lka[s], r_sp = Lsp[s];
             const float r_sx = Lsx[s], r_sz = Lsz[s];
@@ -1658,14 +1670,13 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
             // ---- pressure of row zp = zr - 1 (zr + 2 walking up: the tile rows and
             // the ring), k >= 3; the queue holds vz of rows zr .. zr -/+ 3
             if (k >= 3) {
-                const int zp = up ? zr + 2 : zr - 1;
                 float p = 0.0f;
-                if (zp < n1e) {                               // zp >= z0 - 1 >= -1; row -1 is outside
+                if (bit(mPv, k)) {                            // zp >= z0 - 1 >= -1; row -1 is outside
                     const float vx = r_vx, kap = r_ka;
                     float pxv = r_px, pzv = r_pz;
                     // every lane shifts (inactive lanes would feed the bound value)
                     const float vxm1 = from_left(vx), vxm2 = from_left(vxm1), vxp1 = from_right(vx);
-                    if (zp >= lo && zp <= hiz) {
+                    if (bit(mPbox, k)) {
                         const float der2 = a0 * (vx - vxm1) + a1 * (vxp1 - vxm2);
                         const float vz0 = up ? z2q : z1q;     // vz(zp)
                         const float vzm1 = up ? z1q : z2q;    // vz(zp-1)
@@ -1674,8 +1685,8 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         const float der1 = a0 * (vz0 - vzm1) + a1 * (vzp1 - vzm2);
                         const float tk = dtdx * kap;
                         const float kw2 = tk * c_d2a1;
-                        const float kw1 = tk * lane_row_f(l_d1a1, zp, z0);
-                        const float d1a = lane_row_f(l_d1a, zp, z0);
+                        const float kw1 = tk * lane_f(l_d1a1, k);
+                        const float d1a = lane_f(l_d1a, k);
                         float nx, nz;
                         if (MODE == kBack) {
                             nx = c_d2a * pxv - kw2 * der2;
@@ -1687,31 +1698,32 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         pxv = colbox ? nx : pxv;
                         pzv = colbox ? nz : pzv;
                     }
-                    if (inj_p && zp >= 0) {
-                        const int injq = lane_row_i(l_inj, zp, z0);
-                        if (injq >= 0 && colok) {                     // :1042 (subtracted), :175 (added)
+                    if (inj_p && bit(mPinj, k)) {
+                        const int injq = lane_i(l_injp, k);
+                        if (colok) {                                  // :1042 (subtracted), :175 (added)
                             const GLB int *ptr = glb(S->inj_ptr) + injq * (n2e + 1);
                             pxv = inject_terms<MODE == kBack ? -1 : 1>(pxv, kap, ptr[gx], ptr[gx + 1], glb(S->inj_r),
                                                                        glb(S->inj_cdt), glb(S->inj_amp), S->amp_stride,
                                                                        it * S->amp_tstride);
                         }
                     }
-                    if (MODE == kReinject && zp >= xlo1 && zp <= xhi1)
+                    if (MODE == kReinject && bit(mPsrc, k))
                         pxv = xcol ? pxv + (mdt * r_sp) * kap : pxv;   // ss*dt*sp*kappa, ss = -1 (:1123)
                     p = pxv + pzv;
-                    if (ofs == 1 && zp <= npml) p = 0.0f;        // p(1:npml+1,:) = 0
-                    if (zp >= z0 && zp < z0 + HH && own) {
-                        const unsigned r = (unsigned)zp * ldx4;
+                    if (bit(mPfs, k)) p = 0.0f;                  // p(1:npml+1,:) = 0
+                    if (bit(mPst, k) && own) {
+                        const unsigned r = lane_u(o_p, k);
                         bst(rs_st, vcol, wr + r, pxv);
                         bst(rs_st, vcol, wr + pe4 + r, pzv);
                         if (MODE != kBack) {
-                            if (band_row != nullptr) {
-                                const int slot = lane_row_i(l_band, zp, z0);
-                                if (slot >= 0) bst(rs_band, vcol, (unsigned)slot * ldx4, p);
-                            }
-                            if (wf != nullptr && zp >= npml && zp < npml + n1 && colint)
+                            if (rec_band && bit(mPband, k))
+                                bst(rs_band, vcol, (unsigned)lane_i(l_band, k) * ldx4, p);
+                            if (wf != nullptr && bit(mPint, k) && colint) {
+                                const int zp = up ? z0 + HH + 4 - k : z0 - 4 + k;
                                 (wf + (size_t)it * ((size_t)n1 * n2))[(gx - npml) * n1 + (zp - npml)] = p;
-                        } else if (zp >= npml && zp < npml + n1 && colint) {
+                            }
+                        } else if (bit(mPint, k) && colint) {
+                            const int zp = up ? z0 + HH + 4 - k : z0 - 4 + k;
                             bst_nt(rs_sp, sxo, (unsigned)(zp - npml) * sp4, p);   // sp(:,:,it) (:1060-1064)
                         }
                     }
@@ -1722,19 +1734,18 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
             // ---- velocities of row zv = zr - 3 (zr + 3 walking up: the tile
             // rows), k >= 6; the queue holds p of rows zp .. zp -/+ 3
             if (k >= 6) {
-                const int zv = up ? zr + 3 : zr - 3;
-                if (zv < n1e) {
-                    const bool rowi = zv >= npml && zv < npml + n1;
+                if (bit(mVv, k)) {
+                    const bool rowi = bit(mVint, k);
                     const float p0 = up ? p1q : p2q;     // p(zv)
                     const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
                     float vxv = up ? x1q : x2q, vzv = z3q;
-                    if (zv >= lo && zv <= hiz) {
+                    if (bit(mVbox, k)) {
                         const float der2 = a0 * (pr1 - p0) + a1 * (pr2 - pl1);
                         const float pm1 = up ? p0q : p3q;    // p(zv-1)
                         const float pp1 = up ? p2q : p1q;    // p(zv+1)
                         const float pp2 = up ? p3q : p0q;    // p(zv+2)
                         const float der1 = a0 * (pp1 - p0) + a1 * (pp2 - pm1);
-                        const float d1b = lane_row_f(l_d1b, zv, z0), bw1 = lane_row_f(l_bw1, zv, z0);
+                        const float d1b = lane_f(l_d1b, k), bw1 = lane_f(l_bw1, k);
                         float nx, nz;
                         if (MODE == kBack) {
                             nx = c_d2b * vxv - c_bw2 * der2;
@@ -1746,9 +1757,9 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         vxv = colbox ? nx : vxv;
                         vzv = colbox ? nz : vzv;
                     }
-                    if (inj_v) {
-                        const int q = lane_row_i(l_inj, zv, z0);
-                        if (q >= 0 && colok) {
+                    if (inj_v && bit(mVinj, k)) {
+                        const int q = lane_i(l_injv, k);
+                        if (colok) {
                             const GLB int *ptr = glb(S->inj_ptr) + q * (n2e + 1);
                             for (int m = ptr[gx]; m < ptr[gx + 1]; ++m)
                                 vzv = vzv + glb(S->inj_cdt)[m] * glb(S->inj_amp)[glb(S->inj_r)[m] * S->amp_stride + it * S->amp_tstride];   // * bue (= 1), :252
@@ -1759,14 +1770,14 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
                         vzv = colint ? vzv + mdt * r_sz : vzv;         // :1184
                     }
                     if (own) {
-                        const unsigned r = (unsigned)zv * ldx4;
+                        const unsigned r = lane_u(o_v, k);
                         bst(rs_st, vcol, wr + 2 * pe4 + r, vxv);
-                        if (!(ofs == 1 && zv == npml - 1)) {           // vz(npml,:) is written by the mirror below
+                        if (!bit(mVfs0, k)) {                          // vz(npml,:) is written by the mirror below
                             bst(rs_st, vcol, wr + 3 * pe4 + r, vzv);
-                            if (ofs == 1 && zv == npml) bst(rs_st, vcol, wr + 3 * pe4 + r - ldx4, vzv);   // vz(npml,:) = vz(npml+1,:)
+                            if (bit(mVfs1, k)) bst(rs_st, vcol, wr + 3 * pe4 + r - ldx4, vzv);   // vz(npml,:) = vz(npml+1,:)
                         }
                         if (MODE == kBack && rowi && colint) {         // svx, svz(:,:,it) (:1082-1087)
-                            const unsigned rs = (unsigned)(zv - npml) * sp4;
+                            const unsigned rs = lane_u(o_sv, k);
                             bst_nt(rs_sx, sxo, rs, vxv);
                             bst_nt(rs_sz, sxo, rs, vzv);
                         }
@@ -1831,32 +1842,50 @@ __global__ __launch_bounds__(256) void k_rec_group(const ShotDev *__restrict__ s
     // window column ix2-10 .. ix2+10 of any item stays inside its row
     constexpr int RW = kRecCols + 20;
     __shared__ float sb[kRecSteps][kRecRows][RW];
-    __shared__ int srow[kRecRows];
+    __shared__ unsigned soff[kRecSteps * kRecRows];
     const RecGroup G = groups[blockIdx.x];
     const int j0 = (int)blockIdx.y * kRecSteps;
     const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
     const ShotDev *__restrict__ S = shots + items[G.first].shot;
     const int nt = S->nt, ldx = S->ldx;
-    if (tid < G.nz) srow[tid] = glb(S->band_row)[G.z0 + tid];
-    __syncthreads();
-    // ---- stage: wavefront w takes rows (step j, row r) = w, w+4, ... of the
-    // block, its lanes the columns; a row's loads are in flight together
-    const GLB float *band = glb(S->pband);
-    for (int jr = w; jr < kRecSteps * G.nz; jr += 4) {
-        const int j = jr / G.nz, r = jr - j * G.nz;             // uniform
-        const int it = it0 + j0 + j, slot = srow[r];
+    constexpr unsigned kNone = 0xffffffffu;
+    // byte offset in the band of staged row jr = (step j, row r), or kNone
+    // where the step is past the sweep or the row holds no band slot
+    if (tid < kRecSteps * G.nz) {
+        const int j = tid / G.nz, r = tid - j * G.nz, it = it0 + j0 + j;
+        const int slot = glb(S->band_row)[G.z0 + r];
         const bool live = j0 + j < nsteps && it < nt && slot >= 0;
-        const GLB float *src = band + (live ? (size_t)(it & (kBandSlots - 1)) * S->band_stride + (size_t)slot * ldx + G.x0 : 0);
-        float v[(kRecCols + 63) / 64];
+        soff[tid] = live ? 4u * (unsigned)((it & (kBandSlots - 1)) * S->band_stride + slot * ldx) : kNone;
+    }
+    __syncthreads();
+    // ---- stage: wavefront w takes rows jr = w, w+4, ... of the block, its
+    // lanes the columns; all of a wavefront's rows are loaded before the
+    // first is stored, so their loads are in flight together (buffer loads:
+    // lane offset = the row's offset from LDS + the column's)
+    const Rsrc rs_band = make_rsrc(S->pband);
+    constexpr int RPW = kRecSteps * kRecRows / 4, CPL = (kRecCols + 63) / 64;
+    unsigned vo[CPL];
 #pragma unroll
-        for (int m = 0; m < (kRecCols + 63) / 64; ++m) {
-            const int c = l + 64 * m;
-            v[m] = src[c < G.nx && live ? c : 0];
-        }
+    for (int m = 0; m < CPL; ++m) vo[m] = 4u * (unsigned)(l + 64 * m < G.nx ? G.x0 + l + 64 * m : 0);
+    float v[RPW][CPL];
 #pragma unroll
-        for (int m = 0; m < (kRecCols + 63) / 64; ++m) {
+    for (int i = 0; i < RPW; ++i) {
+        const int jr = w + 4 * i;
+        const unsigned o = jr < kRecSteps * G.nz ? soff[jr] : kNone;
+        const unsigned ob = o == kNone ? 0u : o;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) v[i][m] = bld(rs_band, vo[m] + ob, 0u);
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+        const int jr = w + 4 * i;
+        if (jr >= kRecSteps * G.nz) break;
+        const int j = jr / G.nz, r = jr - j * G.nz;             // uniform
+        const bool live = soff[jr] != kNone;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
             const int c = l + 64 * m;
-            if (c < G.nx) sb[j][r][10 + c] = live ? v[m] : 0.0f;
+            if (c < G.nx) sb[j][r][10 + c] = live ? v[i][m] : 0.0f;
         }
     }
     __syncthreads();
@@ -1875,18 +1904,20 @@ __global__ __launch_bounds__(256) void k_rec_group(const ShotDev *__restrict__ s
     unsigned m2 = 0;
 #pragma unroll
     for (int q = 0; q < 21; ++q) m2 |= (q - 10 >= lo2 && q - 10 <= hi2) ? (1u << q) : 0u;
-    const GLB float *ct = glb(R.coefT);
-    const int cs = R.cstride;
+    // the group's items have consecutive coefficient columns of one table:
+    // item l's coefficient k at coefT(first) + l + k*cstride
+    const Rsrc rs_ct = make_rsrc(items[G.first].coefT);
+    const unsigned cs4 = 4u * (unsigned)items[G.first].cstride, co = 4u * (unsigned)l;
     const int cbase = R.ix2 - 10 - G.x0 + 10;                 // column of i2 = -10 in a staged row
     float acc = 0.0f;
-#pragma unroll 3
+#pragma unroll 2
     for (int a = 0; a < 21; ++a) {
         const int i1 = a - 10;
         const unsigned mr = (i1 >= lo1 && i1 <= hi1) ? m2 : 0u;
         const float *row = &sb[w][clampi(R.ix1 + i1 - G.z0, 0, kRecRows - 1)][cbase];
         float c[21];
 #pragma unroll
-        for (int q = 0; q < 21; ++q) c[q] = ct[(a * 21 + q) * cs];
+        for (int q = 0; q < 21; ++q) c[q] = bld(rs_ct, co + (unsigned)(a * 21 + q) * cs4, 0u);
         // a term outside the window becomes +0 (bits cleared), which adds
         // nothing: acc is never -0; no branch, so every read is in flight
 #pragma unroll
