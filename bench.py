@@ -668,23 +668,29 @@ def main():
     dom = max((1, 2), key=lambda m: step_ms[m])
     dcfg = launch[("forward", "back-propagation", "re-injection")[dom]]
     launch_steps = dcfg[0]
-    march = dcfg[2] == 58
-    kname = "k_march" if march else "k_step2" if launch_steps == 2 else "k_step"
-    modes = {1: f"{kname}<1> (back-propagation)", 2: f"{kname}<2> (re-injection)"}
+    march = dcfg[2] in (52, 58)
+    kname = ("k_march2_reinject" if dcfg[2] == 52 else "k_march" if march
+             else "k_step2" if launch_steps == 2 else "k_step")
+    modes = ({2: f"{kname} ({dcfg[1]}-row segments, two steps per launch)"} if dcfg[2] == 52 else
+             {1: f"{kname}<1> (back-propagation)", 2: f"{kname}<2> (re-injection)"})
     avg_ms = step_ms[dom] / max(st["sweep_steps"][dom], 1) * launch_steps   # per launch
-    # algorithmic bytes of one launch per shot: the wave state read and written
-    # once (px, pz, vx, vz: 32 B) and kappa read (4 B) per extended point, plus
-    # each step's snapshots of the interior (3 fields, 4 B each, or 2 B each +
-    # one 4 B scale per 64 columns)
+    # algorithmic bytes of one launch per shot: per point-update (the unit),
+    # the wave state read and written (px, pz, vx, vz: 32 B) and kappa read
+    # (4 B) per extended point, plus the step's snapshots of the interior (3
+    # fields, 4 B each, or 2 B each + one 4 B scale per 64 columns), times the
+    # steps the launch advances (a two-step launch moves the state through HBM
+    # once: its measured traffic is lower than this)
     snap_b = 12.0 if args.snapshots == "fp32" else 6.0 + 12.0 / 64.0
-    bytes_per_shot_launch = 36.0 * n1e * n2e + launch_steps * snap_b * w.nz * w.nx
+    bytes_per_shot_launch = launch_steps * (36.0 * n1e * n2e + snap_b * w.nz * w.nx)
     alg_bytes = bytes_per_shot_launch * shots_per_launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     snap_code = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snapshots]
-    targs = (f"{dom},{dcfg[1]}" if march                                                # k_march<MODE, ROWS>
+    targs = (None if dcfg[2] == 52                                                      # k_march2_reinject
+             else f"{dom},{dcfg[1]}" if march                                           # k_march<MODE, ROWS>
              else f"{dom},{snap_code},{dcfg[1]},{dcfg[2] // 64}" if launch_steps == 2     # k_step2<MODE, SNAP, TZ, NC>
              else f"{dom},{dcfg[1]},{dcfg[2] // 64},{snap_code}")                     # k_step<MODE, TZ, TXW, SNAP>
-    traffic_key = f"{kname}<{targs}> {w.nz}x{w.nx} {int(shots_per_launch)} shots per launch"
+    traffic_key = (f"{kname}<{targs}>" if targs else f"{kname} rows {dcfg[1]}") + \
+        f" {w.nz}x{w.nx} {int(shots_per_launch)} shots per launch"
     traffic = measured_traffic(traffic_key)
     datum = torch.cat([b["datum"] for b in batches])
     hist = np.concatenate([hists[b] for b in sorted(hists)]) if hists else None

@@ -337,10 +337,13 @@ int daw_set_graph(int on);
 /* Step-kernel tile (rows x columns): 8x128, 16x64, 16x128, 20x64, 24x64, 32x64, 32x128
  * or 64x64, or R x 58 (R = 8, 12, 16, 24, 32): the column march, one wavefront
  * per 58 owned columns walking R rows (forward sweeps, and the Hessian sweeps
- * with FP32 snapshots; with 16-bit snapshots those take 16x64); rows = 0 lets
- * daw_redatum_batch time every shape on a short probe sweep per batch shape
- * and sweep mode and keep the fastest (the default).  Results are identical.
- * No reference counterpart: the launch shape of the autotuner. */
+ * with FP32 snapshots; with 16-bit snapshots those take 16x64), or R x 52
+ * (R = 16, 24, 32, 40, 48): the two-step column march, two time steps per
+ * launch (re-injection sweeps with FP32 snapshots and batched records; the
+ * other sweeps take 16x64, and an odd last step the R x 58 march, R <= 32);
+ * rows = 0 lets daw_redatum_batch time every shape on a short probe sweep per
+ * batch shape and sweep mode and keep the fastest (the default).  Results are
+ * identical.  No reference counterpart: the launch shape of the autotuner. */
 int daw_set_tile(int rows, int cols);
 /* 1 (default): the Hicks records of a sweep (data and datum gathers) run as
  * batched launches every few steps, each workgroup staging the band of up to

@@ -661,8 +661,9 @@ def filmartin2d_dev(din_dev, dout_dev, dt, flow, fhig, llow, lhig):
 
 def set_tile(rows, cols=64):
     """Step-kernel tile rows x cols (8x128, 16x64, 16x128, 20x64, 24x64, 32x64,
-    32x128, 64x64; rows x 58 for the column march, rows = 8, 12, 16, 24, 32),
-    or rows = 0 for the autotuner (default)."""
+    32x128, 64x64; rows x 58 for the column march, rows = 8, 12, 16, 24, 32;
+    rows x 52 for the two-step march of the re-injection, rows = 16, 24, 32,
+    40, 48), or rows = 0 for the autotuner (default)."""
     _check(lib().daw_set_tile(int(rows), int(cols)))
 
 

@@ -49,7 +49,8 @@ static int fail(int code, const char *fmt, ...)
 
 // ------------------------------------------------------------------ state
 // Launch configuration of one sweep mode: time steps per launch and tile;
-// march = 1: the column-march kernel, tz rows per wavefront (txw unused).
+// march = 1: the column-march kernel, tz rows per wavefront (txw unused);
+// march = 2: the two-step column march of the re-injection (steps = 2).
 // rec_batch = 1: the Hicks records of a sweep run as batched launches every
 // few steps (k_rec_group) instead of as record blocks inside the step launches.
 struct LaunchCfg {
@@ -57,7 +58,7 @@ struct LaunchCfg {
 };
 // The tile's columns as the API reports them: 64 per wavefront, or the 58
 // owned columns of a marching wavefront.
-static int tile_cols(const LaunchCfg &c) { return c.march ? kMarchCols : 64 * c.txw; }
+static int tile_cols(const LaunchCfg &c) { return c.march == 2 ? kMarch2Cols : c.march ? kMarchCols : 64 * c.txw; }
 
 struct Global {
     std::mutex mu;
@@ -415,8 +416,16 @@ struct SweepSet {
 // Steps per launch of the sweeps: two only where the tile has a two-step kernel.
 static int steps_per_launch(const LaunchCfg &c)
 {
+    if (c.march == 2) return 2;
     return (!c.march && c.steps == 2 && tile2_shape_ok(c.tz, c.txw)) ? 2 : 1;
 }
+
+// Row heights of the two-step march (a walk of rows + 12 iterations, one per lane).
+static bool march2_rows_ok(int h) { return h == 16 || h == 24 || h == 32 || h == 40 || h == 48; }
+
+// The one-step march that runs a two-step march's last step when the sweep
+// has an odd number of steps (32-row segments for the taller two-step ones).
+static LaunchCfg march2_tail_cfg(const LaunchCfg &c) { return LaunchCfg{1, std::min(c.tz, 32), 0, 1, c.rec_batch}; }
 
 static std::vector<int> tiles_key(const LaunchCfg &c, int mode)
 {
@@ -443,14 +452,35 @@ static std::vector<int2> march_segments(int n1e, int h, int b0, int b1)
 // where the back-propagation marches.
 static LaunchCfg adjoint_cfg(const LaunchCfg &c) { return c.march ? LaunchCfg{1, 16, 1, 0, c.rec_batch} : c; }
 
-// The column march runs the forward sweep, and the Hessian sweeps with FP32 snapshots.
-static bool march_ok(const LaunchCfg &c, int mode, int snap) { return !c.march || mode == kFwd || snap == kSnapF32; }
+// The column march runs the forward sweep, and the Hessian sweeps with FP32
+// snapshots; the two-step march only the re-injection, with FP32 snapshots and
+// batched records.
+static bool march_ok(const LaunchCfg &c, int mode, int snap)
+{
+    if (c.march == 2) return mode == kReinject && snap == kSnapF32 && c.rec_batch && march2_rows_ok(c.tz);
+    return !c.march || mode == kFwd || snap == kSnapF32;
+}
 
 // The configured launch of a mode (16x64 where a fixed march cannot run it).
 static LaunchCfg fixed_cfg(int mode, int snap)
 {
-    const LaunchCfg c{g.steps, g.tz, g.txw, g.march, g.rec_batch};
+    const LaunchCfg c{g.march == 2 ? 2 : g.steps, g.tz, g.txw, g.march, g.rec_batch};
     return march_ok(c, mode, snap) ? c : LaunchCfg{1, 16, 1, 0, g.rec_batch};
+}
+
+// Row segments of the two-step march: h rows each; with the free surface a
+// segment whose step-A velocity rows (z0-3 .. z0+h+2) would end at row
+// npml-1 is one row shorter, so that the segment holding row npml-1 also
+// holds row npml (the kernel's vz mirror needs both).
+static std::vector<int2> march2_segments(int n1e, int h, int npml, int ofs)
+{
+    std::vector<int2> segs;
+    for (int z = 0; z < n1e;) {
+        const int r = (ofs == 1 && z + h + 2 == npml - 1) ? h - 1 : h;
+        segs.push_back(make_int2(z, r));
+        z += r;
+    }
+    return segs;
 }
 
 // Copies the record items of the next sweep to the batch's item buffer
@@ -519,14 +549,40 @@ static int finalize_sweepset(SweepSet &W)
         W.nt_max = std::max(W.nt_max, W.host[s].nt);
         W.pts_per_step += (long long)W.host[s].n1e * W.host[s].n2e;
     }
-    for (int m = 0; m < 4; ++m) {
-        // modes 0-2, then the exact adjoint's tile (the back-propagation's)
-        const LaunchCfg c = m < 3 ? W.cfg[m] : adjoint_cfg(W.cfg[kBack]);
-        const int order = m < 3 ? m : kBack;
+    for (int m = 0; m < 5; ++m) {
+        // modes 0-2, then the exact adjoint's tile (the back-propagation's),
+        // then the one-step march of a two-step re-injection's odd last step
+        if (m == 4 && W.cfg[kReinject].march != 2) continue;
+        const LaunchCfg c = m < 3 ? W.cfg[m] : m == 3 ? adjoint_cfg(W.cfg[kBack]) : march2_tail_cfg(W.cfg[kReinject]);
+        const int order = m < 3 ? m : m == 3 ? kBack : kReinject;
         const std::vector<int> key = tiles_key(c, order);
         if (W.tiles.count(key)) continue;
         std::vector<int> tps(W.nshots);
         std::vector<std::vector<int>> cls(W.nshots);
+        if (c.march == 2) {
+            // per shot: row segments times groups of four 52-column strips;
+            // entry (shot, z0 << 16 | rows << 8 | up << 7 | group), odd
+            // segments walking up
+            std::vector<std::vector<int>> ys(W.nshots);
+            for (int s = 0; s < W.nshots; ++s) {
+                const ShotDev &S = W.host[s];
+                const int groups = ((S.n2e + kMarch2Cols - 1) / kMarch2Cols + 3) / 4;
+                if (groups > 128 || S.n1e >= 32768) return fail(DAW_ERR_ARG, "grid %d x %d too large for the column march", S.n1e, S.n2e);
+                const std::vector<int2> segs = march2_segments(S.n1e, c.tz, S.npml, S.ofs);
+                for (size_t i = 0; i < segs.size(); ++i)
+                    for (int q = 0; q < groups; ++q) {
+                        ys[s].push_back((segs[i].x << 16) | (segs[i].y << 8) | (int)(i & 1) << 7 | q);
+                        cls[s].push_back(0);
+                    }
+                tps[s] = (int)ys[s].size();
+            }
+            std::vector<int2> tl = make_tiles(tps, cls, kTileCostFirst);
+            for (int2 &e : tl) e.y = ys[e.x][e.y];
+            TileList &T = W.tiles[key];
+            T.n = (int)tl.size();
+            if (int rc = upload(T.d, tl)) return rc;
+            continue;
+        }
         if (c.march) {
             // per shot: row segments (the back-propagation's receiver rows in
             // short ones) times groups of four strips, strips fastest; entry
@@ -620,7 +676,19 @@ static void launch_sweep(int mode, SweepSet &W, const RecItem *d_rec, int nrec, 
             P.it = it;
             P.nsub = n;
             P.cur = cur;
-            if (c.march) launch_march(mode, P, c.tz, g.stream);
+            if (c.march == 2) {
+                if (n == 2) {
+                    launch_march2(P, g.stream);
+                } else {                                   // an odd last step
+                    const LaunchCfg c1 = march2_tail_cfg(c);
+                    const TileList &T1 = W.tiles.at(tiles_key(c1, kReinject));
+                    StepParams P1 = P;
+                    P1.tiles = T1.d.as<int2>();
+                    P1.ntiles = T1.n;
+                    launch_march(mode, P1, c1.tz, g.stream);
+                }
+            }
+            else if (c.march) launch_march(mode, P, c.tz, g.stream);
             else if (per == 2) launch_step2(mode, W.snap, P, c.tz, c.txw, g.stream);
             else launch_step(mode, W.snap, P, c.tz, c.txw, g.stream);
             cur ^= 1;
@@ -967,7 +1035,7 @@ static int autotune_tiles(Batch &B)
     static const int shapes[][4] = {{1, 8, 2, 0},  {1, 16, 1, 0}, {1, 16, 2, 0}, {1, 20, 1, 0}, {1, 24, 1, 0},
                                     {1, 32, 1, 0}, {1, 32, 2, 0}, {1, 64, 1, 0}, {2, 16, 1, 0}, {2, 32, 1, 0},
                                     {2, 16, 2, 0}, {1, 8, 0, 1},  {1, 12, 0, 1}, {1, 16, 0, 1}, {1, 24, 0, 1},
-                                    {1, 32, 0, 1}};
+                                    {1, 32, 0, 1}, {2, 24, 0, 2}, {2, 32, 0, 2}, {2, 48, 0, 2}};
     std::vector<LaunchCfg> cand;
     for (const auto &sh : shapes) {
         if (!g.steps_auto && sh[0] != g.steps) continue;
@@ -1076,7 +1144,8 @@ static int autotune_tiles(Batch &B)
                 if (log && med[k] < 1e29f)
                     fprintf(stderr, "[daw tune] %d shots %dx%d grid, %s: %d step(s)/launch, %s %dx%d  %.1f us/step\n",
                             B.nshots, B.geo_of(0).G.n1e, B.geo_of(0).G.n2e, names[mode], cand[k].steps,
-                            cand[k].march ? "march" : "tile", cand[k].tz, tile_cols(cand[k]), 1000.0f * med[k] / probe);
+                            cand[k].march == 2 ? "march2" : cand[k].march ? "march" : "tile", cand[k].tz, tile_cols(cand[k]),
+                            1000.0f * med[k] / probe);
                 best_ms = std::min(best_ms, med[k]);
             }
             // Among the candidates within 1.5 % of the fastest probe, the
@@ -1226,7 +1295,7 @@ static int finish_batch(Batch &B, bool want_snap)
     B.store.resize(B.nshots);
     B.W.nshots = B.nshots;
     B.W.snap = fmt < 0 ? (int)kSnapF32 : fmt;
-    for (int m = 0; m < 3; ++m) B.W.cfg[m] = fixed_cfg(m, B.W.snap);
+    for (int m = 0; m < 3; ++m) B.W.cfg[m] = g.last_cfg[m] = fixed_cfg(m, B.W.snap);   // the autotuner may replace them
     B.W.host.resize(B.nshots);
     B.W.inj_band.assign(B.nshots, make_int2(0, -1));
     B.roff.assign(B.nshots + 1, 0);
@@ -2321,6 +2390,16 @@ int daw_set_tile(int rows, int cols)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     if (rows == 0) { g.tz_auto = true; return 0; }
+    if (cols == kMarch2Cols) {
+        if (!march2_rows_ok(rows))
+            return fail(DAW_ERR_ARG, "two-step column march of %d rows: 16, 24, 32, 40 or 48", rows);
+        if (!g.steps_auto && g.steps == 1) return fail(DAW_ERR_ARG, "the two-step column march advances two steps per launch");
+        g.tz_auto = false;
+        g.tz = rows;
+        g.txw = 0;
+        g.march = 2;
+        return 0;
+    }
     if (cols == kMarchCols) {
         if (!march_rows_ok(rows))
             return fail(DAW_ERR_ARG, "column march of %d rows has no compiled kernel (8, 12, 16, 24, 32)", rows);
@@ -2347,8 +2426,10 @@ int daw_set_steps(int steps)
     std::lock_guard<std::mutex> lk(g.mu);
     if (steps == 0) { g.steps_auto = true; return 0; }
     if (steps != 1 && steps != 2) return fail(DAW_ERR_ARG, "steps per launch %d: 0 (autotuned), 1 or 2", steps);
-    if (steps == 2 && !g.tz_auto && (g.march || !tile2_shape_ok(g.tz, g.txw)))
+    if (steps == 2 && !g.tz_auto && (g.march == 1 || (!g.march && !tile2_shape_ok(g.tz, g.txw))))
         return fail(DAW_ERR_ARG, "tile %dx%d has no two-step kernel", g.tz, g.march ? kMarchCols : 64 * g.txw);
+    if (steps == 1 && !g.tz_auto && g.march == 2)
+        return fail(DAW_ERR_ARG, "the two-step column march advances two steps per launch");
     g.steps_auto = false;
     g.steps = steps;
     return 0;
@@ -2375,7 +2456,7 @@ int daw_get_tile(int *rows, int *cols)
 {
     std::lock_guard<std::mutex> lk(g.mu);
     if (rows) *rows = g.tz;
-    if (cols) *cols = g.march ? kMarchCols : 64 * g.txw;
+    if (cols) *cols = g.march == 2 ? kMarch2Cols : g.march ? kMarchCols : 64 * g.txw;
     return 0;
 }
 

@@ -20,6 +20,11 @@ bool march_rows_ok(int h);
 constexpr int kMarchCols = 58;
 // Rows of the column march's short segments (rows with many Hicks injection terms).
 constexpr int kMarchBandRows = 2;
+// Two-step column march of the re-injection (time steps it and it+1 in one
+// launch): wavefronts of 52 owned columns walking the rows of each tile
+// entry's segment; FP32 snapshots, nt > it+1.
+void launch_march2(const StepParams &P, hipStream_t st);
+constexpr int kMarch2Cols = 52;
 // Hicks record of the items of ngroups record groups at steps it0 ..
 // it0+nsteps-1 (nsteps <= kBandSlots) from the band's time slots, after
 // those steps ran.

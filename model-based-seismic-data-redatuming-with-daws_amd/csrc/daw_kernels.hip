@@ -1790,6 +1790,296 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
     else body(std::integral_constant<int, kMarchBandRows>{});                 // receiver rows
 }
 
+// ---------------------------------------------------------------------------
+// Two-step column march of the re-injection: steps it (A) and it+1 (B) in one
+// walk, so the wave state crosses HBM once per two steps.  A wavefront owns
+// kMarch2Cols = 52 columns x0 .. x0+51 of the rows z0 .. z0+H-1 (H = the
+// tile entry's rows); its lanes hold columns x0-6 .. x0+57: step A's pressure
+// is valid on lanes 2..62 and its velocities on 3..60, step B's pressure on
+// 5..59 and its velocities on 6..57, the owned lanes.  Iteration k loads vz
+// of row r0 (walking down r0 = z0-6+k) and the other fields of row r0-1, and
+// forms A's pressure of row r0-1, A's velocities of row r0-3, B's pressure of
+// row r0-4 and B's velocities of row r0-6 (walking up: r0 = z0+H+5-k and rows
+// r0+2, r0+3, r0+5, r0+6: each stencil takes its rows on the side it needs).
+// Step A's fields stay in register queues until step B has used them; only
+// B's state is stored, plus both steps' record band rows and wavefield.  The
+// operations of each point are those of k_march, in the same order, so the
+// results are bit-identical to two one-step launches.  The vz free-surface
+// mirror of step A (vz(npml-1) = vz(npml)) must reach B's pressure at row
+// npml-2 before it is formed, which a walk up guarantees, so a segment whose
+// A velocity rows reach rows npml-1 or npml walks up (and the host keeps a
+// segment's last A velocity row off npml-1, so such a segment holds both).  The loop runs four
+// iterations per trip (the queue shifts are renamings within a trip); what an
+// iteration needs of its rows is in lane k of the tables, its per-row
+// conditions in the bits of a flag word in lane k.
+// ---------------------------------------------------------------------------
+// per-row conditions of a pressure row (P*) and a velocity row (V*); B's rows
+// of iteration k are A's rows of iteration k-3, so one table serves both
+enum March2Flag : unsigned {
+    kP = 1u << 0, kPBox = 1u << 1, kPSrc = 1u << 2, kPFs = 1u << 3, kPOwn = 1u << 4, kPInt = 1u << 5, kPBand = 1u << 6,
+    kV = 1u << 7, kVBox = 1u << 8, kVInt = 1u << 9, kVFs0 = 1u << 10, kVFs1 = 1u << 11
+};
+
+__global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int2 t = P.tiles[blockIdx.x];
+    const ShotDev *__restrict__ S = P.shots + t.x;
+    const int it = P.it;
+    const int nt = S->nt;
+    if (it + 1 >= nt) return;                    // both steps exist (the host runs a last odd step on k_march)
+    const int n1e = S->n1e, n2e = S->n2e, ldx = S->ldx, npml = S->npml, n1 = S->n1, n2 = S->n2;
+    const int nstrip = (n2e + kMarch2Cols - 1) / kMarch2Cols;
+    const int z0 = t.y >> 16, H = (t.y >> 8) & 255, strip = 4 * (t.y & 127) + wv;
+    const int ofs = S->ofs;
+    const bool up = ((t.y >> 7) & 1) || (ofs == 1 && z0 - 3 <= npml && npml - 1 <= z0 + H + 2);
+    if (strip >= nstrip) return;                 // whole wavefront: no lane of it takes part in a shift
+    const int x0 = strip * kMarch2Cols;
+    const int gx = x0 - 6 + lane;
+    const bool own = lane >= 6 && lane < 6 + kMarch2Cols && gx < n2e;
+    const unsigned gxc = (unsigned)clampi(gx, 0, n2e - 1);
+    const int lo = 2, hiz = n1e - 4, hix = n2e - 4;
+    const bool colbox = gx >= lo && gx <= hix;
+    const bool colint = gx >= npml && gx < npml + n2;
+    const float dtdx = S->dtdx, mdt = -S->dt;
+
+    const unsigned pe = (unsigned)(n1e * ldx);
+    const Rsrc rs_st = make_rsrc(S->px[0]), rs_ka = make_rsrc(S->kappa);
+    const unsigned rd = P.cur ? 16 * pe : 0u, wr = P.cur ? 0u : 16 * pe;   // bytes
+    const unsigned ldx4 = 4u * (unsigned)ldx, pe4 = 4u * pe;
+    const unsigned vcol = 4u * gxc;
+    const int xlo1 = S->xlo1, xhi1 = S->xhi1, xlo2 = S->xlo2, xhi2 = S->xhi2;
+    const int spitch = S->spitch, sxb = S->sxb;
+    const unsigned sp4 = 4u * (unsigned)spitch;
+    const bool xcol = gx >= xlo2 && gx <= xhi2;
+    const unsigned sxs = 4u * (unsigned)(clampi(gx, xlo2, xhi2) - sxb);
+    const unsigned sxi = 4u * (unsigned)(clampi(gx, npml, npml + n2 - 1) - sxb);
+    const size_t plane = (size_t)n1 * spitch;
+    const size_t tA = (size_t)(nt - 1 - it) * plane, tB = (size_t)(nt - 2 - it) * plane;
+    const Rsrc rs_spA = make_rsrc(S->sp + tA), rs_sxA = make_rsrc(S->svx + tA), rs_szA = make_rsrc(S->svz + tA);
+    const Rsrc rs_spB = make_rsrc(S->sp + tB), rs_sxB = make_rsrc(S->svx + tB), rs_szB = make_rsrc(S->svz + tB);
+
+    const GLB float *const prof = glb(S->d1a);
+    const unsigned a1e = (unsigned)n1e;
+    const float c_d2a = ld(prof + 4 * a1e, gxc);
+    const float c_d2a1 = ld(prof + (4 * a1e + n2e), gxc);
+    const float c_d2b = ld(prof + (4 * a1e + 2 * n2e), gxc);
+    const float c_bw2 = ld(prof + (4 * a1e + 3 * n2e), gxc);
+    const GLB int *const band_row = glb(S->band_row);
+    const bool rec_band = band_row != nullptr;
+    const Rsrc rs_bandA = make_rsrc(S->pband + (size_t)(it & (kBandSlots - 1)) * S->band_stride);
+    const Rsrc rs_bandB = make_rsrc(S->pband + (size_t)((it + 1) & (kBandSlots - 1)) * S->band_stride);
+    GLB float *const wf = glb(S->wf);
+
+    // lane k: the rows of iteration k (A's pressure row ap, A's velocity row
+    // av; B's rows of iteration k are those of iteration k-3), their clamped
+    // byte offsets, profile values, band slots and condition flags
+    const int NK = H + 12;
+    const int r0_l = up ? z0 + H + 5 - lane : z0 - 6 + lane;
+    const int ap_l = up ? r0_l + 2 : r0_l - 1, av_l = up ? r0_l + 3 : r0_l - 3;
+    const unsigned apc = (unsigned)clampi(ap_l, 0, n1e - 1), avc = (unsigned)clampi(av_l, 0, n1e - 1);
+    const bool apin = ap_l >= 0 && ap_l < n1e, avin = av_l >= 0 && av_l < n1e;
+    const unsigned o_vz = rd + 3 * pe4 + (unsigned)clampi(r0_l, 0, n1e - 1) * ldx4;
+    const unsigned o_ap = apc * ldx4, o_av = avc * ldx4;
+    const unsigned o_sp = (unsigned)(clampi(ap_l, xlo1, xhi1) - npml) * sp4;
+    const unsigned o_sv = (unsigned)(clampi(av_l, npml, npml + n1 - 1) - npml) * sp4;
+    const float l_d1a = apin ? ld(prof, apc) : 1.0f, l_d1a1 = apin ? ld(prof + a1e, apc) : 1.0f;
+    const float l_d1b = avin ? ld(prof + 2 * a1e, avc) : 1.0f, l_bw1 = avin ? ld(prof + 3 * a1e, avc) : 0.0f;
+    const int l_band = rec_band && apin ? ld(band_row, apc) : -1;
+    auto fl = [](bool c, unsigned f) { return c ? f : 0u; };
+    const unsigned l_flags =
+        fl(ap_l < n1e, kP) | fl(ap_l >= lo && ap_l <= hiz, kPBox) | fl(ap_l >= xlo1 && ap_l <= xhi1, kPSrc) |
+        fl(ofs == 1 && ap_l <= npml, kPFs) | fl(ap_l >= z0 && ap_l < z0 + H, kPOwn) |
+        fl(ap_l >= npml && ap_l < npml + n1, kPInt) | fl(l_band >= 0, kPBand) |
+        fl(av_l < n1e, kV) | fl(av_l >= lo && av_l <= hiz, kVBox) | fl(av_l >= npml && av_l < npml + n1, kVInt) |
+        fl(ofs == 1 && av_l == npml - 1, kVFs0) | fl(ofs == 1 && av_l == npml, kVFs1);
+
+    const float a0 = 1.125f;
+    const float a1 = -(1.0f / 24.0f);
+    constexpr int D = 2;                         // rows of loads in flight ahead
+    constexpr int NS = 4;                        // load slots (k mod 4), one trip's iterations
+    float Lvz[NS] = {}, Lvx[NS] = {}, Lpx[NS] = {}, Lpz[NS] = {}, Lka[NS] = {};
+    float LspA[NS] = {}, LsxA[NS] = {}, LszA[NS] = {}, LspB[NS] = {}, LsxB[NS] = {}, LszB[NS] = {};
+    // every load of iteration k, unconditionally (rows outside the walk read
+    // a clamped in-grid row; nothing stored depends on them)
+    auto issue = [&](int s, int k) {
+        const int kk = k < NK ? k : NK - 1;
+        Lvz[s] = bld(rs_st, vcol, lane_u(o_vz, kk));
+        const unsigned r = lane_u(o_ap, kk);
+        Lvx[s] = bld(rs_st, vcol, rd + 2 * pe4 + r);
+        Lpx[s] = bld(rs_st, vcol, rd + r);
+        Lpz[s] = bld(rs_st, vcol, rd + pe4 + r);
+        Lka[s] = bld(rs_ka, vcol, r);
+        const int kb = kk >= 3 ? kk - 3 : 0;
+        LspA[s] = bld_nt(rs_spA, sxs, lane_u(o_sp, kk));
+        const unsigned ra = lane_u(o_sv, kk);
+        LsxA[s] = bld_nt(rs_sxA, sxi, ra);
+        LszA[s] = bld_nt(rs_szA, sxi, ra);
+        LspB[s] = bld_nt(rs_spB, sxs, lane_u(o_sp, kb));
+        const unsigned rb = lane_u(o_sv, kb);
+        LsxB[s] = bld_nt(rs_sxB, sxi, rb);
+        LszB[s] = bld_nt(rs_szB, sxi, rb);
+    };
+#pragma unroll
+    for (int k = 0; k < D; ++k) issue(k, k);
+
+    // queues, newest first: S0 vz (z), S0 vx of A's pressure rows (x), A's
+    // pressure (p), A's px, pz and kappa (ax, az, ka), A's velocities (bx,
+    // bz), B's pressure (q)
+    float z0q = 0.f, z1q = 0.f, z2q = 0.f, z3q = 0.f, x0q = 0.f, x1q = 0.f, x2q = 0.f;
+    float p0q = 0.f, p1q = 0.f, p2q = 0.f, p3q = 0.f;
+    float ax0 = 0.f, ax1 = 0.f, ax2 = 0.f, ax3 = 0.f, az0 = 0.f, az1 = 0.f, az2 = 0.f, az3 = 0.f;
+    float ka0 = 0.f, ka1 = 0.f, ka2 = 0.f, ka3 = 0.f;
+    float bx0 = 0.f, bx1 = 0.f, bx2 = 0.f, bx3 = 0.f, bz0 = 0.f, bz1 = 0.f, bz2 = 0.f, bz3 = 0.f;
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+
+    for (int k0 = 0; k0 < NK; k0 += NS) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = k0 + j;
+            const float r_vz = Lvz[j], r_vx = Lvx[j], r_px = Lpx[j], r_pz = Lpz[j], r_ka = Lka[j];
+            const float r_spA = LspA[j], r_sxA = LsxA[j], r_szA = LszA[j];
+            const float r_spB = LspB[j], r_sxB = LsxB[j], r_szB = LszB[j];
+            issue((j + D) % NS, k + D);
+            if (k >= NK) break;
+            const unsigned f = (unsigned)__builtin_amdgcn_readlane((int)l_flags, k);
+            const unsigned fb = (unsigned)__builtin_amdgcn_readlane((int)l_flags, k >= 3 ? k - 3 : 0);
+            z3q = z2q; z2q = z1q; z1q = z0q; z0q = r_vz;
+            // ---- step A, pressure (k >= 3)
+            if (k >= 3) {
+                float pxv = r_px, pzv = r_pz, p = 0.0f;
+                const float kap = r_ka;
+                if (f & kP) {
+                    const float vx = r_vx;
+                    const float vxm1 = from_left(vx), vxm2 = from_left(vxm1), vxp1 = from_right(vx);
+                    if (f & kPBox) {
+                        const float der2 = a0 * (vx - vxm1) + a1 * (vxp1 - vxm2);
+                        const float vz0 = up ? z2q : z1q, vzm1 = up ? z1q : z2q;
+                        const float vzp1 = up ? z3q : z0q, vzm2 = up ? z0q : z3q;
+                        const float der1 = a0 * (vz0 - vzm1) + a1 * (vzp1 - vzm2);
+                        const float tk = dtdx * kap;
+                        const float kw2 = tk * c_d2a1;
+                        const float kw1 = tk * lane_f(l_d1a1, k);
+                        const float d1a = lane_f(l_d1a, k);
+                        const float nx = c_d2a * pxv + kw2 * der2;
+                        const float nz = d1a * pzv + kw1 * der1;
+                        pxv = colbox ? nx : pxv;
+                        pzv = colbox ? nz : pzv;
+                    }
+                    if (f & kPSrc) pxv = xcol ? pxv + (mdt * r_spA) * kap : pxv;
+                    p = pxv + pzv;
+                    if (f & kPFs) p = 0.0f;
+                    if ((f & kPOwn) && own) {
+                        if ((f & kPBand) && rec_band) bst(rs_bandA, vcol, (unsigned)lane_i(l_band, k) * ldx4, p);
+                        if (wf != nullptr && (f & kPInt) && colint)
+                            (wf + (size_t)it * ((size_t)n1 * n2))[(gx - npml) * n1 + ((up ? z0 + H + 7 - k : z0 - 7 + k) - npml)] = p;
+                    }
+                }
+                p3q = p2q; p2q = p1q; p1q = p0q; p0q = p;
+                x2q = x1q; x1q = x0q; x0q = r_vx;
+                ax3 = ax2; ax2 = ax1; ax1 = ax0; ax0 = pxv;
+                az3 = az2; az2 = az1; az1 = az0; az0 = pzv;
+                ka3 = ka2; ka2 = ka1; ka1 = ka0; ka0 = kap;
+            }
+            // ---- step A, velocities (k >= 6)
+            if (k >= 6) {
+                float vxv = up ? x1q : x2q, vzv = z3q;
+                if (f & kV) {
+                    const float p0 = up ? p1q : p2q;
+                    const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
+                    if (f & kVBox) {
+                        const float der2 = a0 * (pr1 - p0) + a1 * (pr2 - pl1);
+                        const float pm1 = up ? p0q : p3q, pp1 = up ? p2q : p1q, pp2 = up ? p3q : p0q;
+                        const float der1 = a0 * (pp1 - p0) + a1 * (pp2 - pm1);
+                        const float d1b = lane_f(l_d1b, k), bw1 = lane_f(l_bw1, k);
+                        const float nx = c_d2b * vxv + c_bw2 * der2;
+                        const float nz = d1b * vzv + bw1 * der1;
+                        vxv = colbox ? nx : vxv;
+                        vzv = colbox ? nz : vzv;
+                    }
+                    if (f & kVInt) {
+                        vxv = colint ? vxv + mdt * r_sxA : vxv;
+                        vzv = colint ? vzv + mdt * r_szA : vzv;
+                    }
+                }
+                bx3 = bx2; bx2 = bx1; bx1 = bx0; bx0 = vxv;
+                bz3 = bz2; bz2 = bz1; bz1 = bz0; bz0 = vzv;
+                // vz(npml-1) = vz(npml) after step A; walking up, row npml
+                // came one iteration before row npml-1
+                if (f & kVFs0) bz0 = bz1;
+            }
+            // ---- step B, pressure (k >= 9)
+            if (k >= 9) {
+                float pxv = ax3, pzv = az3, p = 0.0f;
+                const float kap = ka3;
+                if (fb & kP) {
+                    const float vx = up ? bx2 : bx1;
+                    const float vxm1 = from_left(vx), vxm2 = from_left(vxm1), vxp1 = from_right(vx);
+                    if (fb & kPBox) {
+                        const float der2 = a0 * (vx - vxm1) + a1 * (vxp1 - vxm2);
+                        const float vz0 = up ? bz2 : bz1, vzm1 = up ? bz1 : bz2;
+                        const float vzp1 = up ? bz3 : bz0, vzm2 = up ? bz0 : bz3;
+                        const float der1 = a0 * (vz0 - vzm1) + a1 * (vzp1 - vzm2);
+                        const float tk = dtdx * kap;
+                        const float kw2 = tk * c_d2a1;
+                        const float kw1 = tk * lane_f(l_d1a1, k - 3);
+                        const float d1a = lane_f(l_d1a, k - 3);
+                        const float nx = c_d2a * pxv + kw2 * der2;
+                        const float nz = d1a * pzv + kw1 * der1;
+                        pxv = colbox ? nx : pxv;
+                        pzv = colbox ? nz : pzv;
+                    }
+                    if (fb & kPSrc) pxv = xcol ? pxv + (mdt * r_spB) * kap : pxv;
+                    p = pxv + pzv;
+                    if (fb & kPFs) p = 0.0f;
+                    if ((fb & kPOwn) && own) {
+                        const unsigned r = lane_u(o_ap, k - 3);
+                        bst(rs_st, vcol, wr + r, pxv);
+                        bst(rs_st, vcol, wr + pe4 + r, pzv);
+                        if ((fb & kPBand) && rec_band) bst(rs_bandB, vcol, (unsigned)lane_i(l_band, k - 3) * ldx4, p);
+                        if (wf != nullptr && (fb & kPInt) && colint)
+                            (wf + (size_t)(it + 1) * ((size_t)n1 * n2))[(gx - npml) * n1 + ((up ? z0 + H + 10 - k : z0 - 10 + k) - npml)] = p;
+                    }
+                }
+                q3 = q2; q2 = q1; q1 = q0; q0 = p;
+            }
+            // ---- step B, velocities (k >= 12)
+            if (k >= 12 && (fb & kV)) {
+                const float p0 = up ? q1 : q2;
+                const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
+                float vxv = bx3, vzv = bz3;
+                if (fb & kVBox) {
+                    const float der2 = a0 * (pr1 - p0) + a1 * (pr2 - pl1);
+                    const float pm1 = up ? q0 : q3, pp1 = up ? q2 : q1, pp2 = up ? q3 : q0;
+                    const float der1 = a0 * (pp1 - p0) + a1 * (pp2 - pm1);
+                    const float d1b = lane_f(l_d1b, k - 3), bw1 = lane_f(l_bw1, k - 3);
+                    const float nx = c_d2b * vxv + c_bw2 * der2;
+                    const float nz = d1b * vzv + bw1 * der1;
+                    vxv = colbox ? nx : vxv;
+                    vzv = colbox ? nz : vzv;
+                }
+                if (fb & kVInt) {
+                    vxv = colint ? vxv + mdt * r_sxB : vxv;
+                    vzv = colint ? vzv + mdt * r_szB : vzv;
+                }
+                if (own) {
+                    const unsigned r = lane_u(o_av, k - 3);
+                    bst(rs_st, vcol, wr + 2 * pe4 + r, vxv);
+                    if (!(fb & kVFs0)) {
+                        bst(rs_st, vcol, wr + 3 * pe4 + r, vzv);
+                        if (fb & kVFs1) bst(rs_st, vcol, wr + 3 * pe4 + r - ldx4, vzv);
+                    }
+                }
+            }
+        }
+    }
+}
+
+void launch_march2(const StepParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_march2_reinject, dim3(P.ntiles), dim3(kBlock), 0, st, P);
+}
+
 // Row-segment heights of the column march the autotuner may pick.
 #define DAW_MARCH_ROWS(X) X(8) X(12) X(16) X(24) X(32)
 

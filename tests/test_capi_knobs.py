@@ -35,6 +35,25 @@ def test_tile_and_steps_knobs(daw):
         L.daw_set_tile(0, 64)
 
 
+def test_two_step_march_knobs(daw):
+    L = daw.lib()
+    try:
+        for rows in (8, 20, 56, 64):
+            assert L.daw_set_tile(rows, 52) == -1 and b"two-step column march" in L.daw_last_error()
+        for rows in (16, 24, 32, 40, 48):
+            assert L.daw_set_tile(rows, 52) == 0
+            assert daw.get_tile() == (rows, 52)
+        assert L.daw_set_steps(1) == -1 and b"two steps per launch" in L.daw_last_error()
+        assert L.daw_set_steps(2) == 0
+        assert L.daw_set_tile(32, 58) == -1      # the one-step march with two steps per launch
+        assert L.daw_set_tile(0, 64) == 0
+        assert L.daw_set_steps(1) == 0
+        assert L.daw_set_tile(32, 52) == -1 and b"two steps per launch" in L.daw_last_error()
+    finally:
+        L.daw_set_steps(0)
+        L.daw_set_tile(0, 64)
+
+
 def test_get_launch_mode_range(daw):
     L = daw.lib()
     s, r, c = C.c_int(), C.c_int(), C.c_int()
