@@ -22,7 +22,7 @@ constexpr int kMarchCols = 58;
 constexpr int kMarchBandRows = 2;
 // Two-step column march of the re-injection (time steps it and it+1 in one
 // launch): wavefronts of 52 owned columns walking the rows of each tile
-// entry's segment; FP32 snapshots, nt > it+1.
+// entry's segment; FP32 snapshots.  A shot with it+1 == nt takes step it alone.
 void launch_march2(const StepParams &P, hipStream_t st);
 constexpr int kMarch2Cols = 52;
 // Hicks record of the items of ngroups record groups at steps it0 ..

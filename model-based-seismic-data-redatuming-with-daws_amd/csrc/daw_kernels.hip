@@ -1802,7 +1802,9 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
 // row r0-4 and B's velocities of row r0-6 (walking up: r0 = z0+H+5-k and rows
 // r0+2, r0+3, r0+5, r0+6: each stencil takes its rows on the side it needs).
 // Step A's fields stay in register queues until step B has used them; only
-// B's state is stored, plus both steps' record band rows and wavefield.  The
+// B's state is stored, plus both steps' record band rows and wavefield (a
+// shot whose last step is step A, in a batch of shots with different nt,
+// stores A's state from the same walk and skips B).  The
 // operations of each point are those of k_march, in the same order, so the
 // results are bit-identical to two one-step launches.  The vz free-surface
 // mirror of step A (vz(npml-1) = vz(npml)) must reach B's pressure at row
@@ -1817,7 +1819,7 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
 // of iteration k are A's rows of iteration k-3, so one table serves both
 enum March2Flag : unsigned {
     kP = 1u << 0, kPBox = 1u << 1, kPSrc = 1u << 2, kPFs = 1u << 3, kPOwn = 1u << 4, kPInt = 1u << 5, kPBand = 1u << 6,
-    kV = 1u << 7, kVBox = 1u << 8, kVInt = 1u << 9, kVFs0 = 1u << 10, kVFs1 = 1u << 11
+    kV = 1u << 7, kVBox = 1u << 8, kVInt = 1u << 9, kVFs0 = 1u << 10, kVFs1 = 1u << 11, kVOwn = 1u << 12
 };
 
 __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
@@ -1827,7 +1829,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
     const ShotDev *__restrict__ S = P.shots + t.x;
     const int it = P.it;
     const int nt = S->nt;
-    if (it + 1 >= nt) return;                    // both steps exist (the host runs a last odd step on k_march)
+    if (it >= nt) return;
+    // the shot's last step when its nt is odd (a batch's shots may differ in
+    // nt): step A alone, stored from A's rows as the one-step march stores
+    const bool two = it + 1 < nt;
     const int n1e = S->n1e, n2e = S->n2e, ldx = S->ldx, npml = S->npml, n1 = S->n1, n2 = S->n2;
     const int nstrip = (n2e + kMarch2Cols - 1) / kMarch2Cols;
     const int z0 = t.y >> 16, H = (t.y >> 8) & 255, strip = 4 * (t.y & 127) + wv;
@@ -1855,7 +1860,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
     const unsigned sxs = 4u * (unsigned)(clampi(gx, xlo2, xhi2) - sxb);
     const unsigned sxi = 4u * (unsigned)(clampi(gx, npml, npml + n2 - 1) - sxb);
     const size_t plane = (size_t)n1 * spitch;
-    const size_t tA = (size_t)(nt - 1 - it) * plane, tB = (size_t)(nt - 2 - it) * plane;
+    // snapshot planes of the two steps; for a lone step A, B's loads (issued
+    // unconditionally, their values unused) read A's plane, never a plane
+    // before the shot's store
+    const size_t tA = (size_t)(nt - 1 - it) * plane, tB = two ? (size_t)(nt - 2 - it) * plane : tA;
     const Rsrc rs_spA = make_rsrc(S->sp + tA), rs_sxA = make_rsrc(S->svx + tA), rs_szA = make_rsrc(S->svz + tA);
     const Rsrc rs_spB = make_rsrc(S->sp + tB), rs_sxB = make_rsrc(S->svx + tB), rs_szB = make_rsrc(S->svz + tB);
 
@@ -1892,7 +1900,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
         fl(ofs == 1 && ap_l <= npml, kPFs) | fl(ap_l >= z0 && ap_l < z0 + H, kPOwn) |
         fl(ap_l >= npml && ap_l < npml + n1, kPInt) | fl(l_band >= 0, kPBand) |
         fl(av_l < n1e, kV) | fl(av_l >= lo && av_l <= hiz, kVBox) | fl(av_l >= npml && av_l < npml + n1, kVInt) |
-        fl(ofs == 1 && av_l == npml - 1, kVFs0) | fl(ofs == 1 && av_l == npml, kVFs1);
+        fl(ofs == 1 && av_l == npml - 1, kVFs0) | fl(ofs == 1 && av_l == npml, kVFs1) |
+        fl(av_l >= z0 && av_l < z0 + H, kVOwn);
 
     const float a0 = 1.125f;
     const float a1 = -(1.0f / 24.0f);
@@ -1970,6 +1979,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                     p = pxv + pzv;
                     if (f & kPFs) p = 0.0f;
                     if ((f & kPOwn) && own) {
+                        if (!two) {
+                            const unsigned r = lane_u(o_ap, k);
+                            bst(rs_st, vcol, wr + r, pxv);
+                            bst(rs_st, vcol, wr + pe4 + r, pzv);
+                        }
                         if ((f & kPBand) && rec_band) bst(rs_bandA, vcol, (unsigned)lane_i(l_band, k) * ldx4, p);
                         if (wf != nullptr && (f & kPInt) && colint)
                             (wf + (size_t)it * ((size_t)n1 * n2))[(gx - npml) * n1 + ((up ? z0 + H + 7 - k : z0 - 7 + k) - npml)] = p;
@@ -2001,6 +2015,14 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                         vxv = colint ? vxv + mdt * r_sxA : vxv;
                         vzv = colint ? vzv + mdt * r_szA : vzv;
                     }
+                    if (!two && (f & kVOwn) && own) {
+                        const unsigned r = lane_u(o_av, k);
+                        bst(rs_st, vcol, wr + 2 * pe4 + r, vxv);
+                        if (!(f & kVFs0)) {
+                            bst(rs_st, vcol, wr + 3 * pe4 + r, vzv);
+                            if (f & kVFs1) bst(rs_st, vcol, wr + 3 * pe4 + r - ldx4, vzv);
+                        }
+                    }
                 }
                 bx3 = bx2; bx2 = bx1; bx1 = bx0; bx0 = vxv;
                 bz3 = bz2; bz2 = bz1; bz1 = bz0; bz0 = vzv;
@@ -2009,7 +2031,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                 if (f & kVFs0) bz0 = bz1;
             }
             // ---- step B, pressure (k >= 9)
-            if (k >= 9) {
+            if (two && k >= 9) {
                 float pxv = ax3, pzv = az3, p = 0.0f;
                 const float kap = ka3;
                 if (fb & kP) {
@@ -2044,7 +2066,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                 q3 = q2; q2 = q1; q1 = q0; q0 = p;
             }
             // ---- step B, velocities (k >= 12)
-            if (k >= 12 && (fb & kV)) {
+            if (two && k >= 12 && (fb & kV)) {
                 const float p0 = up ? q1 : q2;
                 const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
                 float vxv = bx3, vzv = bz3;

@@ -105,3 +105,43 @@ def test_uneven_of_even_shots_equals_even_batch(gpu_daw, method):
             x, hist, info = O.linearminres(c.vp0, c.npml, c.h, 1, c.nt, c.dt, dobs_np[s][:, ::-1].copy(), c.rec, mu,
                                            its, prec="f64")
             assert rel_l2(x1[s].cpu().numpy(), x) <= 1e-4, s
+
+
+@pytest.mark.parametrize("tile", [(24, 52), (0, 64)], ids=["two-step-march", "autotuned"])
+def test_uneven_odd_nt_shots_alone_equal_batch(gpu_daw, tile):
+    """Shots of odd and even nt in one uneven batch give, bit for bit, what
+    each gives alone: with the two-step march of the re-injection, a shot
+    whose nt is odd takes its last step alone inside a two-step launch."""
+    import torch
+    c = Case(nz=45, nx=150, nt=241)
+    x0, n2s, _, src, recs, dats = _shots(c)
+    nts = np.array([201, 240, 219], np.int32)
+    mu, its = 1e-2, 2
+    dobs_np = [O.forward(c.vp[a:a + w], c.npml, c.h, 1, int(nt), c.dt, c.s[:nt], src[:, k].copy(), recs[k])
+               for k, (a, w, nt) in enumerate(zip(x0, n2s, nts))]
+    s_dev = torch.from_numpy(c.s).cuda()
+
+    def run(ks):
+        dobs = torch.from_numpy(np.concatenate([dobs_np[k].ravel() for k in ks])).cuda()
+        nd = sum(dats[k].shape[1] * int(nts[k]) for k in ks)
+        datum = torch.zeros(nd, dtype=torch.float32, device="cuda")
+        dde = torch.zeros(dobs.numel(), dtype=torch.float32, device="cuda")
+        gpu_daw.redatum_uneven(c.vp0, c.npml, c.h, 1, c.dt, s_dev, x0[ks], n2s[ks], nts[ks],
+                               np.ascontiguousarray(src[:, ks]), [recs[k] for k in ks], dobs, mu, its,
+                               dat_tabs=[dats[k] for k in ks], datum_dev=datum, deltade_dev=dde)
+        return datum.cpu().numpy(), dde.cpu().numpy()
+
+    gpu_daw.set_tile(*tile)
+    try:
+        print("uneven odd nt", tile, flush=True)
+        datum, dde = run([0, 1, 2])
+        ro = do = 0
+        for k in range(3):
+            nr, ndk, nt = recs[k].shape[1], dats[k].shape[1], int(nts[k])
+            d1, x1 = run([k])
+            assert np.array_equal(dde[ro:ro + nr * nt], x1), (tile, k)
+            assert np.array_equal(datum[do:do + ndk * nt], d1), (tile, k)
+            ro += nr * nt
+            do += ndk * nt
+    finally:
+        gpu_daw.set_tile(0)
