@@ -468,17 +468,28 @@ static LaunchCfg fixed_cfg(int mode, int snap)
     return march_ok(c, mode, snap) ? c : LaunchCfg{1, 16, 1, 0, g.rec_batch};
 }
 
-// Row segments of the two-step march: h rows each; with the free surface a
-// segment whose step-A velocity rows (z0-3 .. z0+h+2) would end at row
-// npml-1 is one row shorter, so that the segment holding row npml-1 also
-// holds row npml (the kernel's vz mirror needs both).
+// Row segments of the two-step march: about n1e/h segments of balanced
+// heights near h (the kernel takes a segment's height from its tile entry,
+// up to 52), so no segment is a sliver (241 rows at h = 24: ten of 24-25
+// rows, not eleven with a last one of one row).  With the free surface a
+// segment whose step-A velocity rows (z0-3 .. z0+rows+2) would end at row
+// npml-1 gives its last row to the next segment, so that the segment holding
+// row npml-1 also holds row npml (the kernel's vz mirror needs both).
 static std::vector<int2> march2_segments(int n1e, int h, int npml, int ofs)
 {
+    int nseg = std::max(1, (n1e + h / 2) / h);
+    while ((n1e + nseg - 1) / nseg > 51) ++nseg;
+    const int base = n1e / nseg, rem = n1e % nseg;
+    std::vector<int> rows(nseg);
+    for (int i = 0; i < nseg; ++i) rows[i] = base + (i < rem ? 1 : 0);
     std::vector<int2> segs;
-    for (int z = 0; z < n1e;) {
-        const int r = (ofs == 1 && z + h + 2 == npml - 1) ? h - 1 : h;
-        segs.push_back(make_int2(z, r));
-        z += r;
+    for (size_t i = 0, z = 0; i < rows.size(); z += rows[i], ++i) {
+        if (ofs == 1 && (int)z + rows[i] + 2 == npml - 1 && rows[i] > 1) {
+            --rows[i];
+            if (i + 1 < rows.size()) ++rows[i + 1];
+            else rows.push_back(1);
+        }
+        segs.push_back(make_int2((int)z, rows[i]));
     }
     return segs;
 }
