@@ -809,7 +809,10 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<MODE, TZ, TXW, SNAP>())) vo
     const float c_d2a = colok ? S.fd2a[gxl] : 1.0f, c_d2a1 = colok ? S.fd2a1[gxl] : 1.0f;
     __syncthreads();
 
-    // ---- 2. pressure half-step
+    // ---- 2. pressure half-step; a tile whose pressure rows (z0-1 .. z0+TZ+1)
+    // miss the rows of the Hicks windows looks up no injection entry
+    const bool injt = MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj && S.sd->inj_z0 <= z0 + TZ + 1 &&
+                      S.sd->inj_z1 >= z0 - 1;
 #pragma unroll
     for (int j = 0; j < PROWS; ++j) {
         const int r = rz + NWZ * j;
@@ -817,7 +820,7 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<MODE, TZ, TXW, SNAP>())) vo
             const int gz = z0 - 1 + r;
             float p = 0.0f;
             if (gz >= 0 && gz < n1e && colok) {
-                const int injq = (MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj) ? glb(S.sd->inj_row)[gz] : -1;
+                const int injq = injt ? glb(S.sd->inj_row)[gz] : -1;
                 p = p_point<MODE, VZW_>(S, it, ic, gz, gxl, pxr[j], pzr[j], kar[j], &s_vx[r * VXW_ + cl + 3],
                                         &s_vz[(r + 2) * VZW_ + cl + 1], S.fd1a[gz], S.fd1a1[gz], c_d2a, c_d2a1,
                                         injq, r >= 1 && r <= TZ, lo, hiz, hix, spr[j], pxn, pzn);
@@ -834,7 +837,7 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<MODE, TZ, TXW, SNAP>())) vo
     if (side) {
         float p = 0.0f;
         if (s_gz < n1e && s_gx >= 0 && s_gx < n2e) {
-            const int injq = (MODE != kReinject && S.sd->inj_row != nullptr && !S.sd->vinj) ? glb(S.sd->inj_row)[s_gz] : -1;
+            const int injq = injt ? glb(S.sd->inj_row)[s_gz] : -1;
             p = p_point<MODE, VZW_>(S, it, ic, s_gz, s_gx, s_px, s_pz, s_ka, &s_vx[s_r * VXW_ + s_c + 2],
                                     &s_vz[(s_r + 2) * VZW_ + s_c], S.fd1a[s_gz], S.fd1a1[s_gz], S.fd2a[s_gx],
                                     S.fd2a1[s_gx], injq, false, lo, hiz, hix, s_sp, pxn, pzn);
