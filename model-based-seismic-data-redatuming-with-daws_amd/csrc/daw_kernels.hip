@@ -359,9 +359,13 @@ __device__ __forceinline__ float inject_terms(float acc, float kap, int kb, int 
     return acc;
 }
 
-// Waves per SIMD the register budget must allow: short segments make many
-// wavefronts (201x801 with 24 shots: 7560 at 12 rows), tall ones few; the
-// sweeps with Hicks injection terms hold four more entries in flight.
+// Pressure half-step of one point of the general tile body: the split update
+// inside the stencil box (submodeling2.f90:152-153 forward, :1018-1019
+// back-propagation, :1112-1113 re-injection), the Hicks injection terms of the
+// point's CSR entries (injq = the row's slot, -1 for none), the re-injection's
+// extended source, p = px + pz (:183, :1051, :1127) with the free-surface zero,
+// and for owned points the stores of px, pz, the record band slot and the
+// wavefield.
 template <int MODE, int VZW_>
 __device__ __forceinline__ float p_point(const View &S, int it, int ic, int gz, int gx, float pxv, float pzv,
                                          float kap, const float *vr, const float *vc, float d1a, float d1a1,
@@ -1813,10 +1817,10 @@ __global__ __launch_bounds__(kBlock, (march_min_waves<MODE, H>())) void k_march(
 // mirror of step A (vz(npml-1) = vz(npml)) must reach B's pressure at row
 // npml-2 before it is formed, which a walk up guarantees, so a segment whose
 // A velocity rows reach rows npml-1 or npml walks up (and the host keeps a
-// segment's last A velocity row off npml-1, so such a segment holds both).  The loop runs four
-// iterations per trip (the queue shifts are renamings within a trip); what an
-// iteration needs of its rows is in lane k of the tables, its per-row
-// conditions in the bits of a flag word in lane k.
+// segment's last A velocity row off npml-1, so such a segment holds both).
+// The loop runs four iterations per trip (the queue shifts are renamings
+// within a trip); what an iteration needs of its rows is in lane k of the
+// tables, its per-row conditions in the bits of a flag word in lane k.
 // ---------------------------------------------------------------------------
 // per-row conditions of a pressure row (P*) and a velocity row (V*); B's rows
 // of iteration k are A's rows of iteration k-3, so one table serves both
