@@ -115,3 +115,48 @@ def test_modeling_batch_deep_datum_interior_tiles(tiles):
             osis, owf = O.forward(c.vp, c.npml, c.h, 1, c.nt, c.dt, c.s, src[:, s].copy(), c.rec, want_wavefield=True)
             assert np.array_equal(sis[s], osis)
             assert np.array_equal(dat[s], O.resample(owf, c.npml, c.dat, 1))
+
+
+def _ring_case(tz, edge, **kw):
+    """A case whose 21-row Hicks window (of the source and of every receiver)
+    ends exactly on a ring row of one tz-row tile: edge "top" puts the
+    window's last row on the top ring row z0-1 of the tile at z0, "bottom" its
+    first row on the bottom ring row z0+tz+1.  That tile's own rows hold no
+    other window row, so it must still apply the terms of its ring row.  The
+    offset of +0.37 / -0.38 cell from the nearest node gives that edge row a
+    nonzero coefficient (it is exactly zero on the other side)."""
+    npml, h = 12, 10.0
+    if edge == "top":
+        z0 = -(-71 // tz) * tz
+        zrec, frac = z0 - 11, 0.37
+    else:
+        z0 = max(0, -(-(49 - tz) // tz) * tz)
+        zrec, frac = z0 + tz + 11, -0.38
+    depth = (zrec - npml + frac) * h
+    c = _case(src_z=depth, rec_z=depth, **kw)
+    assert c.npml == npml and float(c.h) == h
+    assert int(c.rec[0, 0]) - 1 == zrec and int(c.src[0]) - 1 == zrec
+    t = np.asarray(O.hicks2d_tile(int(c.rec[2, 0]), int(c.rec[3, 0]), int(c.rec[4, 0]), c.ofs))
+    assert np.abs(t[:, 20 if edge == "top" else 0]).max() > 0      # axis 1 is the z offset -10..10
+    ring = z0 - 1 if edge == "top" else z0 + tz + 1
+    assert ring == (zrec + 10 if edge == "top" else zrec - 10)
+    return c
+
+
+@pytest.mark.parametrize("edge", ["top", "bottom"])
+def test_injection_window_edge_on_tile_ring(tiles, edge):
+    """Tiles whose pressure rows (z0-1 .. z0+tz+1) miss every Hicks window
+    row skip the injection lookup; a window that ends on a tile's ring row
+    must still reach that tile, in the forward sweep (source) and in the
+    back-propagation of the residuals (receivers)."""
+    for tile in TILES:
+        c = _ring_case(tile[0], edge)
+        ref = O.forward(c.vp, c.npml, c.h, 1, c.nt, c.dt, c.s, c.src, c.rec)
+        res = c.rng.standard_normal((c.rec.shape[1], c.nt)).astype(np.float32)
+        ref_h = O.esource_data(c.vp0, c.npml, c.h, 1, c.nt, c.dt, res, c.rec)
+        tiles.set_tile(*tile)
+        print("injection window edge on tile ring", edge, tile, flush=True)
+        sis = tiles.submodeling(c.vp, c.npml, c.h, 1, c.nt, c.dt, c.s, c.src, c.rec)
+        assert np.array_equal(sis, ref), (tile, rel_l2(sis, ref))
+        a = tiles.submodeling_adjoint_esource_data(c.vp0, c.npml, c.h, 1, c.nt, c.dt, res, c.rec)
+        assert np.array_equal(a, ref_h), (tile, rel_l2(a, ref_h))
