@@ -345,13 +345,16 @@ static int make_band(Band &B, const std::vector<const PointSet *> &sets, const G
 // b % 8, and each XCD gets one contiguous run of tiles so that halo re-reads
 // of neighbouring tiles hit its L2).  Within each XCD's run, kTileRaster
 // keeps raster order; kTileCostFirst puts the tiles in decreasing cost class
-// cls[s][t] (2 = sponge or boundary tile, 1 = interior tile, 0 = tile mostly
-// past the grid), raster order inside a class.  Measured at 201x801 with 24
-// shots per launch: back-propagation 52.8-54.6 us per launch cost-first
-// against 65.7 raster (16x64 tiles), 55.1 against 57.9 (20x64); re-injection,
-// whose interior tiles read the snapshots from HBM, is slower cost-first
-// (59.8-61.2 against 57.1 us: its HBM reads then bunch up in the second half
-// of the launch) and the forward sweep gains nothing.
+// cls[s][t], raster order inside a class.  The back-propagation's classes are
+// 2 = tile whose pressure rows meet the rows of the receivers' Hicks windows
+// (the receiver terms' dependent loads make these the longest tiles), 1 =
+// any other tile, 0 = tile mostly past the grid.  Measured at 201x801 with
+// 24 shots per launch, 16x64 tiles, one process: 41.6 us per launch with the
+// band tiles first, 42.4 with the sponge and boundary tiles first as well,
+// 44.5 in raster order.  Re-injection, whose interior tiles read the
+// snapshots from HBM, is slower cost-first (59.8-61.2 against 57.1 us on an
+// earlier kernel: its HBM reads then bunch up in the second half of the
+// launch) and the forward sweep gains nothing.
 enum TileOrder { kTileRaster = 0, kTileCostFirst = 1 };
 static int tile_order(int mode) { return mode == kBack ? kTileCostFirst : kTileRaster; }
 
@@ -631,12 +634,13 @@ static int finalize_sweepset(SweepSet &W)
             const int ntx = (S.n2e + txt - 1) / txt;
             tps[s] = ((S.n1e + c.tz - 1) / c.tz) * ntx;
             cls[s].resize(tps[s]);
+            // first and last row of the receivers' windows (none: z1 < z0)
+            const int2 b = order == kBack && s < (int)W.inj_band.size() ? W.inj_band[s] : make_int2(0, -1);
             for (int t = 0; t < tps[s]; ++t) {
                 const int z0 = (t / ntx) * c.tz, x0 = (t % ntx) * txt;
                 const int rows = std::min(c.tz, S.n1e - z0), cols = std::min(txt, S.n2e - x0);
-                const bool inner = z0 - 1 >= S.npml && z0 + c.tz + 1 < S.npml + S.n1 && x0 - 1 >= S.npml &&
-                                   x0 + txt + 1 < S.npml + S.n2;
-                cls[s][t] = (2 * rows < c.tz || 2 * cols < txt) ? 0 : inner ? 1 : 2;
+                const bool band = b.x <= z0 + c.tz + 1 && b.y >= z0 - 1;     // pressure rows z0-1 .. z0+tz+1
+                cls[s][t] = (2 * rows < c.tz || 2 * cols < txt) ? 0 : band ? 2 : 1;
             }
         }
         std::vector<int2> tl = make_tiles(tps, cls, tile_order(order));
