@@ -1957,12 +1957,21 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
             const float r_spA = LspA[j], r_sxA = LsxA[j], r_szA = LszA[j];
             const float r_spB = LspB[j], r_sxB = LsxB[j], r_szB = LszB[j];
             issue((j + D) % NS, k + D);
-            if (k >= NK) break;
+            // the walk is the kernel's last work: ending here, rather than
+            // leaving the trip, keeps the trip's end the loop's only back-edge
+            if (k >= NK) return;
             const unsigned f = (unsigned)__builtin_amdgcn_readlane((int)l_flags, k);
             const unsigned fb = (unsigned)__builtin_amdgcn_readlane((int)l_flags, k >= 3 ? k - 3 : 0);
             z3q = z2q; z2q = z1q; z1q = z0q; z0q = r_vz;
-            // ---- step A, pressure (k >= 3)
-            if (k >= 3) {
+            // The four phases run at every iteration, so that every queue
+            // shifts once per iteration and a trip's shifts are renamings.
+            // A phase's results are valid from iteration 3 (A pressure), 6
+            // (A velocities), 9 (B pressure) and 12 (B velocities): each
+            // reads queue entries of its last four iterations only, so values
+            // formed earlier leave the queues unread, and each store is kept
+            // to owned rows, which no iteration before those reaches.
+            // ---- step A, pressure
+            {
                 float pxv = r_px, pzv = r_pz, p = 0.0f;
                 const float kap = r_ka;
                 if (f & kP) {
@@ -2002,8 +2011,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                 az3 = az2; az2 = az1; az1 = az0; az0 = pzv;
                 ka3 = ka2; ka2 = ka1; ka1 = ka0; ka0 = kap;
             }
-            // ---- step A, velocities (k >= 6)
-            if (k >= 6) {
+            // ---- step A, velocities
+            {
                 float vxv = up ? x1q : x2q, vzv = z3q;
                 if (f & kV) {
                     const float p0 = up ? p1q : p2q;
@@ -2037,8 +2046,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                 // came one iteration before row npml-1
                 if (f & kVFs0) bz0 = bz1;
             }
-            // ---- step B, pressure (k >= 9)
-            if (two && k >= 9) {
+            // ---- step B, pressure (stored when the shot takes step B)
+            {
                 float pxv = ax3, pzv = az3, p = 0.0f;
                 const float kap = ka3;
                 if (fb & kP) {
@@ -2061,7 +2070,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                     if (fb & kPSrc) pxv = xcol ? pxv + (mdt * r_spB) * kap : pxv;
                     p = pxv + pzv;
                     if (fb & kPFs) p = 0.0f;
-                    if ((fb & kPOwn) && own) {
+                    if (two && (fb & kPOwn) && own) {
                         const unsigned r = lane_u(o_ap, k - 3);
                         bst(rs_st, vcol, wr + r, pxv);
                         bst(rs_st, vcol, wr + pe4 + r, pzv);
@@ -2072,8 +2081,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                 }
                 q3 = q2; q2 = q1; q1 = q0; q0 = p;
             }
-            // ---- step B, velocities (k >= 12)
-            if (two && k >= 12 && (fb & kV)) {
+            // ---- step B, velocities (B's velocity rows are the owned rows
+            // from iteration 12 on)
+            if (fb & kV) {
                 const float p0 = up ? q1 : q2;
                 const float pr1 = from_right(p0), pr2 = from_right(pr1), pl1 = from_left(p0);
                 float vxv = bx3, vzv = bz3;
@@ -2091,7 +2101,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_march2_reinject(StepParams P)
                     vxv = colint ? vxv + mdt * r_sxB : vxv;
                     vzv = colint ? vzv + mdt * r_szB : vzv;
                 }
-                if (own) {
+                if (two && (fb & kVOwn) && own) {
                     const unsigned r = lane_u(o_av, k - 3);
                     bst(rs_st, vcol, wr + 2 * pe4 + r, vxv);
                     if (!(fb & kVFs0)) {
