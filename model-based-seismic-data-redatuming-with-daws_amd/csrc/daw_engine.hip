@@ -1050,7 +1050,8 @@ static int autotune_tiles(Batch &B)
     static const int shapes[][4] = {{1, 8, 2, 0},  {1, 16, 1, 0}, {1, 16, 2, 0}, {1, 20, 1, 0}, {1, 24, 1, 0},
                                     {1, 32, 1, 0}, {1, 32, 2, 0}, {1, 64, 1, 0}, {2, 16, 1, 0}, {2, 32, 1, 0},
                                     {2, 16, 2, 0}, {1, 8, 0, 1},  {1, 12, 0, 1}, {1, 16, 0, 1}, {1, 24, 0, 1},
-                                    {1, 32, 0, 1}, {2, 24, 0, 2}, {2, 32, 0, 2}, {2, 48, 0, 2}};
+                                    {1, 32, 0, 1}, {2, 16, 0, 2}, {2, 24, 0, 2}, {2, 32, 0, 2},
+                                    {2, 48, 0, 2}};
     std::vector<LaunchCfg> cand;
     for (const auto &sh : shapes) {
         if (!g.steps_auto && sh[0] != g.steps) continue;
